@@ -1,0 +1,222 @@
+/*
+ * pnjl_hip.h -- C ABI of libpnjl_hip.so, the MI355X (gfx950) backend of the PNJL
+ * equilibrium-scan hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an int
+ * status (PNJL_OK = 0) and never throws or longjmps across the boundary.  The last
+ * error message of a context is available from pnjl_last_error().
+ *
+ * Pointer convention: unless a call says otherwise, array arguments are HOST
+ * pointers owned by the caller; the library stages them to HBM, runs, and copies
+ * back before returning.  With PNJL_FLAG_DEVICE_PTRS every array argument of
+ * that call (inputs and the output struct's arrays) is instead a device pointer
+ * into HBM of the context's device (e.g. a torch.cuda tensor's data_ptr()); the
+ * call then launches on `stream` (NULL = the null stream) and synchronises it
+ * before returning unless PNJL_FLAG_ASYNC is also given.
+ *
+ * Units: fm^-1 for T, mu, masses and momenta (MeV / 197.327), as in the
+ * reference (src/Constants_PNJL.jl:80-97).
+ */
+#ifndef PNJL_HIP_H
+#define PNJL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* status codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define PNJL_OK               0
+#define PNJL_ERR_ARG          1   /* bad argument (size, policy, NULL pointer) */
+#define PNJL_ERR_HIP          2   /* HIP runtime error (message in pnjl_last_error) */
+#define PNJL_ERR_NO_DEVICE    3   /* no gfx950 device visible */
+#define PNJL_ERR_CHANNEL      4   /* polarization channel other than P/S */
+#define PNJL_ERR_STATE        5   /* call order (e.g. surrogate used before trained) */
+
+/* flags */
+#define PNJL_FLAG_DEVICE_PTRS 0x1
+#define PNJL_FLAG_ASYNC       0x2
+
+/* ------------------------------------------------------------------------ */
+/* model parameters                                                          */
+/* replaces: src/Constants_PNJL.jl:17-103 + config/pnjl/default.toml:1-21    */
+/* ------------------------------------------------------------------------ */
+typedef struct pnjl_model_params {
+    double hbarc;          /* MeV fm                          (197.327)        */
+    double Lambda;         /* cutoff, fm^-1                   (602.3/hbarc)     */
+    double G;              /* fm^2                            (1.835/Lambda^2)  */
+    double K;              /* fm^5                            (12.36/Lambda^5)  */
+    double m_ud0;          /* fm^-1                           (5.5/hbarc)       */
+    double m_s0;           /* fm^-1                           (140.7/hbarc)     */
+    double T0;             /* fm^-1                           (210/hbarc)       */
+    double a0, a1, a2, b3; /* Polyakov potential             (3.51,-2.47,15.2,-1.75) */
+    double rho0;           /* fm^-3                           (0.16)            */
+    int    N_color;        /* 3                                                  */
+} pnjl_model_params;
+
+/* Fills the default profile (config/pnjl/default.toml). */
+void pnjl_default_params(pnjl_model_params* out);
+
+/* ------------------------------------------------------------------------ */
+/* context                                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct pnjl_ctx pnjl_ctx;
+
+/* Creates a context on HIP device `device`.  params == NULL -> defaults.   */
+int  pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out);
+void pnjl_ctx_destroy(pnjl_ctx* ctx);
+const char* pnjl_last_error(const pnjl_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* quadrature nodes                                                          */
+/* replaces: cached_nodes(p_num, t_num)  src/pnjl/core/Integrals.jl:87-109   */
+/* p nodes = GL on [0,10] fm^-1, cos(theta) nodes = GL on [0,1], weights x2. */
+/* Writes the column-major p_num x t_num meshes (index j + l*p_num) when the */
+/* output pointers are non-NULL (host pointers).                            */
+/* ------------------------------------------------------------------------ */
+int pnjl_nodes(pnjl_ctx* ctx, int p_num, int t_num,
+               double* p_mesh, double* cos_mesh, double* coef_mesh);
+
+/* Gauss-Legendre nodes/weights on [a,b], ascending.
+ * replaces: gauleg  src/integration/GaussLegendre.jl:94-119                 */
+int pnjl_gauleg(double a, double b, int n, double* nodes, double* weights);
+
+/* ------------------------------------------------------------------------ */
+/* gap solver                                                                */
+/* replaces: PNJL.solve(::FixedMu, T_fm, mu_fm; xi, seed_strategy, p_num,     */
+/*           t_num, iterations, trust_region_fallback,                      */
+/*           auto_multiseed_fallback, residual_norm_max)                     */
+/*           src/pnjl/solver/ImplicitSolver.jl:211-328, solve_multi :532-559 */
+/* ------------------------------------------------------------------------ */
+
+/* seed policies (src/pnjl/solver/SeedStrategies.jl) */
+#define PNJL_SEED_DEFAULT        0  /* DefaultSeed(:auto)            :200-225 */
+#define PNJL_SEED_HADRON         1  /* DefaultSeed(phase_hint=:hadron)         */
+#define PNJL_SEED_QUARK          2  /* DefaultSeed(phase_hint=:quark)          */
+#define PNJL_SEED_MULTI          3  /* MultiSeed() -> solve_multi   :257-284 */
+#define PNJL_SEED_EXPLICIT       4  /* per-point seed (ContinuitySeed with a
+                                       previous solution)            :313-325 */
+#define PNJL_SEED_CONTINUITY     5  /* ContinuitySeed chains        :303-345 */
+#define PNJL_SEED_PHASE_AWARE    6  /* PhaseAwareContinuitySeed chains with the
+                                       driver's MultiSeed first point
+                                       (scripts/relaxtime/run_gap_transport_scan.jl:410-442) */
+#define PNJL_SEED_SURROGATE      7  /* learned seed; plain seeds as fallback  */
+#define PNJL_SEED_MULTIROOT3     8  /* surrogate + HADRON + QUARK, min Omega  */
+
+typedef struct pnjl_solve_opts {
+    double xtol;                  /* NLsolve xtol (reference 1e-9)           */
+    double ftol;                  /* NLsolve ftol (reference 1e-9)           */
+    double residual_norm_max;     /* "converged" gate (reference 1e-6)       */
+    double phi_tol;               /* physicality tolerance (1e-8)            */
+    int    max_iter;              /* NLsolve `iterations` (1000; driver 40)  */
+    int    trust_region_fallback; /* 1                                       */
+    int    auto_multiseed_fallback;/* 1                                      */
+    int    surrogate_max_steps;   /* Newton steps before a learned seed is
+                                     abandoned (8)                           */
+    int    chain_len;             /* CONTINUITY / PHASE_AWARE: points per
+                                     chain (consecutive in the batch)       */
+    int    reserved;
+} pnjl_solve_opts;
+
+void pnjl_default_solve_opts(pnjl_solve_opts* out);
+
+/* Structure-of-arrays result; each array holds n entries (x, M, n_q, n_qbar:
+ * n*5 / n*3, point-major).  Maps 1:1 onto SolverResult
+ * (src/pnjl/solver/ImplicitSolver.jl:178-193) plus the number densities of
+ * calculate_number_densities (src/pnjl/core/Thermodynamics.jl:255-281).     */
+typedef struct pnjl_solve_out {
+    double*  x;          /* n*5: phi_u, phi_d, phi_s, Phi, Phibar            */
+    double*  omega;      /* n                                                */
+    double*  pressure;   /* n                                                */
+    double*  rho_norm;   /* n                                                */
+    double*  entropy;    /* n                                                */
+    double*  energy;     /* n                                                */
+    double*  masses;     /* n*3                                              */
+    double*  n_q;        /* n*3  quark number densities  (may be NULL)       */
+    double*  n_qbar;     /* n*3  antiquark densities     (may be NULL)       */
+    double*  residual_norm; /* n                                             */
+    int32_t* iterations; /* n                                                */
+    int32_t* converged;  /* n   reference `converged` (res <= residual_norm_max,
+                                f-converged, physical)                       */
+    int32_t* flags;      /* n   PNJL_FLAG_* bits below (may be NULL)         */
+    int32_t* newton_steps; /* n total Newton/TR steps spent on the point,
+                              all attempts (may be NULL)                     */
+    int32_t* passes;     /* n   flavour node passes spent on the point (each
+                                covers p_num x t_eff nodes; t_eff = 1 at
+                                xi = 0, else t_num) (may be NULL)            */
+} pnjl_solve_out;
+
+/* per-point flag bits */
+#define PNJL_PT_TR_USED        0x01  /* trust-region candidate chosen          */
+#define PNJL_PT_MULTISEED      0x02  /* automatic MultiSeed result returned    */
+#define PNJL_PT_EXCEPTION      0x04  /* non-finite residual at the seed (the
+                                        reference throws IsFiniteException)   */
+#define PNJL_PT_NO_ROOT        0x08  /* MultiSeed: no seed converged (reference
+                                        throws ErrorException)                */
+#define PNJL_PT_SURR_FALLBACK  0x10  /* learned seed abandoned -> plain seeds  */
+#define PNJL_PT_SURR_OMEGA     0x20  /* learned root lost the 3-seed Omega check */
+#define PNJL_PT_FIRST_ORDER    0x40  /* >= 2 distinct physical roots (MULTIROOT3) */
+#define PNJL_PT_CHAIN_RESET    0x80  /* chain: phase switch re-seeded the point */
+
+/* Solves n FixedMu points in one persistent-kernel launch.
+ * T, mu, xi: n values each.  seed: n*5 (EXPLICIT) or NULL.
+ * p_num/t_num: quadrature sizes (reference defaults 64 and 8).             */
+int pnjl_solve_fixedmu_batch(pnjl_ctx* ctx, int64_t n,
+                             const double* T, const double* mu, const double* xi,
+                             const double* seed, int seed_policy,
+                             int p_num, int t_num,
+                             const pnjl_solve_opts* opts,
+                             const pnjl_solve_out* out,
+                             int flags, void* stream);
+
+/* Residual F = grad_x P (5), Jacobian J = dF/dx (5x5, row-major) and Omega at
+ * given states -- the building block of the Newton loop.
+ * replaces: gap_conditions  src/pnjl/solver/Conditions.jl:72-81 and the
+ *           ForwardDiff Jacobian NLsolve builds from it.                    */
+int pnjl_gap_residual_batch(pnjl_ctx* ctx, int64_t n,
+                            const double* x, const double* T, const double* mu,
+                            const double* xi, int p_num, int t_num,
+                            double* F, double* J, double* omega,
+                            int flags, void* stream);
+
+/* Phase-boundary tables for PNJL_SEED_PHASE_AWARE chains: rows
+ * (xi, T_MeV, mu_MeV) of boundary.csv and (xi, T_CEP_MeV, mu_CEP_MeV) of
+ * cep.csv (host pointers).  For a chain of a given xi the rows with
+ * |xi_row - xi| <= 1e-6 are used, sorted by T, with the first matching CEP.
+ * replaces: load_phase_boundary / interpolate_mu_c / _get_current_phase
+ *           src/pnjl/solver/SeedStrategies.jl:388-475, 762-782           */
+int pnjl_set_phase_boundary(pnjl_ctx* ctx, int nrows, const double* xi, const double* T_MeV,
+                            const double* mu_MeV, int ncep, const double* cep_xi,
+                            const double* cep_T_MeV, const double* cep_mu_MeV);
+
+/* Launch geometry of the persistent scan kernel; chosen per config by the
+ * autotuner (pnjl_autotune) or set explicitly.                             */
+typedef struct pnjl_launch_cfg {
+    int lanes_per_point;  /* 1,2,4,8,16,32,64 -> 64/lanes points per wavefront */
+    int chunk;            /* points taken from the work queue per pop        */
+    int waves_per_cu;     /* persistent grid size = 256 CUs x waves_per_cu   */
+    int reserved;
+} pnjl_launch_cfg;
+
+int pnjl_set_launch_cfg(pnjl_ctx* ctx, const pnjl_launch_cfg* cfg);
+int pnjl_get_launch_cfg(const pnjl_ctx* ctx, pnjl_launch_cfg* cfg);
+
+/* Autotuner: times the candidate launch geometries (lanes_per_point x chunk x
+ * waves_per_cu) on the first `sample` points of the batch (host pointers as
+ * in pnjl_solve_fixedmu_batch), keeps the fastest in the context and returns
+ * it in *best (and its time per point in *ns_per_point).                   */
+int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* mu, const double* xi,
+                  const double* seed, int seed_policy, int p_num, int t_num,
+                  const pnjl_solve_opts* opts, pnjl_launch_cfg* best, double* ns_per_point);
+
+/* Kernel time of the last solve/residual launch in milliseconds (HIP events
+ * recorded around the kernel on the call's stream).                        */
+double pnjl_last_kernel_ms(const pnjl_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PNJL_HIP_H */

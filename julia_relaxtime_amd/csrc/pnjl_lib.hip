@@ -1,0 +1,661 @@
+// pnjl_lib.hip -- host side of libpnjl_hip.so: the extern "C" ABI declared in
+// include/pnjl_hip.h.  Owns the device buffers, the node tables, the work
+// queue counter, the per-group scratch, the phase-boundary tables and the
+// launch geometry of a context; launches the kernels of pnjl_scan_kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pnjl_hip.h"
+#include "pnjl_scan_kernels.h"
+
+struct pnjl_ctx {
+    int device = 0;
+    int num_cu = 256;
+    pnjl_model_params params{};
+    pnjl::DevParams dp{};
+    std::string err;
+    pnjl_launch_cfg cfg{4, 4, 8, 0};
+    std::map<std::pair<int, int>, double*> tabs;
+    double* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    unsigned long long* queue = nullptr;
+    double* bnd = nullptr;
+    int nb = 0;
+    std::vector<void*> stage;   // device staging buffers for host-pointer calls
+    std::vector<size_t> stage_bytes;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+int set_err(pnjl_ctx* c, int code, const std::string& m)
+{
+    if (c) c->err = m;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_err(ctx, PNJL_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// n-point Gauss-Legendre rule on [-1,1], ascending (Newton iteration on the
+// Legendre three-term recurrence from the Tricomi-type initial guesses).
+void gl_rule(int n, std::vector<double>& x, std::vector<double>& w)
+{
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < (n + 1) / 2; i++) {
+        double z = cos(pi * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; k++) {
+                double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) { p0 = 1.0; p1 = z; }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            double dz = p1 / dp;
+            z -= dz;
+            if (fabs(dz) <= 1e-17 * (1.0 + fabs(z)) && it > 2) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int k = 2; k <= n; k++) {
+            double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        if (n == 1) { p0 = 1.0; p1 = z; }
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        double wi = 2.0 / ((1.0 - z * z) * dp * dp);
+        x[n - 1 - i] = z;
+        x[i] = -z;
+        w[n - 1 - i] = wi;
+        w[i] = wi;
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+void gauleg_host(double a, double b, int n, std::vector<double>& x, std::vector<double>& w)
+{
+    gl_rule(n, x, w);
+    double sc = (b - a) / 2.0, sh = (b + a) / 2.0;
+    for (int i = 0; i < n; i++) {
+        x[i] = sc * x[i] + sh;
+        w[i] = sc * w[i];
+    }
+}
+
+// meshes of cached_nodes (Integrals.jl:87-97)
+void build_meshes(int pn, int tn, std::vector<double>& p, std::vector<double>& c, std::vector<double>& C)
+{
+    std::vector<double> px, pw, tx, tw;
+    gauleg_host(0.0, 10.0, pn, px, pw);
+    gauleg_host(0.0, 1.0, tn, tx, tw);
+    const double pi = 3.14159265358979323846;
+    double twopi2 = (2.0 * pi) * (2.0 * pi);
+    p.resize((size_t)pn * tn);
+    c.resize((size_t)pn * tn);
+    C.resize((size_t)pn * tn);
+    for (int l = 0; l < tn; l++)
+        for (int j = 0; j < pn; j++) {
+            size_t k = (size_t)j + (size_t)l * pn;
+            p[k] = px[j];
+            c[k] = tx[l];
+            C[k] = (pw[j] * (tw[l] * 2.0)) * (px[j] * px[j]) / twopi2;
+        }
+}
+
+int get_tab(pnjl_ctx* ctx, int pn, int tn, double** out)
+{
+    auto key = std::make_pair(pn, tn);
+    auto it = ctx->tabs.find(key);
+    if (it != ctx->tabs.end()) {
+        *out = it->second;
+        return PNJL_OK;
+    }
+    std::vector<double> p, c, C;
+    build_meshes(pn, tn, p, c, C);
+    // device table: pp[pn] | cc[tn] | C[pn*tn] | Cs[pn]
+    std::vector<double> tab((size_t)pn + tn + (size_t)pn * tn + pn);
+    for (int j = 0; j < pn; j++) tab[j] = p[j] * p[j];
+    for (int l = 0; l < tn; l++) tab[pn + l] = c[(size_t)l * pn] * c[(size_t)l * pn];
+    for (size_t k = 0; k < (size_t)pn * tn; k++) tab[pn + tn + k] = C[k];
+    for (int j = 0; j < pn; j++) {
+        double s = 0.0;
+        for (int l = 0; l < tn; l++) s += C[(size_t)j + (size_t)l * pn];
+        tab[pn + tn + (size_t)pn * tn + j] = s;
+    }
+    double* d = nullptr;
+    HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(double)));
+    HIPCHK(ctx, hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    ctx->tabs[key] = d;
+    *out = d;
+    return PNJL_OK;
+}
+
+int ensure_scratch(pnjl_ctx* ctx, size_t bytes)
+{
+    if (ctx->scratch_bytes >= bytes) return PNJL_OK;
+    if (ctx->scratch) HIPCHK(ctx, hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->scratch, bytes));
+    ctx->scratch_bytes = bytes;
+    return PNJL_OK;
+}
+
+// staging slot k of at least `bytes`
+int stage_buf(pnjl_ctx* ctx, size_t k, size_t bytes, void** out)
+{
+    if (ctx->stage.size() <= k) {
+        ctx->stage.resize(k + 1, nullptr);
+        ctx->stage_bytes.resize(k + 1, 0);
+    }
+    if (ctx->stage_bytes[k] < bytes) {
+        if (ctx->stage[k]) HIPCHK(ctx, hipFree(ctx->stage[k]));
+        ctx->stage[k] = nullptr;
+        ctx->stage_bytes[k] = 0;
+        HIPCHK(ctx, hipMalloc(&ctx->stage[k], bytes));
+        ctx->stage_bytes[k] = bytes;
+    }
+    *out = ctx->stage[k];
+    return PNJL_OK;
+}
+
+bool valid_lpp(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64; }
+
+void launch_scan(const pnjl::ScanArgs& A, int lpp, int blocks, size_t lds, hipStream_t st)
+{
+    using namespace pnjl;
+    switch (lpp) {
+    case 1: hipLaunchKernelGGL(scan_kernel<1>, dim3(blocks), dim3(256), lds, st, A); break;
+    case 2: hipLaunchKernelGGL(scan_kernel<2>, dim3(blocks), dim3(256), lds, st, A); break;
+    case 4: hipLaunchKernelGGL(scan_kernel<4>, dim3(blocks), dim3(256), lds, st, A); break;
+    case 8: hipLaunchKernelGGL(scan_kernel<8>, dim3(blocks), dim3(256), lds, st, A); break;
+    case 16: hipLaunchKernelGGL(scan_kernel<16>, dim3(blocks), dim3(256), lds, st, A); break;
+    case 32: hipLaunchKernelGGL(scan_kernel<32>, dim3(blocks), dim3(256), lds, st, A); break;
+    default: hipLaunchKernelGGL(scan_kernel<64>, dim3(blocks), dim3(256), lds, st, A); break;
+    }
+}
+
+int check_policy(int p)
+{
+    return (p >= PNJL_SEED_DEFAULT && p <= PNJL_SEED_SURROGATE) ? PNJL_OK : PNJL_ERR_ARG;
+}
+
+// core of pnjl_solve_fixedmu_batch with device pointers
+int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const double* xi, const double* seed,
+              int policy, int pn, int tn, const pnjl_solve_opts* o, const pnjl_solve_out* out,
+              const pnjl_launch_cfg& cfg, hipStream_t stream)
+{
+    double* tab = nullptr;
+    int rc = get_tab(ctx, pn, tn, &tab);
+    if (rc) return rc;
+    pnjl::ScanArgs A{};
+    A.P = ctx->dp;
+    A.n_points = n;
+    A.policy = policy;
+    bool chains = (policy == PNJL_SEED_CONTINUITY || policy == PNJL_SEED_PHASE_AWARE);
+    A.chain_len = chains ? (o->chain_len > 0 ? o->chain_len : (int)std::min<int64_t>(n, 1 << 30)) : 1;
+    A.n_items = chains ? (n + A.chain_len - 1) / A.chain_len : n;
+    A.T = T;
+    A.mu = mu;
+    A.xi = xi;
+    A.seed = seed;
+    A.x_out = out->x;
+    A.omega = out->omega;
+    A.pressure = out->pressure;
+    A.rho_norm = out->rho_norm;
+    A.entropy = out->entropy;
+    A.energy = out->energy;
+    A.masses = out->masses;
+    A.n_q = out->n_q;
+    A.n_qbar = out->n_qbar;
+    A.resnorm = out->residual_norm;
+    A.iterations = out->iterations;
+    A.converged = out->converged;
+    A.flags = out->flags;
+    A.steps = out->newton_steps;
+    A.passes = out->passes;
+    A.tab = tab;
+    A.pn = pn;
+    A.tn = tn;
+    A.xtol = o->xtol;
+    A.ftol = o->ftol;
+    A.resmax = o->residual_norm_max;
+    A.phi_tol = o->phi_tol;
+    A.max_iter = o->max_iter;
+    A.tr_fb = o->trust_region_fallback;
+    A.auto_ms = o->auto_multiseed_fallback;
+    A.surr_cap = o->surrogate_max_steps;
+    A.queue = ctx->queue;
+    A.chunk = std::max(1, cfg.chunk);
+    A.bnd = ctx->bnd;
+    A.nb = ctx->nb;
+    A.max_trips = 9LL * (2LL * o->max_iter + 4) + 16;
+    int blocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / 4);
+    int64_t groups = (int64_t)blocks * 256 / cfg.lanes_per_point;
+    // never more groups than work items (fewer blocks for small batches)
+    int64_t need_groups = std::max<int64_t>(1, (A.n_items + A.chunk - 1) / A.chunk);
+    if (groups > need_groups) {
+        int64_t gpb = 256 / cfg.lanes_per_point;
+        blocks = (int)std::max<int64_t>(1, (need_groups + gpb - 1) / gpb);
+        groups = (int64_t)blocks * gpb;
+    }
+    A.n_groups = groups;
+    rc = ensure_scratch(ctx, (size_t)groups * pnjl::SCRATCH_FIELDS * sizeof(double));
+    if (rc) return rc;
+    A.scratch = ctx->scratch;
+    size_t lds = sizeof(double) * ((size_t)pn + tn + (size_t)pn * tn + pn);
+    hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(1), 0, stream, ctx->queue);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
+    launch_scan(A, cfg.lanes_per_point, blocks, lds, stream);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, stream));
+    return PNJL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pnjl_default_params(pnjl_model_params* o)
+{
+    o->hbarc = 197.327;
+    o->Lambda = 602.3 / o->hbarc;
+    double L2 = o->Lambda * o->Lambda;
+    o->G = 1.835 / L2;
+    o->K = 12.36 / (L2 * L2 * o->Lambda);
+    o->m_ud0 = 5.5 / o->hbarc;
+    o->m_s0 = 140.7 / o->hbarc;
+    o->T0 = 210.0 / o->hbarc;
+    o->a0 = 3.51;
+    o->a1 = -2.47;
+    o->a2 = 15.2;
+    o->b3 = -1.75;
+    o->rho0 = 0.16;
+    o->N_color = 3;
+}
+
+void pnjl_default_solve_opts(pnjl_solve_opts* o)
+{
+    o->xtol = 1e-9;
+    o->ftol = 1e-9;
+    o->residual_norm_max = 1e-6;
+    o->phi_tol = 1e-8;
+    o->max_iter = 1000;
+    o->trust_region_fallback = 1;
+    o->auto_multiseed_fallback = 1;
+    o->surrogate_max_steps = 8;
+    o->chain_len = 0;
+    o->reserved = 0;
+}
+
+int pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out)
+{
+    if (!out) return PNJL_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PNJL_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return PNJL_ERR_ARG;
+    pnjl_ctx* c = new pnjl_ctx();
+    c->device = device;
+    if (params)
+        c->params = *params;
+    else
+        pnjl_default_params(&c->params);
+    const pnjl_model_params& p = c->params;
+    c->dp = pnjl::DevParams{p.Lambda, p.G, p.K, p.m_ud0, p.m_s0, p.T0, p.a0, p.a1, p.a2, p.b3, p.rho0,
+                            (double)p.N_color};
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return PNJL_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cu = prop.multiProcessorCount;
+    if (hipMalloc(&c->queue, 64) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return PNJL_ERR_HIP;
+    }
+    *out = c;
+    return PNJL_OK;
+}
+
+void pnjl_ctx_destroy(pnjl_ctx* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipDeviceSynchronize();
+    for (auto& kv : c->tabs) hipFree(kv.second);
+    for (void* b : c->stage)
+        if (b) hipFree(b);
+    if (c->scratch) hipFree(c->scratch);
+    if (c->queue) hipFree(c->queue);
+    if (c->bnd) hipFree(c->bnd);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    delete c;
+}
+
+const char* pnjl_last_error(const pnjl_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+double pnjl_last_kernel_ms(const pnjl_ctx* c) { return c ? c->last_ms : 0.0; }
+
+int pnjl_gauleg(double a, double b, int n, double* nodes, double* weights)
+{
+    if (n <= 0 || !(a < b) || !nodes || !weights) return PNJL_ERR_ARG;
+    std::vector<double> x, w;
+    gauleg_host(a, b, n, x, w);
+    memcpy(nodes, x.data(), sizeof(double) * n);
+    memcpy(weights, w.data(), sizeof(double) * n);
+    return PNJL_OK;
+}
+
+int pnjl_nodes(pnjl_ctx* ctx, int p_num, int t_num, double* p_mesh, double* cos_mesh, double* coef_mesh)
+{
+    if (p_num <= 0 || t_num <= 0 || p_num > 4096 || t_num > 64 || (int64_t)p_num * t_num > 4096)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_nodes: p_num/t_num out of range");
+    std::vector<double> p, c, C;
+    build_meshes(p_num, t_num, p, c, C);
+    if (p_mesh) memcpy(p_mesh, p.data(), sizeof(double) * p.size());
+    if (cos_mesh) memcpy(cos_mesh, c.data(), sizeof(double) * c.size());
+    if (coef_mesh) memcpy(coef_mesh, C.data(), sizeof(double) * C.size());
+    if (ctx) {
+        double* d;
+        HIPCHK(ctx, hipSetDevice(ctx->device));
+        return get_tab(ctx, p_num, t_num, &d);
+    }
+    return PNJL_OK;
+}
+
+int pnjl_set_phase_boundary(pnjl_ctx* ctx, int nrows, const double* xi, const double* T, const double* mu,
+                            int ncep, const double* cxi, const double* cT, const double* cmu)
+{
+    if (!ctx || nrows < 0 || ncep < 0) return PNJL_ERR_ARG;
+    using pnjl::kBndStride;
+    using pnjl::kMaxBndRows;
+    std::vector<double> xs;
+    for (int i = 0; i < nrows; i++)
+        if (std::find(xs.begin(), xs.end(), xi[i]) == xs.end()) xs.push_back(xi[i]);
+    std::vector<double> h(xs.size() * kBndStride, 0.0);
+    for (size_t k = 0; k < xs.size(); k++) {
+        double* s = &h[k * kBndStride];
+        std::vector<std::pair<double, double>> rows;
+        for (int i = 0; i < nrows; i++)
+            if (xi[i] == xs[k]) rows.push_back({T[i], mu[i]});
+        std::stable_sort(rows.begin(), rows.end(),
+                         [](const std::pair<double, double>& a, const std::pair<double, double>& b) {
+                             return a.first < b.first;
+                         });
+        if ((int)rows.size() > kMaxBndRows)
+            return set_err(ctx, PNJL_ERR_ARG, "pnjl_set_phase_boundary: too many rows for one xi");
+        s[0] = xs[k];
+        s[1] = NAN;
+        for (int c = 0; c < ncep; c++)
+            if (fabs(cxi[c] - xs[k]) <= 1e-6) {
+                s[1] = cT[c];
+                break;
+            }
+        s[2] = (double)rows.size();
+        for (size_t i = 0; i < rows.size(); i++) {
+            s[4 + i] = rows[i].first;
+            s[4 + kMaxBndRows + i] = rows[i].second;
+        }
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (ctx->bnd) HIPCHK(ctx, hipFree(ctx->bnd));
+    ctx->bnd = nullptr;
+    ctx->nb = 0;
+    if (!h.empty()) {
+        HIPCHK(ctx, hipMalloc(&ctx->bnd, h.size() * sizeof(double)));
+        HIPCHK(ctx, hipMemcpy(ctx->bnd, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        ctx->nb = (int)xs.size();
+    }
+    return PNJL_OK;
+}
+
+int pnjl_set_launch_cfg(pnjl_ctx* ctx, const pnjl_launch_cfg* cfg)
+{
+    if (!ctx || !cfg || !valid_lpp(cfg->lanes_per_point) || cfg->chunk < 1 || cfg->waves_per_cu < 1 ||
+        cfg->waves_per_cu > 64)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_set_launch_cfg: invalid geometry");
+    ctx->cfg = *cfg;
+    return PNJL_OK;
+}
+
+int pnjl_get_launch_cfg(const pnjl_ctx* ctx, pnjl_launch_cfg* cfg)
+{
+    if (!ctx || !cfg) return PNJL_ERR_ARG;
+    *cfg = ctx->cfg;
+    return PNJL_OK;
+}
+
+static int solve_impl(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const double* xi,
+                      const double* seed, int policy, int pn, int tn, const pnjl_solve_opts* opts,
+                      const pnjl_solve_out* out, int flags, void* stream, const pnjl_launch_cfg& cfg)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || !T || !mu || !xi || !out || check_policy(policy))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_solve_fixedmu_batch: bad argument");
+    if ((policy == PNJL_SEED_EXPLICIT || policy == PNJL_SEED_SURROGATE) && !seed)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_solve_fixedmu_batch: this policy needs seeds");
+    if (!out->x || !out->omega || !out->pressure || !out->rho_norm || !out->entropy || !out->energy ||
+        !out->masses || !out->residual_norm || !out->iterations || !out->converged)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_solve_fixedmu_batch: missing output array");
+    if (pn <= 0 || tn <= 0 || (int64_t)pn * tn > 4096)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_solve_fixedmu_batch: p_num/t_num out of range");
+    pnjl_solve_opts o;
+    if (opts)
+        o = *opts;
+    else
+        pnjl_default_solve_opts(&o);
+    if (o.max_iter < 0 || o.max_iter > 100000) return set_err(ctx, PNJL_ERR_ARG, "max_iter out of range");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if (flags & PNJL_FLAG_DEVICE_PTRS) {
+        rc = solve_dev(ctx, n, T, mu, xi, seed, policy, pn, tn, &o, out, cfg, st);
+        if (rc) return rc;
+    } else {
+        // stage host arrays into HBM
+        size_t nd = (size_t)n * sizeof(double);
+        void* b[19];
+        size_t sizes[] = {nd, nd, nd, 5 * nd, 5 * nd, nd, nd, nd, nd, nd, 3 * nd, 3 * nd, 3 * nd, nd,
+                          (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4};
+        for (int k = 0; k < 19; k++) {
+            rc = stage_buf(ctx, k, sizes[k], &b[k]);
+            if (rc) return rc;
+        }
+        HIPCHK(ctx, hipMemcpyAsync(b[0], T, nd, hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(b[1], mu, nd, hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(b[2], xi, nd, hipMemcpyHostToDevice, st));
+        if (seed) HIPCHK(ctx, hipMemcpyAsync(b[3], seed, 5 * nd, hipMemcpyHostToDevice, st));
+        pnjl_solve_out d;
+        d.x = (double*)b[4];
+        d.omega = (double*)b[5];
+        d.pressure = (double*)b[6];
+        d.rho_norm = (double*)b[7];
+        d.entropy = (double*)b[8];
+        d.energy = (double*)b[9];
+        d.masses = (double*)b[10];
+        d.n_q = out->n_q ? (double*)b[11] : nullptr;
+        d.n_qbar = out->n_qbar ? (double*)b[12] : nullptr;
+        d.residual_norm = (double*)b[13];
+        d.iterations = (int32_t*)b[14];
+        d.converged = (int32_t*)b[15];
+        d.flags = out->flags ? (int32_t*)b[16] : nullptr;
+        d.newton_steps = out->newton_steps ? (int32_t*)b[17] : nullptr;
+        d.passes = out->passes ? (int32_t*)b[18] : nullptr;
+        rc = solve_dev(ctx, n, (double*)b[0], (double*)b[1], (double*)b[2], seed ? (double*)b[3] : nullptr, policy,
+                       pn, tn, &o, &d, cfg, st);
+        if (rc) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(out->x, d.x, 5 * nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->omega, d.omega, nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->pressure, d.pressure, nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->rho_norm, d.rho_norm, nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->entropy, d.entropy, nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->energy, d.energy, nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->masses, d.masses, 3 * nd, hipMemcpyDeviceToHost, st));
+        if (out->n_q) HIPCHK(ctx, hipMemcpyAsync(out->n_q, d.n_q, 3 * nd, hipMemcpyDeviceToHost, st));
+        if (out->n_qbar) HIPCHK(ctx, hipMemcpyAsync(out->n_qbar, d.n_qbar, 3 * nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->residual_norm, d.residual_norm, nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->iterations, d.iterations, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(out->converged, d.converged, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        if (out->flags) HIPCHK(ctx, hipMemcpyAsync(out->flags, d.flags, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        if (out->newton_steps)
+            HIPCHK(ctx, hipMemcpyAsync(out->newton_steps, d.newton_steps, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        if (out->passes) HIPCHK(ctx, hipMemcpyAsync(out->passes, d.passes, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    }
+    if (!(flags & PNJL_FLAG_ASYNC) || !(flags & PNJL_FLAG_DEVICE_PTRS)) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    }
+    return PNJL_OK;
+}
+
+int pnjl_solve_fixedmu_batch(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const double* xi,
+                             const double* seed, int seed_policy, int p_num, int t_num, const pnjl_solve_opts* opts,
+                             const pnjl_solve_out* out, int flags, void* stream)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    return solve_impl(ctx, n, T, mu, xi, seed, seed_policy, p_num, t_num, opts, out, flags, stream, ctx->cfg);
+}
+
+int pnjl_gap_residual_batch(pnjl_ctx* ctx, int64_t n, const double* x, const double* T, const double* mu,
+                            const double* xi, int pn, int tn, double* F, double* J, double* omega, int flags,
+                            void* stream)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || !x || !T || !mu || !xi || !F || !J || !omega || pn <= 0 || tn <= 0 || (int64_t)pn * tn > 4096)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_gap_residual_batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    double* tab = nullptr;
+    int rc = get_tab(ctx, pn, tn, &tab);
+    if (rc) return rc;
+    const double *dx = x, *dT = T, *dmu = mu, *dxi = xi;
+    double *dF = F, *dJ = J, *dom = omega;
+    size_t nd = (size_t)n * sizeof(double);
+    if (!(flags & PNJL_FLAG_DEVICE_PTRS)) {
+        void* b[7];
+        size_t sizes[] = {5 * nd, nd, nd, nd, 5 * nd, 25 * nd, nd};
+        for (int k = 0; k < 7; k++) {
+            rc = stage_buf(ctx, 20 + k, sizes[k], &b[k]);
+            if (rc) return rc;
+        }
+        HIPCHK(ctx, hipMemcpyAsync(b[0], x, 5 * nd, hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(b[1], T, nd, hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(b[2], mu, nd, hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(b[3], xi, nd, hipMemcpyHostToDevice, st));
+        dx = (double*)b[0];
+        dT = (double*)b[1];
+        dmu = (double*)b[2];
+        dxi = (double*)b[3];
+        dF = (double*)b[4];
+        dJ = (double*)b[5];
+        dom = (double*)b[6];
+    }
+    pnjl::ResArgs R{};
+    R.P = ctx->dp;
+    R.n = n;
+    R.x = dx;
+    R.T = dT;
+    R.mu = dmu;
+    R.xi = dxi;
+    R.F = dF;
+    R.J = dJ;
+    R.omega = dom;
+    R.tab = tab;
+    R.pn = pn;
+    R.tn = tn;
+    rc = ensure_scratch(ctx, (size_t)n * pnjl::SCRATCH_FIELDS * sizeof(double));
+    if (rc) return rc;
+    R.scratch = ctx->scratch;
+    const int LPP = 16;
+    int64_t blocks = (n * LPP + 255) / 256;
+    size_t lds = sizeof(double) * ((size_t)pn + tn + (size_t)pn * tn + pn);
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(pnjl::residual_kernel<LPP>, dim3((unsigned)blocks), dim3(256), lds, st, R);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
+    if (!(flags & PNJL_FLAG_DEVICE_PTRS)) {
+        HIPCHK(ctx, hipMemcpyAsync(F, dF, 5 * nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(J, dJ, 25 * nd, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(omega, dom, nd, hipMemcpyDeviceToHost, st));
+    }
+    if (!(flags & PNJL_FLAG_ASYNC) || !(flags & PNJL_FLAG_DEVICE_PTRS)) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    }
+    return PNJL_OK;
+}
+
+int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* mu, const double* xi,
+                  const double* seed, int policy, int pn, int tn, const pnjl_solve_opts* opts, pnjl_launch_cfg* best,
+                  double* ns_per_point)
+{
+    if (!ctx || sample <= 0 || !T || !mu || !xi) return PNJL_ERR_ARG;
+    std::vector<double> x(5 * sample), om(sample), pr(sample), rh(sample), en(sample), ee(sample), ms(3 * sample),
+        rs(sample);
+    std::vector<int32_t> it(sample), cv(sample);
+    pnjl_solve_out out{};
+    out.x = x.data();
+    out.omega = om.data();
+    out.pressure = pr.data();
+    out.rho_norm = rh.data();
+    out.entropy = en.data();
+    out.energy = ee.data();
+    out.masses = ms.data();
+    out.residual_norm = rs.data();
+    out.iterations = it.data();
+    out.converged = cv.data();
+    const int lpps[] = {1, 2, 4, 8, 16, 64};
+    const int chunks[] = {1, 4, 16};
+    const int wpcs[] = {4, 8, 16};
+    pnjl_launch_cfg bcfg = ctx->cfg;
+    double bms = 1e300;
+    for (int l : lpps)
+        for (int c : chunks)
+            for (int w : wpcs) {
+                pnjl_launch_cfg cfg{l, c, w, 0};
+                // warm, then time
+                int rc = solve_impl(ctx, sample, T, mu, xi, seed, policy, pn, tn, opts, &out, 0, nullptr, cfg);
+                if (rc) return rc;
+                rc = solve_impl(ctx, sample, T, mu, xi, seed, policy, pn, tn, opts, &out, 0, nullptr, cfg);
+                if (rc) return rc;
+                double t = ctx->last_ms;
+                if (t < bms) {
+                    bms = t;
+                    bcfg = cfg;
+                }
+            }
+    ctx->cfg = bcfg;
+    if (best) *best = bcfg;
+    if (ns_per_point) *ns_per_point = bms * 1e6 / (double)sample;
+    return PNJL_OK;
+}
+
+}  // extern "C"

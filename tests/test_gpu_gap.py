@@ -1,0 +1,204 @@
+"""GPU parity of the gap path: libpnjl_hip.so (through its C ABI) against the
+CPU oracle on the same seeded inputs, and against the reference's own stored
+scans.  Tolerances: roots / Omega to 1e-10 relative;
+the node pass itself to ~1e-13."""
+import numpy as np
+import pytest
+
+from conftest import golden_rows
+
+pytestmark = pytest.mark.gpu
+HB = 197.327
+
+
+def _grid(nT, nmu, xis, Tlo=20, Thi=300, mlo=0, mhi=450):
+    T, mu, xi = np.meshgrid(np.linspace(Tlo, Thi, nT), np.linspace(mlo, mhi, nmu), np.asarray(xis, float),
+                            indexing="ij")
+    return T.ravel() / HB, mu.ravel() / HB, xi.ravel()
+
+
+def _x_close(a, b, rtol=1e-10):
+    """state vectors: relative to max(|x|, 1) per component"""
+    return np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b))
+
+
+def test_residual_and_jacobian_match_oracle(gpu_ctx, oracle):
+    rng = np.random.default_rng(11)
+    n = 300
+    T = rng.uniform(15, 320, n) / HB
+    mu = rng.uniform(0, 460, n) / HB
+    xi = rng.choice([0.0, -0.4, 0.2, 0.4], n)
+    x = np.stack([rng.uniform(-2, -0.1, n), rng.uniform(-2, -0.1, n), rng.uniform(-2.3, -0.5, n),
+                  rng.uniform(0, 1, n), rng.uniform(0, 1, n)], 1)
+    x[: n // 2, 1] = x[: n // 2, 0]          # half on the phi_u == phi_d manifold
+    for pn, tn in ((64, 4), (24, 8)):
+        F, J, om = gpu_ctx.residual_batch(x, T, mu, xi, pn, tn)
+        for i in range(0, n, 3):
+            o, g, H = oracle.omega_derivs(x[i], mu[i], T[i], xi[i], pn, tn)
+            assert abs(om[i] - o) <= 1e-13 * max(1.0, abs(o)), i
+            scale = max(1.0, np.abs(g).max())
+            assert np.max(np.abs(F[i] + g)) <= 1e-12 * scale, i
+            hs = max(1.0, np.abs(H).max())
+            assert np.max(np.abs(J[i] + H)) <= 1e-12 * hs, i
+
+
+@pytest.mark.parametrize("policy,name", [(0, "default"), (3, "multi"), (1, "hadron"), (2, "quark")])
+def test_policy_parity_small_grid(gpu_ctx, oracle, policy, name):
+    T, mu, xi = _grid(10, 10, [0.0, -0.4, 0.3])
+    opts_g = None
+    r = gpu_ctx.solve_batch(T, mu, xi, policy=policy, p_num=64, t_num=4)
+    o = oracle.solve_batch(T, mu, xi, policy=policy, p_num=64, t_num=4, nthreads=8)
+    assert np.array_equal(r["converged"], o["converged"])
+    ok = o["converged"].astype(bool)
+    assert ok.mean() > 0.9
+    assert np.all(_x_close(r["x"][ok], o["x"][ok]))
+    assert np.all(np.abs(r["omega"][ok] - o["omega"][ok]) <= 1e-10 * np.abs(o["omega"][ok]))
+    for k in ("pressure", "entropy", "energy", "rho_norm"):
+        assert np.all(np.abs(r[k][ok] - o[k][ok]) <= 1e-9 * np.maximum(1, np.abs(o[k][ok]))), k
+    assert np.all(np.abs(r["n_q"][ok] - o["n_q"][ok]) <= 1e-9)
+    # integer by-products: identical except at points whose residual sits on the ftol edge
+    assert np.mean(r["iterations"] == o["iterations"]) > 0.98
+    assert np.mean(r["flags"] == o["flags"]) > 0.98
+
+
+def test_explicit_seed_and_asymmetric_states(gpu_ctx, oracle):
+    rng = np.random.default_rng(5)
+    n = 120
+    T = rng.uniform(30, 300, n) / HB
+    mu = rng.uniform(0, 450, n) / HB
+    xi = rng.uniform(-0.4, 0.4, n)
+    seed = np.stack([rng.uniform(-1.9, -0.2, n), rng.uniform(-1.9, -0.2, n), rng.uniform(-2.2, -0.8, n),
+                     rng.uniform(0, 1, n), rng.uniform(0, 1, n)], 1)
+    r = gpu_ctx.solve_batch(T, mu, xi, policy=4, seed=seed, p_num=32, t_num=4)
+    o = oracle.solve_batch(T, mu, xi, policy=4, seed=seed, p_num=32, t_num=4, nthreads=8)
+    assert np.array_equal(r["converged"], o["converged"])
+    ok = o["converged"].astype(bool)
+    # asymmetric seeds converge to the symmetric root: phi_u = phi_d to 1e-9
+    assert np.all(np.abs(r["x"][ok, 0] - r["x"][ok, 1]) < 1e-9)
+    assert np.all(_x_close(r["x"][ok], o["x"][ok], 1e-9))
+    assert np.all(np.abs(r["omega"][ok] - o["omega"][ok]) <= 1e-10 * np.abs(o["omega"][ok]))
+
+
+def _driver(name):
+    rows = golden_rows(name)
+    keys = sorted(set((float(r["xi"]), float(r["muB_MeV"])) for r in rows))
+    order = []
+    L = None
+    for k in keys:
+        sel = sorted([i for i, r in enumerate(rows) if (float(r["xi"]), float(r["muB_MeV"])) == k],
+                     key=lambda i: float(rows[i]["T_MeV"]))
+        L = len(sel)
+        order += sel
+    return [rows[i] for i in order], L
+
+
+@pytest.mark.parametrize("name", ["driver_gap_scan.csv", "driver_gap_scan_xi-0p6to0p6.csv"])
+def test_driver_chains_match_reference_scan(gpu_ctx, name):
+    """PhaseAwareContinuitySeed chains on the GPU reproduce the reference's
+    own run_gap_transport_scan.jl output (p12/t6, iterations=40)."""
+    from julia_relaxtime_amd import _lib
+    rows, L = _driver(name)
+    f = lambda k: np.array([float(r[k]) for r in rows])
+    r = gpu_ctx.solve_batch(f("T_fm"), f("muq_fm"), f("xi"), policy=6, p_num=12, t_num=6,
+                            opts=_lib.default_opts(max_iter=40, chain_len=L))
+    assert r["converged"].all()
+    om = f("omega_fm4inv")
+    assert np.max(np.abs(r["omega"] - om) / np.abs(om)) < 1e-12
+    M = np.stack([f("m_u"), f("m_d"), f("m_s")], 1)
+    assert np.max(np.abs(r["masses"] - M)) < 1e-8
+    assert np.max(np.abs(r["entropy"] - f("s_fm3inv"))) < 1e-8
+    assert np.mean(r["iterations"] == f("iterations").astype(int)) > 0.95
+
+
+def test_continuity_chain_c1(gpu_ctx, oracle):
+    """C1: 32x32 (T 50-300, mu 0-400 MeV), xi = 0, p = 64, one ContinuitySeed
+    chain in scan order (T outer, mu inner), gate |res| < 1e-10."""
+    from julia_relaxtime_amd import _lib
+    T, mu, xi = _grid(32, 32, [0.0], 50, 300, 0, 400)
+    n = T.size
+    r = gpu_ctx.solve_batch(T, mu, xi, policy=5, p_num=64, t_num=8,
+                            opts=_lib.default_opts(ftol=1e-10, chain_len=n))
+    o = oracle.solve_batch(T, mu, xi, policy=5, p_num=64, t_num=8,
+                           opts=oracle.default_opts(ftol=1e-10, chain_len=n))
+    assert np.array_equal(r["converged"], o["converged"])
+    ok = o["converged"].astype(bool)
+    assert ok.all()
+    assert np.all(r["residual_norm"] < 1e-10)
+    assert np.all(_x_close(r["x"], o["x"]))
+    assert np.all(np.abs(r["omega"] - o["omega"]) <= 1e-10 * np.abs(o["omega"]))
+
+
+def test_c2_full_grid_properties(gpu_ctx, oracle):
+    """C2 at full size (64x64x9 = 36,864 points, p=64, t=4, DefaultSeed per
+    point, |res| < 1e-10): size-independent properties on every point plus
+    oracle parity on a seeded sample."""
+    from julia_relaxtime_amd import _lib
+    xis = np.round(np.arange(-0.4, 0.41, 0.1), 10)
+    T, mu, xi = _grid(64, 64, xis)
+    opts = _lib.default_opts(ftol=1e-10, max_iter=40)
+    r = gpu_ctx.solve_batch(T, mu, xi, policy=0, p_num=64, t_num=4, opts=opts)
+    conv = r["converged"].astype(bool)
+    assert conv.mean() > 0.99
+    assert np.all(r["residual_norm"][conv] < 1e-10)
+    assert np.all(r["x"][conv, 0] == r["x"][conv, 1])
+    assert np.all((r["x"][conv, 3] >= -1e-8) & (r["x"][conv, 3] <= 1 + 1e-8))
+    assert np.all(r["masses"][conv] > 0)
+    assert np.allclose(r["pressure"], -r["omega"], rtol=0, atol=0, equal_nan=True)
+    rng = np.random.default_rng(20261202)
+    idx = rng.choice(T.size, 400, replace=False)
+    o = oracle.solve_batch(T[idx], mu[idx], xi[idx], policy=0, p_num=64, t_num=4,
+                           opts=oracle.default_opts(ftol=1e-10, max_iter=40), nthreads=8)
+    assert np.array_equal(r["converged"][idx], o["converged"])
+    ok = o["converged"].astype(bool)
+    assert np.all(_x_close(r["x"][idx][ok], o["x"][ok]))
+    assert np.all(np.abs(r["omega"][idx][ok] - o["omega"][ok]) <= 1e-10 * np.abs(o["omega"][ok]))
+
+
+def test_launch_geometry_does_not_change_results(gpu_ctx):
+    """Queue chunk size and persistent grid size only move work between
+    groups: bit-identical results.  Lanes per point change the summation
+    order: equal to rounding."""
+    T, mu, xi = _grid(12, 12, [0.0, 0.3])
+    base_cfg = gpu_ctx.launch_cfg()
+    try:
+        outs = []
+        for lpp, chunk, wpc in ((4, 1, 4), (4, 16, 8), (4, 3, 2)):
+            gpu_ctx.set_launch_cfg(lpp, chunk, wpc)
+            outs.append(gpu_ctx.solve_batch(T, mu, xi, policy=0, p_num=64, t_num=4))
+        for o in outs[1:]:
+            for k in ("x", "omega", "iterations", "converged", "entropy"):
+                assert np.array_equal(o[k], outs[0][k], equal_nan=True), k
+        for lpp in (1, 2, 8, 16, 64):
+            gpu_ctx.set_launch_cfg(lpp, 4, 8)
+            o = gpu_ctx.solve_batch(T, mu, xi, policy=0, p_num=64, t_num=4)
+            ok = outs[0]["converged"].astype(bool)
+            assert np.array_equal(o["converged"], outs[0]["converged"])
+            assert np.all(_x_close(o["x"][ok], outs[0]["x"][ok], 1e-11))
+    finally:
+        gpu_ctx.set_launch_cfg(**base_cfg)
+
+
+def test_device_pointer_path(gpu_ctx):
+    """Inputs already in HBM (torch tensors) give the same results as the host path."""
+    import torch
+    T, mu, xi = _grid(6, 6, [0.0, 0.2])
+    h = gpu_ctx.solve_batch(T, mu, xi, policy=0, p_num=64, t_num=4)
+    dev = torch.device("cuda", 0)
+    d = gpu_ctx.solve_batch(torch.tensor(T, device=dev), torch.tensor(mu, device=dev), torch.tensor(xi, device=dev),
+                            policy=0, p_num=64, t_num=4)
+    torch.cuda.synchronize()
+    for k in ("x", "omega", "iterations", "converged"):
+        assert np.array_equal(d[k].cpu().numpy(), h[k], equal_nan=True), k
+
+
+def test_error_paths(gpu_ctx):
+    from julia_relaxtime_amd import _lib
+    T = np.array([0.5])
+    with pytest.raises(_lib.PnjlError):
+        gpu_ctx.solve_batch(T, T, T, policy=4, seed=None)          # EXPLICIT needs seeds
+    with pytest.raises(_lib.PnjlError):
+        gpu_ctx.solve_batch(T, T, T, policy=42)
+    with pytest.raises(_lib.PnjlError):
+        gpu_ctx.solve_batch(T, T, T, policy=0, p_num=4096, t_num=2)
+    with pytest.raises(_lib.PnjlError):
+        gpu_ctx.set_launch_cfg(3, 1, 1)
