@@ -81,6 +81,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise PnjlError(f"{LIB_PATH} is missing: build it with julia_relaxtime_amd.build.build() "
                         "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    # torch wheels ship their own HIP/HSA runtime.  Loading torch first lets the
+    # library bind to that same runtime (same sonames), so a process that also
+    # uses torch for device memory holds one HIP runtime, not two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.pnjl_default_params.argtypes = [C.POINTER(ModelParams)]
