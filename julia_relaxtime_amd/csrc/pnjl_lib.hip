@@ -258,15 +258,111 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
         blocks = (int)std::max<int64_t>(1, (need_groups + gpb - 1) / gpb);
         groups = (int64_t)blocks * gpb;
     }
-    A.n_groups = groups;
-    rc = ensure_scratch(ctx, (size_t)groups * pnjl::SCRATCH_FIELDS * sizeof(double));
+    // the later pipeline stages run the full persistent grid: size the
+    // per-group scratch for the larger of the two grids
+    const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / 4);
+    const int64_t cap_groups = std::max<int64_t>(groups, (int64_t)fullblocks * 256 / cfg.lanes_per_point);
+    A.n_groups = cap_groups;
+    rc = ensure_scratch(ctx, (size_t)cap_groups * pnjl::SCRATCH_FIELDS * sizeof(double));
     if (rc) return rc;
     A.scratch = ctx->scratch;
     size_t lds = sizeof(double) * ((size_t)pn + tn + (size_t)pn * tn + pn);
-    hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(1), 0, stream, ctx->queue);
-    HIPCHK(ctx, hipGetLastError());
+    if (chains) {
+        A.stage = pnjl::STAGE_NONE;
+        hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, ctx->queue, 8);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
+        launch_scan(A, cfg.lanes_per_point, blocks, lds, stream);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(ctx->ev1, stream));
+        return PNJL_OK;
+    }
+    // ---- staged pipeline: [learned seed] -> primary attempt -> MultiSeed candidates -> selection
+    const size_t rec = pnjl::REC_SIZE * sizeof(double);
+    const bool surr = (policy == PNJL_SEED_SURROGATE), multi = (policy == PNJL_SEED_MULTI);
+    void *recS = nullptr, *recP = nullptr, *recM = nullptr, *listP = nullptr, *listM = nullptr, *mslot = nullptr;
+    if ((rc = stage_buf(ctx, 40, (size_t)n * rec * (multi ? 6 : 6), &recM))) return rc;
+    if (!multi) {
+        if ((rc = stage_buf(ctx, 41, (size_t)n * rec, &recP))) return rc;
+        if ((rc = stage_buf(ctx, 42, (size_t)n * 4, &listM))) return rc;
+        if ((rc = stage_buf(ctx, 43, (size_t)n * 4, &mslot))) return rc;
+    }
+    if (surr) {
+        if ((rc = stage_buf(ctx, 44, (size_t)n * rec, &recS))) return rc;
+        if ((rc = stage_buf(ctx, 45, (size_t)n * 4, &listP))) return rc;
+    }
+    unsigned long long* q = ctx->queue;  // q[0..3]: stage work counters, q[4]: |listP|, q[5]: |listM|
     HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
-    launch_scan(A, cfg.lanes_per_point, blocks, lds, stream);
+    hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, 8);
+    if (A.passes)
+        hipLaunchKernelGGL(pnjl::zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A.passes, n);
+    if (A.steps)
+        hipLaunchKernelGGL(pnjl::zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A.steps, n);
+    HIPCHK(ctx, hipGetLastError());
+    pnjl::ScanArgs S = A;
+    S.mslot = (int32_t*)mslot;
+    if (multi) {
+        S.stage = pnjl::STAGE_MULTI;
+        S.mult = 6;
+        S.item_list = nullptr;
+        S.queue = q + 2;
+        S.rec_out = (double*)recM;
+        launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+    } else {
+        if (surr) {
+            S.stage = pnjl::STAGE_SURR;
+            S.mult = 1;
+            S.item_list = nullptr;
+            S.queue = q + 0;
+            S.rec_out = (double*)recS;
+            S.next_list = (int32_t*)listP;
+            S.next_count = q + 4;
+            launch_scan(S, cfg.lanes_per_point, blocks, lds, stream);
+            HIPCHK(ctx, hipGetLastError());
+        }
+        S.stage = pnjl::STAGE_PRIMARY;
+        S.mult = 1;
+        S.item_list = surr ? (const int32_t*)listP : nullptr;
+        S.item_count = q + 4;
+        S.queue = q + 1;
+        S.rec_out = (double*)recP;
+        S.next_list = (int32_t*)listM;
+        S.next_count = q + 5;
+        launch_scan(S, cfg.lanes_per_point, surr ? fullblocks : blocks, lds, stream);
+        HIPCHK(ctx, hipGetLastError());
+        S.stage = pnjl::STAGE_MULTI;
+        S.mult = 6;
+        S.item_list = (const int32_t*)listM;
+        S.item_count = q + 5;
+        S.queue = q + 2;
+        S.rec_out = (double*)recM;
+        S.next_list = nullptr;
+        S.next_count = nullptr;
+        launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+    }
+    HIPCHK(ctx, hipGetLastError());
+    pnjl::FinArgs F{};
+    F.n = n;
+    F.plan = multi ? pnjl::PLAN_MULTI : (surr ? pnjl::PLAN_SURR : pnjl::PLAN_DEFAULT);
+    F.auto_ms = o->auto_multiseed_fallback;
+    F.recS = (const double*)recS;
+    F.recP = (const double*)recP;
+    F.recM = (const double*)recM;
+    F.mslot = (const int32_t*)mslot;
+    F.x_out = out->x;
+    F.omega = out->omega;
+    F.pressure = out->pressure;
+    F.rho_norm = out->rho_norm;
+    F.entropy = out->entropy;
+    F.energy = out->energy;
+    F.masses = out->masses;
+    F.n_q = out->n_q;
+    F.n_qbar = out->n_qbar;
+    F.resnorm = out->residual_norm;
+    F.iterations = out->iterations;
+    F.converged = out->converged;
+    F.flags = out->flags;
+    hipLaunchKernelGGL(pnjl::finalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, F);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev1, stream));
     return PNJL_OK;

@@ -83,7 +83,21 @@ struct ScanArgs {
     const double* bnd;
     int nb;
     int64_t max_trips;    // hard cap on trips per point (safety exit)
+    // staged pipeline for the per-point policies (stage != STAGE_NONE): each
+    // item is one single-seed attempt; results go to rec_out, failures are
+    // appended to next_list for the following stage
+    int stage;
+    int mult;                               // attempts per listed point
+    const int32_t* item_list;               // points of this stage (null: all points)
+    const unsigned long long* item_count;   // device count of item_list
+    double* rec_out;                        // REC_SIZE doubles per record
+    int32_t* next_list;
+    unsigned long long* next_count;
+    int32_t* mslot;                         // point -> its index in the MultiSeed list
 };
+
+// stages of the per-point pipeline
+enum Stage { STAGE_NONE = 0, STAGE_SURR = 1, STAGE_PRIMARY = 2, STAGE_MULTI = 3, STAGE_MR3 = 4 };
 
 constexpr int kMaxBndRows = 128;
 constexpr int kBndStride = 4 + 2 * kMaxBndRows;
@@ -706,6 +720,8 @@ struct St {
     double x[5], xold[5];
     int phase, it, attempt, flags, steps, passes;
     int prev_phase;
+    int k;                    // staged pipeline: attempt index within the listed point
+    int64_t item_ix;          // staged pipeline: record index of the attempt
     bool fresh;               // next pass is the first evaluation of the phase
     bool sym;                 // seed has phi_u == phi_d: keep them equal
     bool have_best;
@@ -842,12 +858,37 @@ __device__ __noinline__ void begin_point(const ScanArgs& A, const Scr& s, St& st
     }
 }
 
+// Staged pipeline: an attempt ended -> its record goes to rec_out; a failed
+// learned seed or a failed primary attempt appends the point to the next stage.
+__device__ void staged_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl, bool conv, bool exc)
+{
+    if (gl == 0) {
+        double* r = A.rec_out + st.item_ix * REC_SIZE;
+        for (int f = 0; f < REC_SIZE; f++) r[f] = s.at(blk + f);
+        if (A.passes) atomicAdd(&A.passes[st.pt], st.passes);
+        if (A.steps) atomicAdd(&A.steps[st.pt], st.steps);
+        bool append = false;
+        if (A.stage == STAGE_SURR) append = !conv;
+        else if (A.stage == STAGE_PRIMARY) append = !conv && A.auto_ms && !exc;
+        if (append) {
+            unsigned long long j = atomicAdd(A.next_count, 1ULL);
+            A.next_list[j] = (int32_t)st.pt;
+            if (A.stage == STAGE_PRIMARY) A.mslot[st.pt] = (int32_t)j;
+        }
+    }
+    st.phase = PH_IDLE;
+}
+
 // An attempt (one single-seed solve) ended with its result in record `blk`.
 __device__ __noinline__ void attempt_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl)
 {
     bool exc = s.at(blk + RF_EXC) != 0.0;
     bool conv = !exc && rec_good(s, blk, A.resmax);
     s.at(blk + RF_CONV) = conv ? 1.0 : 0.0;
+    if (A.stage != STAGE_NONE) {
+        staged_done(A, s, st, blk, gl, conv, exc);
+        return;
+    }
     double x0[5];
     if (st.attempt == ATT_SURR) {
         if (conv) {
@@ -1110,13 +1151,80 @@ __device__ __noinline__ void control_step(const ScanArgs& A, const Scr& s, St& s
     attempt_done(A, s, st, choose_candidate(A, s), gl);
 }
 
+// Staged pipeline: sets up attempt st.k of point st.pt for the kernel's stage.
+__device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st)
+{
+    int64_t i = st.pt;
+    st.T = A.T[i];
+    st.mu = A.mu[i];
+    st.xi = A.xi[i];
+    st.flags = 0;
+    st.steps = 0;
+    st.passes = 0;
+    st.trips = 0;
+    st.have_best = false;
+    st.plan_multi = false;
+    double x0[5];
+    if (A.stage == STAGE_SURR) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
+        start_attempt(st, s, ATT_SURR, x0);
+        return;
+    }
+    if (A.stage == STAGE_MULTI) {
+        multi_seed(st.k, st.T, st.mu, x0);
+        start_attempt(st, s, st.k, x0);
+        return;
+    }
+    if (A.stage == STAGE_MR3) {
+        // learned seed, then the chirally broken and restored reference seeds
+        if (st.k == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
+        } else {
+            seed_const(st.k == 1 ? 0 : 1, x0);
+        }
+        start_attempt(st, s, st.k, x0);
+        return;
+    }
+    // STAGE_PRIMARY
+    switch (A.policy) {
+    case PNJL_SEED_HADRON: default_seed(1, st.T, st.mu, x0); break;
+    case PNJL_SEED_QUARK: default_seed(2, st.T, st.mu, x0); break;
+    case PNJL_SEED_EXPLICIT:
+#pragma unroll
+        for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
+        break;
+    default: default_seed(0, st.T, st.mu, x0); break;
+    }
+    start_attempt(st, s, ATT_PRIMARY, x0);
+}
+
 // ---------------------------------------------------------------------------
 // the persistent kernel
 // ---------------------------------------------------------------------------
 // next work point for the group; false when the queue is drained
 template <int LPP>
-__device__ __forceinline__ bool next_point(const ScanArgs& A, St& st, int gl, bool chains)
+__device__ __forceinline__ bool next_point(const ScanArgs& A, St& st, int gl, bool chains, int64_t n_items)
 {
+    if (A.stage != STAGE_NONE) {
+        if (st.item >= st.item_end) {
+            unsigned long long base = 0;
+            if (gl == 0) base = atomicAdd(A.queue, (unsigned long long)A.chunk);
+            base = __shfl(base, 0, LPP);
+            st.item = (int64_t)base;
+            st.item_end = st.item + A.chunk;
+            if (st.item_end > n_items) st.item_end = n_items;
+        }
+        if (st.item >= st.item_end) return false;
+        int64_t it = st.item++;
+        int64_t j = it / A.mult;
+        st.k = (int)(it - j * A.mult);
+        st.pt = A.item_list ? (int64_t)A.item_list[j] : j;
+        // records: per point for single-attempt stages, per item otherwise
+        st.item_ix = (A.mult == 1) ? st.pt : it;
+        return true;
+    }
     if (chains && st.pt + 1 < st.pt_end) {
         st.pt++;
         return true;
@@ -1154,8 +1262,12 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
 
     const int gl = threadIdx.x % LPP;
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPP;
+    if (g >= A.n_groups) return;  // beyond the scratch the host allocated
     const Scr s{A.scratch + g * SCRATCH_FIELDS};
-    const bool chains = (A.policy == PNJL_SEED_CONTINUITY || A.policy == PNJL_SEED_PHASE_AWARE);
+    const bool chains = (A.stage == STAGE_NONE) &&
+                        (A.policy == PNJL_SEED_CONTINUITY || A.policy == PNJL_SEED_PHASE_AWARE);
+    const int64_t n_items = (A.stage == STAGE_NONE) ? A.n_items
+                            : (A.item_list ? (int64_t)(*A.item_count) : A.n_points) * A.mult;
     St st;
     st.phase = PH_IDLE;
     st.item = 0;
@@ -1164,11 +1276,16 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
     st.pt_end = 0;
     st.have_prev = false;
     st.prev_phase = 0;
+    st.k = 0;
+    st.item_ix = 0;
 
     while (true) {
         if (st.phase == PH_IDLE) {
-            if (!next_point<LPP>(A, st, gl, chains)) break;
-            begin_point(A, s, st);
+            if (!next_point<LPP>(A, st, gl, chains, n_items)) break;
+            if (A.stage == STAGE_NONE)
+                begin_point(A, s, st);
+            else
+                begin_item(A, s, st);
         }
         {
             double x[5];
@@ -1183,7 +1300,10 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
             double xn[5] = {qnan(), qnan(), qnan(), qnan(), qnan()};
             put_exc_rec(s, SB_SINGLE, xn);
             s.at(SB_SINGLE + RF_CONV) = 0.0;
-            finish_point(A, s, st, SB_SINGLE, gl);
+            if (A.stage == STAGE_NONE)
+                finish_point(A, s, st, SB_SINGLE, gl);
+            else
+                staged_done(A, s, st, SB_SINGLE, gl, false, true);
         }
     }
 }
@@ -1230,6 +1350,118 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs R)
     }
 }
 
-__global__ void zero_u64(unsigned long long* p) { *p = 0ULL; }
+__global__ void zero_u64(unsigned long long* p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ULL;
+}
+
+__global__ void zero_i32(int32_t* p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p && i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// staged pipeline: per-point selection of the final result
+// ---------------------------------------------------------------------------
+enum Plan { PLAN_DEFAULT = 0, PLAN_SURR = 1, PLAN_MULTI = 2, PLAN_MR3 = 3 };
+
+struct FinArgs {
+    int64_t n;
+    int plan;
+    int auto_ms;
+    const double* recS;   // per point (PLAN_SURR)
+    const double* recP;   // per point (PLAN_DEFAULT, PLAN_SURR)
+    const double* recM;   // 6 per MultiSeed point, or 6 / 3 per point (PLAN_MULTI / PLAN_MR3)
+    const int32_t* mslot;
+    double *x_out, *omega, *pressure, *rho_norm, *entropy, *energy, *masses, *n_q, *n_qbar, *resnorm;
+    int32_t *iterations, *converged, *flags;
+};
+
+// sameness of two roots (DualBranchScan._is_same_solution, DualBranchScan.jl:409-419)
+__device__ __forceinline__ bool same_root(const double* a, const double* b)
+{
+    return fabs(a[RF_X + 0] - b[RF_X + 0]) < 0.01 && fabs(a[RF_X + 3] - b[RF_X + 3]) < 0.01 &&
+           fabs(a[RF_M + 0] - b[RF_M + 0]) * 197.327 < 0.01 * 197.327;
+}
+
+// argmin Omega over the converged records (first on ties), -1 if none
+__device__ __forceinline__ int best_of(const double* base, int m)
+{
+    int best = -1;
+    for (int k = 0; k < m; k++) {
+        const double* r = base + (int64_t)k * REC_SIZE;
+        if (r[RF_CONV] != 0.0 && (best < 0 || r[RF_OMEGA] < base[(int64_t)best * REC_SIZE + RF_OMEGA])) best = k;
+    }
+    return best;
+}
+
+__global__ void finalize_kernel(FinArgs F)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= F.n) return;
+    const double* rec = nullptr;
+    int flags = 0;
+    bool conv_override = false;
+    if (F.plan == PLAN_MULTI || F.plan == PLAN_MR3) {
+        const int m = (F.plan == PLAN_MULTI) ? 6 : 3;
+        const double* base = F.recM + p * m * REC_SIZE;
+        int b = best_of(base, m);
+        if (b >= 0) {
+            rec = base + (int64_t)b * REC_SIZE;
+            if (F.plan == PLAN_MULTI) flags |= PNJL_PT_MULTISEED;
+            if (F.plan == PLAN_MR3) {
+                // first-order flag: two distinct converged roots
+                for (int i = 0; i < m; i++)
+                    for (int j = i + 1; j < m; j++) {
+                        const double* ri = base + (int64_t)i * REC_SIZE;
+                        const double* rj = base + (int64_t)j * REC_SIZE;
+                        if (ri[RF_CONV] != 0.0 && rj[RF_CONV] != 0.0 && !same_root(ri, rj)) flags |= PNJL_PT_FIRST_ORDER;
+                    }
+                // the learned seed's root lost the Omega check / did not converge
+                if (base[RF_CONV] == 0.0) flags |= PNJL_PT_SURR_FALLBACK;
+                else if (b != 0 && !same_root(base, rec)) flags |= PNJL_PT_SURR_OMEGA;
+            }
+        } else {
+            rec = base;
+            flags |= PNJL_PT_NO_ROOT;
+            conv_override = true;
+        }
+    } else {
+        if (F.plan == PLAN_SURR) {
+            const double* rs = F.recS + p * REC_SIZE;
+            if (rs[RF_CONV] != 0.0) rec = rs;
+            else flags |= PNJL_PT_SURR_FALLBACK;
+        }
+        if (!rec) {
+            const double* single = F.recP + p * REC_SIZE;
+            bool conv = single[RF_CONV] != 0.0, exc = single[RF_EXC] != 0.0;
+            rec = single;
+            if (!conv && F.auto_ms && !exc) {
+                const double* base = F.recM + (int64_t)F.mslot[p] * 6 * REC_SIZE;
+                int b = best_of(base, 6);
+                if (b >= 0) {
+                    rec = base + (int64_t)b * REC_SIZE;
+                    flags |= PNJL_PT_MULTISEED;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 5; k++) F.x_out[p * 5 + k] = rec[RF_X + k];
+    F.omega[p] = rec[RF_OMEGA];
+    F.pressure[p] = rec[RF_P];
+    F.rho_norm[p] = rec[RF_RHO];
+    F.entropy[p] = rec[RF_S];
+    F.energy[p] = rec[RF_E];
+    for (int k = 0; k < 3; k++) {
+        F.masses[p * 3 + k] = rec[RF_M + k];
+        if (F.n_q) F.n_q[p * 3 + k] = rec[RF_NQ + k];
+        if (F.n_qbar) F.n_qbar[p * 3 + k] = rec[RF_NQB + k];
+    }
+    F.resnorm[p] = rec[RF_RES];
+    F.iterations[p] = (int32_t)rec[RF_IT];
+    F.converged[p] = conv_override ? 0 : (int32_t)rec[RF_CONV];
+    if (F.flags) F.flags[p] = (int32_t)rec[RF_FLAGS] | flags;
+}
 
 }  // namespace pnjl
