@@ -118,8 +118,23 @@ typedef struct pnjl_solve_opts {
                                      abandoned (8)                           */
     int    chain_len;             /* CONTINUITY / PHASE_AWARE: points per
                                      chain (consecutive in the batch)       */
-    int    reserved;
+    int    surrogate_judge;       /* SURROGATE inside the judge mask:
+                                     PNJL_JUDGE_REFERENCE (0) or
+                                     PNJL_JUDGE_OMEGA (1)                    */
 } pnjl_solve_opts;
+
+/* judge modes of PNJL_SEED_SURROGATE inside the ambiguity mask
+ * (pnjl_set_judge_mask):
+ *   REFERENCE  the learned seed is not used there: the reference DefaultSeed
+ *              policy (hint seed, automatic MultiSeed fallback) solves the
+ *              point, so the scan returns the reference's own root branch;
+ *   OMEGA      the learned (or fallback) root is compared with the roots of
+ *              the MultiSeed candidates -- HADRON and hint-QUARK, plus the
+ *              other four where either of those fails -- and the
+ *              minimum-Omega root is kept (the thermodynamically stable
+ *              branch).                                                    */
+#define PNJL_JUDGE_REFERENCE 0
+#define PNJL_JUDGE_OMEGA     1
 
 void pnjl_default_solve_opts(pnjl_solve_opts* out);
 
@@ -157,9 +172,12 @@ typedef struct pnjl_solve_out {
 #define PNJL_PT_NO_ROOT        0x08  /* MultiSeed: no seed converged (reference
                                         throws ErrorException)                */
 #define PNJL_PT_SURR_FALLBACK  0x10  /* learned seed abandoned -> plain seeds  */
-#define PNJL_PT_SURR_OMEGA     0x20  /* learned root lost the 3-seed Omega check */
-#define PNJL_PT_FIRST_ORDER    0x40  /* >= 2 distinct physical roots (MULTIROOT3) */
+#define PNJL_PT_SURR_OMEGA     0x20  /* learned root lost the Omega check      */
+#define PNJL_PT_FIRST_ORDER    0x40  /* >= 2 distinct physical roots (MULTIROOT3,
+                                        SURROGATE with PNJL_JUDGE_OMEGA)      */
 #define PNJL_PT_CHAIN_RESET    0x80  /* chain: phase switch re-seeded the point */
+#define PNJL_PT_SURR_MASKED    0x100 /* in the judge mask: the reference policy
+                                        solved the point (PNJL_JUDGE_REFERENCE) */
 
 /* Solves n FixedMu points in one persistent-kernel launch.
  * T, mu, xi: n values each.  seed: n*5 (EXPLICIT) or NULL.
@@ -192,6 +210,14 @@ int pnjl_set_phase_boundary(pnjl_ctx* ctx, int nrows, const double* xi, const do
                             const double* mu_MeV, int ncep, const double* cep_xi,
                             const double* cep_T_MeV, const double* cep_mu_MeV);
 
+/* Ambiguity mask of the learned seed's judge (no counterpart in the
+ * reference).  A point of a PNJL_SEED_SURROGATE batch whose (T, mu, xi)
+ * falls in a set bin of this nT x nmu x nxi grid over [lo, hi] (row-major,
+ * T slowest), or outside the box, is judged as opts->surrogate_judge says
+ * (PNJL_JUDGE_*).  mask == NULL clears it (no judge).                       */
+int pnjl_set_judge_mask(pnjl_ctx* ctx, int nT, int nmu, int nxi, const double* lo3, const double* hi3,
+                        const uint8_t* mask);
+
 /* Launch geometry of the persistent scan kernel; chosen per config by the
  * autotuner (pnjl_autotune) or set explicitly.                             */
 typedef struct pnjl_launch_cfg {
@@ -215,6 +241,42 @@ int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* 
 /* Kernel time of the last solve/residual launch in milliseconds (HIP events
  * recorded around the kernel on the call's stream).                        */
 double pnjl_last_kernel_ms(const pnjl_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* learned seed (no counterpart in the reference: it feeds                  */
+/* PNJL_SEED_SURROGATE / PNJL_SEED_MULTIROOT3, whose fallback and judge are  */
+/* the reference seeds of SeedStrategies.jl)                                 */
+/* MLP 3 -> 128 -> 128 -> 128 -> 4 (SiLU) on normalised (T, mu, xi) giving   */
+/* normalised (phi_u, phi_s, Phi, Phibar); BF16 weights/activations, FP32   */
+/* MFMA accumulation, FP32 master weights trained by Adam on the GPU.       */
+/* Parameters are one packed FP32 vector, row-major:                        */
+/*   W1[128][3] b1[128] W2[128][128] b2[128] W3[128][128] b3[128]           */
+/*   W4[4][128] b4[4]   (34052 values)                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct pnjl_surrogate pnjl_surrogate;
+
+int  pnjl_surrogate_param_count(void);
+int  pnjl_surrogate_create(int device, uint64_t init_seed, pnjl_surrogate** out);
+void pnjl_surrogate_destroy(pnjl_surrogate* s);
+const char* pnjl_surrogate_last_error(const pnjl_surrogate* s);
+double pnjl_surrogate_last_ms(const pnjl_surrogate* s);
+/* inputs map [in_lo, in_hi] -> [-1, 1]; outputs y = mean + std * net      */
+int  pnjl_surrogate_set_scaling(pnjl_surrogate* s, const double* in_lo3, const double* in_hi3,
+                                const double* out_mean4, const double* out_std4);
+int  pnjl_surrogate_get_params(pnjl_surrogate* s, float* params);
+int  pnjl_surrogate_set_params(pnjl_surrogate* s, const float* params);
+/* Adam (beta 0.9/0.999, eps 1e-8) on the MSE of the normalised outputs;
+ * each step draws `batch` rows of (T, mu, xi, roots[n*5]) at random.
+ * loss_out: `steps` floats or NULL.                                        */
+int  pnjl_surrogate_train(pnjl_surrogate* s, int64_t n, const double* T, const double* mu,
+                          const double* xi, const double* roots, int steps, int batch, float lr,
+                          uint64_t seed, float* loss_out, int flags, void* stream);
+/* loss and gradient of the MSE over rows 0..n-1 (no update; host pointers) */
+int  pnjl_surrogate_loss_grad(pnjl_surrogate* s, int64_t n, const double* T, const double* mu,
+                              const double* xi, const double* roots, float* loss, float* grad);
+/* seeds[n*5] = (phi_u, phi_u, phi_s, Phi, Phibar) predicted at (T, mu, xi)  */
+int  pnjl_surrogate_infer(pnjl_surrogate* s, int64_t n, const double* T, const double* mu,
+                          const double* xi, double* seeds, int flags, void* stream);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,8 @@ SEED_CONTINUITY, SEED_PHASE_AWARE, SEED_SURROGATE, SEED_MULTIROOT3 = 5, 6, 7, 8
 # per-point flags
 PT_TR_USED, PT_MULTISEED, PT_EXCEPTION, PT_NO_ROOT = 0x01, 0x02, 0x04, 0x08
 PT_SURR_FALLBACK, PT_SURR_OMEGA, PT_FIRST_ORDER, PT_CHAIN_RESET = 0x10, 0x20, 0x40, 0x80
+PT_SURR_MASKED = 0x100
+JUDGE_REFERENCE, JUDGE_OMEGA = 0, 1
 
 
 class PnjlError(RuntimeError):
@@ -41,7 +43,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("xtol", C.c_double), ("ftol", C.c_double), ("residual_norm_max", C.c_double),
                 ("phi_tol", C.c_double), ("max_iter", C.c_int), ("trust_region_fallback", C.c_int),
                 ("auto_multiseed_fallback", C.c_int), ("surrogate_max_steps", C.c_int),
-                ("chain_len", C.c_int), ("reserved", C.c_int)]
+                ("chain_len", C.c_int), ("surrogate_judge", C.c_int)]
 
 
 class LaunchCfg(C.Structure):
@@ -104,6 +106,7 @@ def load():
     L.pnjl_gauleg.argtypes = [C.c_double, C.c_double, C.c_int, _DP, _DP]
     L.pnjl_nodes.argtypes = [vp, C.c_int, C.c_int, _DP, _DP, _DP]
     L.pnjl_set_phase_boundary.argtypes = [vp, C.c_int, _DP, _DP, _DP, C.c_int, _DP, _DP, _DP]
+    L.pnjl_set_judge_mask.argtypes = [vp, C.c_int, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_uint8)]
     L.pnjl_set_launch_cfg.argtypes = [vp, C.POINTER(LaunchCfg)]
     L.pnjl_get_launch_cfg.argtypes = [vp, C.POINTER(LaunchCfg)]
     L.pnjl_solve_fixedmu_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
@@ -112,7 +115,7 @@ def load():
                                           C.c_int, vp]
     L.pnjl_autotune.argtypes = [vp, C.c_int64, _DP, _DP, _DP, _DP, C.c_int, C.c_int, C.c_int,
                                 C.POINTER(SolveOpts), C.POINTER(LaunchCfg), C.POINTER(C.c_double)]
-    for name in ("pnjl_ctx_create", "pnjl_gauleg", "pnjl_nodes", "pnjl_set_phase_boundary",
+    for name in ("pnjl_ctx_create", "pnjl_gauleg", "pnjl_nodes", "pnjl_set_phase_boundary", "pnjl_set_judge_mask",
                  "pnjl_set_launch_cfg", "pnjl_get_launch_cfg", "pnjl_solve_fixedmu_batch",
                  "pnjl_gap_residual_batch", "pnjl_autotune"):
         getattr(L, name).restype = C.c_int
@@ -204,6 +207,18 @@ class Context:
         ccols = [np.ascontiguousarray(cep[:, i]) for i in range(3)]
         self._check(self._L.pnjl_set_phase_boundary(self.h, len(cols[0]), *[_dptr(c) for c in cols], len(ccols[0]),
                                                     *[_dptr(c) for c in ccols]), "pnjl_set_phase_boundary")
+
+    def set_judge_mask(self, mask, lo, hi):
+        """Ambiguity mask (uint8 [nT, nmu, nxi] over the box [lo, hi]) of the
+        learned seed's 3-seed Omega check; None clears it."""
+        if mask is None:
+            self._check(self._L.pnjl_set_judge_mask(self.h, 0, 0, 0, None, None, None), "pnjl_set_judge_mask")
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        self._check(self._L.pnjl_set_judge_mask(self.h, m.shape[0], m.shape[1], m.shape[2], _dptr(lo), _dptr(hi),
+                                                m.ctypes.data_as(C.POINTER(C.c_uint8))), "pnjl_set_judge_mask")
 
     def set_launch_cfg(self, lanes_per_point=4, chunk=4, waves_per_cu=8):
         cfg = LaunchCfg(lanes_per_point, chunk, waves_per_cu, 0)

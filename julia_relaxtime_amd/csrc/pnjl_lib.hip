@@ -17,6 +17,9 @@
 #include "../../include/pnjl_hip.h"
 #include "pnjl_scan_kernels.h"
 
+// device work counters of the staged pipeline (u64 words)
+constexpr int kQueueWords = 16;
+
 struct pnjl_ctx {
     int device = 0;
     int num_cu = 256;
@@ -30,6 +33,9 @@ struct pnjl_ctx {
     unsigned long long* queue = nullptr;
     double* bnd = nullptr;
     int nb = 0;
+    uint8_t* jmask = nullptr;
+    int jdim[3] = {0, 0, 0};
+    double jlo[3] = {0, 0, 0}, jhi[3] = {1, 1, 1};
     std::vector<void*> stage;   // device staging buffers for host-pointer calls
     std::vector<size_t> stage_bytes;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -196,7 +202,7 @@ void launch_scan(const pnjl::ScanArgs& A, int lpp, int blocks, size_t lds, hipSt
 
 int check_policy(int p)
 {
-    return (p >= PNJL_SEED_DEFAULT && p <= PNJL_SEED_SURROGATE) ? PNJL_OK : PNJL_ERR_ARG;
+    return (p >= PNJL_SEED_DEFAULT && p <= PNJL_SEED_MULTIROOT3) ? PNJL_OK : PNJL_ERR_ARG;
 }
 
 // core of pnjl_solve_fixedmu_batch with device pointers
@@ -244,6 +250,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     A.tr_fb = o->trust_region_fallback;
     A.auto_ms = o->auto_multiseed_fallback;
     A.surr_cap = o->surrogate_max_steps;
+    A.judge_mode = o->surrogate_judge;
     A.queue = ctx->queue;
     A.chunk = std::max(1, cfg.chunk);
     A.bnd = ctx->bnd;
@@ -269,7 +276,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     size_t lds = sizeof(double) * ((size_t)pn + tn + (size_t)pn * tn + pn);
     if (chains) {
         A.stage = pnjl::STAGE_NONE;
-        hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, ctx->queue, 8);
+        hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, ctx->queue, kQueueWords);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
         launch_scan(A, cfg.lanes_per_point, blocks, lds, stream);
@@ -280,20 +287,32 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     // ---- staged pipeline: [learned seed] -> primary attempt -> MultiSeed candidates -> selection
     const size_t rec = pnjl::REC_SIZE * sizeof(double);
     const bool surr = (policy == PNJL_SEED_SURROGATE), multi = (policy == PNJL_SEED_MULTI);
+    const bool mr3 = (policy == PNJL_SEED_MULTIROOT3);
     void *recS = nullptr, *recP = nullptr, *recM = nullptr, *listP = nullptr, *listM = nullptr, *mslot = nullptr;
     if ((rc = stage_buf(ctx, 40, (size_t)n * rec * (multi ? 6 : 6), &recM))) return rc;
-    if (!multi) {
+    if (!multi && !mr3) {
         if ((rc = stage_buf(ctx, 41, (size_t)n * rec, &recP))) return rc;
         if ((rc = stage_buf(ctx, 42, (size_t)n * 4, &listM))) return rc;
         if ((rc = stage_buf(ctx, 43, (size_t)n * 4, &mslot))) return rc;
     }
+    void *recJ = nullptr, *listJ = nullptr, *jslot = nullptr, *jb = nullptr, *listJ2 = nullptr;
+    const bool omega_judge = surr && ctx->jmask && o->surrogate_judge == PNJL_JUDGE_OMEGA;
     if (surr) {
         if ((rc = stage_buf(ctx, 44, (size_t)n * rec, &recS))) return rc;
         if ((rc = stage_buf(ctx, 45, (size_t)n * 4, &listP))) return rc;
+        if (omega_judge) {
+            if ((rc = stage_buf(ctx, 46, (size_t)n * pnjl::JUDGE_SEEDS * rec, &recJ))) return rc;
+            if ((rc = stage_buf(ctx, 47, (size_t)n * 4, &listJ))) return rc;
+            if ((rc = stage_buf(ctx, 48, (size_t)n * 4, &jslot))) return rc;
+            if ((rc = stage_buf(ctx, 49, (size_t)n * 4, &jb))) return rc;
+            if ((rc = stage_buf(ctx, 50, (size_t)n * 4, &listJ2))) return rc;
+        }
     }
-    unsigned long long* q = ctx->queue;  // q[0..3]: stage work counters, q[4]: |listP|, q[5]: |listM|
+    // q[0..3]: stage work counters, q[4]: |listP|, q[5]: |listM|, q[6]: |listJ|, q[7]: judge work
+    // counter, q[8]: |listJ2|, q[9]: second judge work counter
+    unsigned long long* q = ctx->queue;
     HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
-    hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, 8);
+    hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, kQueueWords);
     if (A.passes)
         hipLaunchKernelGGL(pnjl::zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A.passes, n);
     if (A.steps)
@@ -301,9 +320,9 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     HIPCHK(ctx, hipGetLastError());
     pnjl::ScanArgs S = A;
     S.mslot = (int32_t*)mslot;
-    if (multi) {
-        S.stage = pnjl::STAGE_MULTI;
-        S.mult = 6;
+    if (multi || mr3) {
+        S.stage = multi ? pnjl::STAGE_MULTI : pnjl::STAGE_MR3;
+        S.mult = multi ? 6 : 3;
         S.item_list = nullptr;
         S.queue = q + 2;
         S.rec_out = (double*)recM;
@@ -317,6 +336,20 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
             S.rec_out = (double*)recS;
             S.next_list = (int32_t*)listP;
             S.next_count = q + 4;
+            if (ctx->jmask) {
+                S.judge_mask = ctx->jmask;
+                for (int k = 0; k < 3; k++) {
+                    S.jdim[k] = ctx->jdim[k];
+                    S.jlo[k] = ctx->jlo[k];
+                    S.jhi[k] = ctx->jhi[k];
+                }
+                if (omega_judge) {
+                    S.judge_list = (int32_t*)listJ;
+                    S.judge_count = q + 6;
+                    S.jslot = (int32_t*)jslot;
+                    S.jb = (int32_t*)jb;
+                }
+            }
             launch_scan(S, cfg.lanes_per_point, blocks, lds, stream);
             HIPCHK(ctx, hipGetLastError());
         }
@@ -338,17 +371,46 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
         S.rec_out = (double*)recM;
         S.next_list = nullptr;
         S.next_count = nullptr;
+        S.judge_mask = nullptr;
+        S.jslot = nullptr;
         launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+        if (omega_judge) {
+            HIPCHK(ctx, hipGetLastError());
+            S.stage = pnjl::STAGE_JUDGE;
+            S.mult = 2;
+            S.item_list = (const int32_t*)listJ;
+            S.item_count = q + 6;
+            S.queue = q + 7;
+            S.rec_out = (double*)recJ;
+            S.jb = (int32_t*)jb;
+            S.judge2_list = (int32_t*)listJ2;
+            S.judge2_count = q + 8;
+            launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+            HIPCHK(ctx, hipGetLastError());
+            S.stage = pnjl::STAGE_JUDGE2;
+            S.mult = pnjl::JUDGE_SEEDS - 2;
+            S.item_list = (const int32_t*)listJ2;
+            S.item_count = q + 8;
+            S.queue = q + 9;
+            S.jslot = (int32_t*)jslot;
+            S.jb = nullptr;
+            S.judge2_list = nullptr;
+            S.judge2_count = nullptr;
+            launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+        }
     }
     HIPCHK(ctx, hipGetLastError());
     pnjl::FinArgs F{};
     F.n = n;
-    F.plan = multi ? pnjl::PLAN_MULTI : (surr ? pnjl::PLAN_SURR : pnjl::PLAN_DEFAULT);
+    F.plan = multi ? pnjl::PLAN_MULTI : (mr3 ? pnjl::PLAN_MR3 : (surr ? pnjl::PLAN_SURR : pnjl::PLAN_DEFAULT));
     F.auto_ms = o->auto_multiseed_fallback;
     F.recS = (const double*)recS;
     F.recP = (const double*)recP;
     F.recM = (const double*)recM;
     F.mslot = (const int32_t*)mslot;
+    F.recJ = (const double*)recJ;
+    F.jslot = (const int32_t*)jslot;
+    F.jb = (const int32_t*)jb;
     F.x_out = out->x;
     F.omega = out->omega;
     F.pressure = out->pressure;
@@ -401,7 +463,7 @@ void pnjl_default_solve_opts(pnjl_solve_opts* o)
     o->auto_multiseed_fallback = 1;
     o->surrogate_max_steps = 8;
     o->chain_len = 0;
-    o->reserved = 0;
+    o->surrogate_judge = PNJL_JUDGE_REFERENCE;
 }
 
 int pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out)
@@ -427,7 +489,7 @@ int pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->num_cu = prop.multiProcessorCount;
-    if (hipMalloc(&c->queue, 64) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+    if (hipMalloc(&c->queue, kQueueWords * sizeof(unsigned long long)) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
         hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
         return PNJL_ERR_HIP;
@@ -447,6 +509,7 @@ void pnjl_ctx_destroy(pnjl_ctx* c)
     if (c->scratch) hipFree(c->scratch);
     if (c->queue) hipFree(c->queue);
     if (c->bnd) hipFree(c->bnd);
+    if (c->jmask) hipFree(c->jmask);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     delete c;
@@ -529,6 +592,31 @@ int pnjl_set_phase_boundary(pnjl_ctx* ctx, int nrows, const double* xi, const do
     return PNJL_OK;
 }
 
+int pnjl_set_judge_mask(pnjl_ctx* ctx, int nT, int nmu, int nxi, const double* lo, const double* hi,
+                        const uint8_t* mask)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (ctx->jmask) HIPCHK(ctx, hipFree(ctx->jmask));
+    ctx->jmask = nullptr;
+    if (!mask) return PNJL_OK;
+    if (nT <= 0 || nmu <= 0 || nxi <= 0 || (int64_t)nT * nmu * nxi > (1 << 26) || !lo || !hi)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_set_judge_mask: bad geometry");
+    for (int k = 0; k < 3; k++)
+        if (!(hi[k] > lo[k])) return set_err(ctx, PNJL_ERR_ARG, "pnjl_set_judge_mask: empty box");
+    size_t bytes = (size_t)nT * nmu * nxi;
+    HIPCHK(ctx, hipMalloc(&ctx->jmask, bytes));
+    HIPCHK(ctx, hipMemcpy(ctx->jmask, mask, bytes, hipMemcpyHostToDevice));
+    ctx->jdim[0] = nT;
+    ctx->jdim[1] = nmu;
+    ctx->jdim[2] = nxi;
+    for (int k = 0; k < 3; k++) {
+        ctx->jlo[k] = lo[k];
+        ctx->jhi[k] = hi[k];
+    }
+    return PNJL_OK;
+}
+
 int pnjl_set_launch_cfg(pnjl_ctx* ctx, const pnjl_launch_cfg* cfg)
 {
     if (!ctx || !cfg || !valid_lpp(cfg->lanes_per_point) || cfg->chunk < 1 || cfg->waves_per_cu < 1 ||
@@ -552,7 +640,7 @@ static int solve_impl(pnjl_ctx* ctx, int64_t n, const double* T, const double* m
     if (!ctx) return PNJL_ERR_ARG;
     if (n < 0 || !T || !mu || !xi || !out || check_policy(policy))
         return set_err(ctx, PNJL_ERR_ARG, "pnjl_solve_fixedmu_batch: bad argument");
-    if ((policy == PNJL_SEED_EXPLICIT || policy == PNJL_SEED_SURROGATE) && !seed)
+    if ((policy == PNJL_SEED_EXPLICIT || policy == PNJL_SEED_SURROGATE || policy == PNJL_SEED_MULTIROOT3) && !seed)
         return set_err(ctx, PNJL_ERR_ARG, "pnjl_solve_fixedmu_batch: this policy needs seeds");
     if (!out->x || !out->omega || !out->pressure || !out->rho_norm || !out->entropy || !out->energy ||
         !out->masses || !out->residual_norm || !out->iterations || !out->converged)
@@ -565,6 +653,8 @@ static int solve_impl(pnjl_ctx* ctx, int64_t n, const double* T, const double* m
     else
         pnjl_default_solve_opts(&o);
     if (o.max_iter < 0 || o.max_iter > 100000) return set_err(ctx, PNJL_ERR_ARG, "max_iter out of range");
+    if (o.surrogate_judge != PNJL_JUDGE_REFERENCE && o.surrogate_judge != PNJL_JUDGE_OMEGA)
+        return set_err(ctx, PNJL_ERR_ARG, "surrogate_judge must be PNJL_JUDGE_REFERENCE or PNJL_JUDGE_OMEGA");
     if (n == 0) return PNJL_OK;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;

@@ -74,6 +74,7 @@ struct ScanArgs {
     // solver options
     double xtol, ftol, resmax, phi_tol;
     int max_iter, tr_fb, auto_ms, surr_cap;
+    int judge_mode;                         // PNJL_JUDGE_* (SURROGATE inside the mask)
     // work queue and scratch
     unsigned long long* queue;
     int chunk;
@@ -94,10 +95,33 @@ struct ScanArgs {
     int32_t* next_list;
     unsigned long long* next_count;
     int32_t* mslot;                         // point -> its index in the MultiSeed list
+    // learned-seed judge: a converged root inside the ambiguity mask is
+    // appended to judge_list for the Omega check against the MultiSeed
+    // candidates (HADRON and hint-QUARK first; where one of them fails, the
+    // other four candidates via judge2_list)
+    const uint8_t* judge_mask;              // [nT][nmu][nxi] bins over [jlo, jhi]
+    int jdim[3];
+    double jlo[3], jhi[3];
+    int32_t* judge_list;
+    unsigned long long* judge_count;
+    int32_t* jslot;                         // point -> its index in the judge list (-1 none)
+    int32_t* jb;                            // judge index -> 1 once its point is in judge2_list
+    int32_t* judge2_list;
+    unsigned long long* judge2_count;
 };
 
 // stages of the per-point pipeline
-enum Stage { STAGE_NONE = 0, STAGE_SURR = 1, STAGE_PRIMARY = 2, STAGE_MULTI = 3, STAGE_MR3 = 4 };
+enum Stage {
+    STAGE_NONE = 0,
+    STAGE_SURR = 1,
+    STAGE_PRIMARY = 2,
+    STAGE_MULTI = 3,
+    STAGE_MR3 = 4,
+    STAGE_JUDGE = 5,    // MultiSeed candidates 0, 1 (HADRON, hint-QUARK) of a judged point
+    STAGE_JUDGE2 = 6    // candidates 2..5, where candidate 0 or 1 did not converge
+};
+// judge records per judged point: the six MultiSeed candidates
+constexpr int JUDGE_SEEDS = 6;
 
 constexpr int kMaxBndRows = 128;
 constexpr int kBndStride = 4 + 2 * kMaxBndRows;
@@ -858,6 +882,22 @@ __device__ __noinline__ void begin_point(const ScanArgs& A, const Scr& s, St& st
     }
 }
 
+// ambiguity mask lookup (bins over the training box; outside the box: judged)
+__device__ __forceinline__ bool in_judge_region(const ScanArgs& A, double T, double mu, double xi)
+{
+    const double v[3] = {T, mu, xi};
+    int64_t idx = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double u = (v[k] - A.jlo[k]) / (A.jhi[k] - A.jlo[k]);
+        if (!(u >= 0.0 && u <= 1.0)) return true;
+        int b = (int)(u * A.jdim[k]);
+        b = b < 0 ? 0 : (b >= A.jdim[k] ? A.jdim[k] - 1 : b);
+        idx = idx * A.jdim[k] + b;
+    }
+    return A.judge_mask[idx] != 0;
+}
+
 // Staged pipeline: an attempt ended -> its record goes to rec_out; a failed
 // learned seed or a failed primary attempt appends the point to the next stage.
 __device__ void staged_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl, bool conv, bool exc)
@@ -874,6 +914,27 @@ __device__ void staged_done(const ScanArgs& A, const Scr& s, St& st, int blk, in
             unsigned long long j = atomicAdd(A.next_count, 1ULL);
             A.next_list[j] = (int32_t)st.pt;
             if (A.stage == STAGE_PRIMARY) A.mslot[st.pt] = (int32_t)j;
+        }
+        if ((A.stage == STAGE_SURR || A.stage == STAGE_PRIMARY) && A.jslot) {
+            // learned-seed plan: a converged root inside the ambiguity mask (the
+            // learned root, or the plain root of a point whose learned seed
+            // failed) is checked against the MultiSeed candidates' roots
+            int32_t js = -1;
+            if (conv && in_judge_region(A, st.T, st.mu, st.xi)) {
+                unsigned long long j = atomicAdd(A.judge_count, 1ULL);
+                A.judge_list[j] = (int32_t)st.pt;
+                A.jb[j] = 0;
+                js = (int32_t)j;
+            }
+            if (A.jslot) A.jslot[st.pt] = js;
+        }
+        if (A.stage == STAGE_JUDGE && !conv) {
+            // HADRON or hint-QUARK failed: the other MultiSeed candidates run
+            const int64_t js = st.item_ix / JUDGE_SEEDS;
+            if (atomicCAS(&A.jb[js], 0, 1) == 0) {
+                unsigned long long j = atomicAdd(A.judge2_count, 1ULL);
+                A.judge2_list[j] = (int32_t)st.pt;
+            }
         }
     }
     st.phase = PH_IDLE;
@@ -1152,7 +1213,7 @@ __device__ __noinline__ void control_step(const ScanArgs& A, const Scr& s, St& s
 }
 
 // Staged pipeline: sets up attempt st.k of point st.pt for the kernel's stage.
-__device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st)
+__device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st, int gl)
 {
     int64_t i = st.pt;
     st.T = A.T[i];
@@ -1166,6 +1227,18 @@ __device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st)
     st.plan_multi = false;
     double x0[5];
     if (A.stage == STAGE_SURR) {
+        if (A.judge_mode == PNJL_JUDGE_REFERENCE && A.judge_mask && in_judge_region(A, st.T, st.mu, st.xi)) {
+            // ambiguity region: the reference policy takes the point unseen
+            if (gl == 0) {
+                double* r = A.rec_out + st.item_ix * REC_SIZE;
+                r[RF_CONV] = 0.0;
+                r[RF_FLAGS] = (double)PNJL_PT_SURR_MASKED;
+                unsigned long long j = atomicAdd(A.next_count, 1ULL);
+                A.next_list[j] = (int32_t)st.pt;
+            }
+            st.phase = PH_IDLE;
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
         start_attempt(st, s, ATT_SURR, x0);
@@ -1174,6 +1247,13 @@ __device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st)
     if (A.stage == STAGE_MULTI) {
         multi_seed(st.k, st.T, st.mu, x0);
         start_attempt(st, s, st.k, x0);
+        return;
+    }
+    if (A.stage == STAGE_JUDGE || A.stage == STAGE_JUDGE2) {
+        // the Omega check of a learned-seed root: the MultiSeed candidates
+        const int c = (A.stage == STAGE_JUDGE) ? st.k : 2 + st.k;
+        multi_seed(c, st.T, st.mu, x0);
+        start_attempt(st, s, c, x0);
         return;
     }
     if (A.stage == STAGE_MR3) {
@@ -1221,8 +1301,11 @@ __device__ __forceinline__ bool next_point(const ScanArgs& A, St& st, int gl, bo
         int64_t j = it / A.mult;
         st.k = (int)(it - j * A.mult);
         st.pt = A.item_list ? (int64_t)A.item_list[j] : j;
-        // records: per point for single-attempt stages, per item otherwise
-        st.item_ix = (A.mult == 1) ? st.pt : it;
+        // records: per point for single-attempt stages, per item otherwise;
+        // judge records: JUDGE_SEEDS per judge index
+        if (A.stage == STAGE_JUDGE) st.item_ix = j * JUDGE_SEEDS + st.k;
+        else if (A.stage == STAGE_JUDGE2) st.item_ix = (int64_t)A.jslot[st.pt] * JUDGE_SEEDS + 2 + st.k;
+        else st.item_ix = (A.mult == 1) ? st.pt : it;
         return true;
     }
     if (chains && st.pt + 1 < st.pt_end) {
@@ -1285,7 +1368,8 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
             if (A.stage == STAGE_NONE)
                 begin_point(A, s, st);
             else
-                begin_item(A, s, st);
+                begin_item(A, s, st, gl);
+            if (st.phase == PH_IDLE) continue;   // handed on without an attempt
         }
         {
             double x[5];
@@ -1374,6 +1458,9 @@ struct FinArgs {
     const double* recP;   // per point (PLAN_DEFAULT, PLAN_SURR)
     const double* recM;   // 6 per MultiSeed point, or 6 / 3 per point (PLAN_MULTI / PLAN_MR3)
     const int32_t* mslot;
+    const double* recJ;   // JUDGE_SEEDS per judged point (the MultiSeed candidates)
+    const int32_t* jslot;
+    const int32_t* jb;    // judge index -> candidates 2..5 were run
     double *x_out, *omega, *pressure, *rho_norm, *entropy, *energy, *masses, *n_q, *n_qbar, *resnorm;
     int32_t *iterations, *converged, *flags;
 };
@@ -1431,7 +1518,7 @@ __global__ void finalize_kernel(FinArgs F)
         if (F.plan == PLAN_SURR) {
             const double* rs = F.recS + p * REC_SIZE;
             if (rs[RF_CONV] != 0.0) rec = rs;
-            else flags |= PNJL_PT_SURR_FALLBACK;
+            else flags |= ((int)rs[RF_FLAGS] & PNJL_PT_SURR_MASKED) ? PNJL_PT_SURR_MASKED : PNJL_PT_SURR_FALLBACK;
         }
         if (!rec) {
             const double* single = F.recP + p * REC_SIZE;
@@ -1445,6 +1532,22 @@ __global__ void finalize_kernel(FinArgs F)
                     flags |= PNJL_PT_MULTISEED;
                 }
             }
+        }
+        const int32_t js = (F.plan == PLAN_SURR && F.jslot) ? F.jslot[p] : -1;
+        if (js >= 0) {
+            // Omega check: minimum Omega among the learned (or fallback) root and
+            // the roots of the six MultiSeed candidates
+            const double* r0 = rec;
+            bool distinct = false;
+            const int nk = F.jb[js] ? JUDGE_SEEDS : 2;
+            for (int k = 0; k < nk; k++) {
+                const double* rj = F.recJ + ((int64_t)js * JUDGE_SEEDS + k) * REC_SIZE;
+                if (rj[RF_CONV] == 0.0) continue;
+                if (!same_root(rj, r0)) distinct = true;
+                if (rj[RF_OMEGA] < rec[RF_OMEGA]) rec = rj;
+            }
+            if (distinct) flags |= PNJL_PT_FIRST_ORDER;
+            if (rec != r0 && !same_root(rec, r0)) flags |= PNJL_PT_SURR_OMEGA;
         }
     }
     for (int k = 0; k < 5; k++) F.x_out[p * 5 + k] = rec[RF_X + k];

@@ -20,8 +20,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (SEED_CONTINUITY, SEED_DEFAULT, SEED_EXPLICIT, SEED_HADRON, SEED_MULTI, SEED_PHASE_AWARE,
-                   SEED_QUARK, PT_EXCEPTION, PT_NO_ROOT, PnjlError)
+from ._lib import (SEED_CONTINUITY, SEED_DEFAULT, SEED_EXPLICIT, SEED_HADRON, SEED_MULTI, SEED_MULTIROOT3,
+                   SEED_PHASE_AWARE, SEED_QUARK, SEED_SURROGATE, PT_EXCEPTION, PT_NO_ROOT, PnjlError)
 
 HBARC = 197.327
 _DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
@@ -143,6 +143,37 @@ class PhaseAwareContinuitySeed(SeedStrategy):
         return "hadron" if mu_MeV < muc else "quark"
 
 
+@dataclass
+class SurrogateSeed(SeedStrategy):
+    """Learned seed (julia_relaxtime_amd.surrogate.Surrogate).  Newton from the
+    learned seed gets `max_steps` steps; a point it does not solve in time is
+    handed to the plain DefaultSeed policy.  Inside the model's ambiguity mask
+    (`model.judge_mask`, from Surrogate.build_judge_mask) the judge decides:
+      "reference"  DefaultSeed solves the point (the reference's own branch);
+      "omega"      the minimum-Omega root among the learned root and the
+                   MultiSeed candidates is kept (the stable branch);
+      "multiroot3" every point, mask or not, is solved from the learned,
+                   HADRON and QUARK seeds and the minimum-Omega root kept."""
+    model: object = None
+    max_steps: int = 8
+    judge: str = "reference"
+
+    def policy(self):
+        if self.judge not in ("reference", "omega", "multiroot3"):
+            raise ValueError(f"unknown judge {self.judge!r}")
+        return SEED_MULTIROOT3 if self.judge == "multiroot3" else SEED_SURROGATE
+
+    def judge_mode(self):
+        return _lib.JUDGE_OMEGA if self.judge == "omega" else _lib.JUDGE_REFERENCE
+
+    def install(self, ctx):
+        m = getattr(self.model, "judge_mask", None)
+        if m is None or self.judge == "multiroot3":
+            ctx.set_judge_mask(None, None, None)
+        else:
+            ctx.set_judge_mask(m, self.model.in_lo, self.model.in_hi)
+
+
 def get_seed(s: SeedStrategy, theta, mode=None):
     """get_seed(strategy, [T, mu], FixedMu()) -> 5-vector (SeedStrategies.jl:204-839)."""
     T_fm, mu_fm = float(theta[0]), float(theta[1]) if len(theta) > 1 else 0.0
@@ -159,6 +190,8 @@ def get_seed(s: SeedStrategy, theta, mode=None):
         if s.previous_solution is not None:
             return list(s.previous_solution)[:5]
         return get_seed(s.fallback, theta)
+    if isinstance(s, SurrogateSeed):
+        return list(s.model.infer(np.array([T_fm]), np.array([mu_fm]), np.array([0.0]))[0])
     if isinstance(s, PhaseAwareContinuitySeed):
         cur = s.current_phase(T_fm * HBARC, mu_fm * HBARC)
         if s.previous_solution is None:
@@ -227,11 +260,13 @@ def context(device: int = 0) -> _lib.Context:
     return c
 
 
-def _opts(iterations, trust_region_fallback, auto_multiseed_fallback, residual_norm_max, xtol, ftol, chain_len=0):
+def _opts(iterations, trust_region_fallback, auto_multiseed_fallback, residual_norm_max, xtol, ftol, chain_len=0,
+          surrogate_max_steps=8, surrogate_judge=0):
     return _lib.default_opts(max_iter=int(iterations), trust_region_fallback=int(trust_region_fallback),
                              auto_multiseed_fallback=int(auto_multiseed_fallback),
                              residual_norm_max=float(residual_norm_max), xtol=float(xtol), ftol=float(ftol),
-                             chain_len=int(chain_len))
+                             chain_len=int(chain_len), surrogate_max_steps=int(surrogate_max_steps),
+                             surrogate_judge=int(surrogate_judge))
 
 
 def _policy_for(seed_strategy, T_fm, mu_fm):
@@ -252,6 +287,8 @@ def _policy_for(seed_strategy, T_fm, mu_fm):
             return SEED_MULTI, None
         g = get_seed(s, [T_fm, mu_fm])
         return SEED_EXPLICIT, g
+    if isinstance(s, SurrogateSeed):
+        return s.policy(), None
     raise TypeError(f"unsupported seed strategy {type(s).__name__}")
 
 
@@ -267,10 +304,17 @@ def solve(mode, T_fm, mu_fm, *, xi=0.0, seed_strategy=None, p_num=64, t_num=8, i
         raise TypeError("only FixedMu() is supported by this backend")
     s = seed_strategy if seed_strategy is not None else DefaultSeed()
     policy, seed = _policy_for(s, T_fm, mu_fm)
-    r = context(device).solve_batch(np.array([float(T_fm)]), np.array([float(mu_fm)]), np.array([float(xi)]),
-                                    policy=policy, seed=None if seed is None else np.array([seed]), p_num=p_num,
-                                    t_num=t_num, opts=_opts(iterations, trust_region_fallback,
-                                                            auto_multiseed_fallback, residual_norm_max, xtol, ftol))
+    steps, judge = 8, _lib.JUDGE_REFERENCE
+    ctx = context(device)
+    if isinstance(s, SurrogateSeed):
+        seed = s.model.infer(np.array([float(T_fm)]), np.array([float(mu_fm)]), np.array([float(xi)]))[0]
+        steps, judge = s.max_steps, s.judge_mode()
+        s.install(ctx)
+    r = ctx.solve_batch(np.array([float(T_fm)]), np.array([float(mu_fm)]), np.array([float(xi)]),
+                        policy=policy, seed=None if seed is None else np.array([seed]), p_num=p_num,
+                        t_num=t_num, opts=_opts(iterations, trust_region_fallback, auto_multiseed_fallback,
+                                                residual_norm_max, xtol, ftol, surrogate_max_steps=steps,
+                                                surrogate_judge=judge))
     fl = int(r["flags"][0])
     if fl & PT_EXCEPTION:
         raise FloatingPointError("non-finite residual at the seed (NLsolve IsFiniteException)")
@@ -305,8 +349,15 @@ def solve_batch(mode, T_fm, mu_fm, *, xi=0.0, seed_strategy=None, seeds=None, ch
     mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu_fm, dtype=np.float64), (n,)))
     xiv = np.ascontiguousarray(np.broadcast_to(np.asarray(xi, dtype=np.float64), (n,)))
     s = seed_strategy if seed_strategy is not None else DefaultSeed()
+    steps, judge = 8, _lib.JUDGE_REFERENCE
+    ctx = context(device)
     if seeds is not None:
         policy = SEED_EXPLICIT
+    elif isinstance(s, SurrogateSeed):
+        policy = s.policy()
+        seeds = s.model.infer(T, mu, xiv)
+        steps, judge = s.max_steps, s.judge_mode()
+        s.install(ctx)
     elif isinstance(s, ContinuitySeed):
         policy = SEED_CONTINUITY
     elif isinstance(s, PhaseAwareContinuitySeed):
@@ -318,14 +369,15 @@ def solve_batch(mode, T_fm, mu_fm, *, xi=0.0, seed_strategy=None, seeds=None, ch
     else:
         policy = SEED_EXPLICIT
         seeds = np.array([get_seed(s, [T[i], mu[i]]) for i in range(n)])
-    r = context(device).solve_batch(T, mu, xiv, policy=policy, seed=seeds, p_num=p_num, t_num=t_num,
-                                    opts=_opts(iterations, trust_region_fallback, auto_multiseed_fallback,
-                                               residual_norm_max, xtol, ftol, chain_len))
+    r = ctx.solve_batch(T, mu, xiv, policy=policy, seed=seeds, p_num=p_num, t_num=t_num,
+                        opts=_opts(iterations, trust_region_fallback, auto_multiseed_fallback, residual_norm_max,
+                                   xtol, ftol, chain_len, steps, judge))
     if as_results:
         return [_result(r, i, float(mu[i]), float(xiv[i])) for i in range(n)]
     return r
 
 
-__all__ = ["FixedMu", "DefaultSeed", "MultiSeed", "ContinuitySeed", "PhaseAwareContinuitySeed", "SolverResult",
+__all__ = ["FixedMu", "DefaultSeed", "MultiSeed", "ContinuitySeed", "PhaseAwareContinuitySeed", "SurrogateSeed",
+           "SolverResult",
            "solve", "solve_batch", "update", "reset", "get_seed", "context", "HBARC", "PnjlError",
            "load_phase_boundary", "interpolate_mu_c", "auto_phase_hint"]

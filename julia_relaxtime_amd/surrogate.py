@@ -1,0 +1,227 @@
+"""The learned seed: an MLP (T, mu, xi) -> (phi_u, phi_s, Phi, Phibar) trained
+on the GPU (libpnjl_hip.so: BF16-MFMA forward/backward, FP32 Adam) from roots
+the plain solver produced, and `SurrogateSeed`, the seed strategy that uses
+it.  The reference seeds stay the fallback and the judge: a learned seed whose
+Newton run does not converge in `surrogate_max_steps` steps hands the point
+to the plain DefaultSeed policy (PNJL_SEED_SURROGATE); inside an ambiguity
+mask learned on the training set (several roots, first-order region) the
+root is checked against the six MultiSeed candidates and the minimum-Omega
+root is kept; MULTIROOT3 keeps the minimum-Omega root among the learned,
+hadron and quark seeds everywhere.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import PnjlError
+
+HBARC = 197.327
+NPARAM = 34052
+# the build's training domain: T 20-300 MeV, mu 0-450 MeV, xi -0.4..0.4
+DOMAIN_LO = np.array([20.0 / HBARC, 0.0, -0.4])
+DOMAIN_HI = np.array([300.0 / HBARC, 450.0 / HBARC, 0.4])
+
+_DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+
+
+@_lib.register_extra
+def _register(L):
+    vp = C.c_void_p
+    L.pnjl_surrogate_param_count.restype = C.c_int
+    L.pnjl_surrogate_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+    L.pnjl_surrogate_create.restype = C.c_int
+    L.pnjl_surrogate_destroy.argtypes = [vp]
+    L.pnjl_surrogate_destroy.restype = None
+    L.pnjl_surrogate_last_error.argtypes = [vp]
+    L.pnjl_surrogate_last_error.restype = C.c_char_p
+    L.pnjl_surrogate_last_ms.argtypes = [vp]
+    L.pnjl_surrogate_last_ms.restype = C.c_double
+    L.pnjl_surrogate_set_scaling.argtypes = [vp, _DP, _DP, _DP, _DP]
+    L.pnjl_surrogate_get_params.argtypes = [vp, _FP]
+    L.pnjl_surrogate_set_params.argtypes = [vp, _FP]
+    L.pnjl_surrogate_train.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                       _FP, C.c_int, vp]
+    L.pnjl_surrogate_loss_grad.argtypes = [vp, C.c_int64, _DP, _DP, _DP, _DP, _FP, _FP]
+    L.pnjl_surrogate_infer.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, vp]
+    for n in ("pnjl_surrogate_set_scaling", "pnjl_surrogate_get_params", "pnjl_surrogate_set_params",
+              "pnjl_surrogate_train", "pnjl_surrogate_loss_grad", "pnjl_surrogate_infer"):
+        getattr(L, n).restype = C.c_int
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Surrogate:
+    """GPU-resident MLP seed model (one pnjl_surrogate)."""
+
+    def __init__(self, device: int = 0, init_seed: int = 0):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        rc = self._L.pnjl_surrogate_create(device, init_seed, C.byref(h))
+        if rc != 0:
+            raise PnjlError(f"pnjl_surrogate_create failed with status {rc}")
+        self.h = h
+        self.device = device
+        self.in_lo, self.in_hi = DOMAIN_LO.copy(), DOMAIN_HI.copy()
+        self.out_mean, self.out_std = np.zeros(4), np.ones(4)
+        self._push_scaling()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.pnjl_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            m = self._L.pnjl_surrogate_last_error(self.h)
+            raise PnjlError(f"{what} failed with status {rc}: {m.decode() if m else ''}")
+
+    def _push_scaling(self):
+        a = [_d(v) for v in (self.in_lo, self.in_hi, self.out_mean, self.out_std)]
+        self._check(self._L.pnjl_surrogate_set_scaling(self.h, *[v.ctypes.data_as(_DP) for v in a]), "set_scaling")
+
+    def set_scaling(self, in_lo=None, in_hi=None, out_mean=None, out_std=None):
+        if in_lo is not None:
+            self.in_lo = np.asarray(in_lo, float)
+        if in_hi is not None:
+            self.in_hi = np.asarray(in_hi, float)
+        if out_mean is not None:
+            self.out_mean = np.asarray(out_mean, float)
+        if out_std is not None:
+            self.out_std = np.asarray(out_std, float)
+        self._push_scaling()
+
+    def fit_output_scaling(self, roots):
+        y = np.stack([roots[:, 0], roots[:, 2], roots[:, 3], roots[:, 4]], 1)
+        self.set_scaling(out_mean=y.mean(0), out_std=np.maximum(y.std(0), 1e-6))
+
+    @property
+    def params(self):
+        p = np.zeros(NPARAM, dtype=np.float32)
+        self._check(self._L.pnjl_surrogate_get_params(self.h, p.ctypes.data_as(_FP)), "get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != NPARAM:
+            raise ValueError(f"expected {NPARAM} parameters")
+        self._check(self._L.pnjl_surrogate_set_params(self.h, p.ctypes.data_as(_FP)), "set_params")
+
+    def train(self, T, mu, xi, roots, steps=20000, batch=4096, lr=1e-3, seed=4):
+        """Adam on the GPU; returns the per-step loss (numpy) and stores the
+        training kernel time in `last_ms`."""
+        T, mu, xi, roots = _d(T), _d(mu), _d(xi), _d(roots).reshape(-1, 5)
+        n = T.size
+        loss = np.zeros(max(steps, 1), dtype=np.float32)
+        rc = self._L.pnjl_surrogate_train(self.h, n, T.ctypes.data, _d(mu).ctypes.data, xi.ctypes.data,
+                                          roots.ctypes.data, steps, batch, lr, seed, loss.ctypes.data_as(_FP), 0,
+                                          None)
+        self._check(rc, "pnjl_surrogate_train")
+        self.last_ms = self._L.pnjl_surrogate_last_ms(self.h)
+        return loss[:steps]
+
+    def loss_grad(self, T, mu, xi, roots):
+        T, mu, xi, roots = _d(T), _d(mu), _d(xi), _d(roots).reshape(-1, 5)
+        loss = C.c_float()
+        g = np.zeros(NPARAM, dtype=np.float32)
+        rc = self._L.pnjl_surrogate_loss_grad(self.h, T.size, T.ctypes.data_as(_DP), mu.ctypes.data_as(_DP),
+                                              xi.ctypes.data_as(_DP), roots.ctypes.data_as(_DP), C.byref(loss),
+                                              g.ctypes.data_as(_FP))
+        self._check(rc, "pnjl_surrogate_loss_grad")
+        return loss.value, g
+
+    def infer(self, T, mu, xi, out=None, stream=None):
+        """Seeds (n,5) = (phi_u, phi_u, phi_s, Phi, Phibar).  numpy in -> numpy
+        out; torch.cuda tensors in -> a torch tensor in HBM."""
+        if type(T).__module__.startswith("torch"):
+            import torch
+            n = T.numel()
+            if out is None:
+                out = torch.empty((n, 5), dtype=torch.float64, device=T.device)
+            st = stream if stream is not None else torch.cuda.current_stream(T.device)
+            rc = self._L.pnjl_surrogate_infer(self.h, n, T.data_ptr(), mu.data_ptr(), xi.data_ptr(), out.data_ptr(),
+                                              _lib.FLAG_DEVICE_PTRS, C.c_void_p(st.cuda_stream))
+            self._check(rc, "pnjl_surrogate_infer")
+            return out
+        T, mu, xi = _d(T).ravel(), _d(mu).ravel(), _d(xi).ravel()
+        seeds = np.zeros((T.size, 5))
+        rc = self._L.pnjl_surrogate_infer(self.h, T.size, T.ctypes.data, mu.ctypes.data, xi.ctypes.data,
+                                          seeds.ctypes.data, 0, None)
+        self._check(rc, "pnjl_surrogate_infer")
+        return seeds
+
+    def save(self, path):
+        extra = {} if getattr(self, "judge_mask", None) is None else {"judge_mask": self.judge_mask}
+        np.savez(path, params=self.params, in_lo=self.in_lo, in_hi=self.in_hi, out_mean=self.out_mean,
+                 out_std=self.out_std, **extra)
+
+    def load(self, path):
+        z = np.load(path)
+        self.set_scaling(z["in_lo"], z["in_hi"], z["out_mean"], z["out_std"])
+        self.params = z["params"]
+        self.judge_mask = z["judge_mask"] if "judge_mask" in z.files else None
+        return self
+
+    def build_judge_mask(self, ctx, T, mu, xi, plain_roots=None, p_num=64, t_num=4, opts=None, bins=(64, 64, 8),
+                         dilate=1, multi_root=False):
+        """Ambiguity mask of the learned seed's judge, learned on probe points
+        (the training set, optionally with more random points): a bin is set
+        where the learned-seed root, the plain (DefaultSeed) root and the
+        MultiSeed (argmin-Omega) root do not all agree, or a solve failed;
+        with `multi_root`, also where the learned, HADRON and QUARK seeds reach
+        distinct roots.  The set is dilated by `dilate` bins.  Stored in
+        self.judge_mask and installed in `ctx`."""
+        from ._lib import SEED_DEFAULT, SEED_MULTI, SEED_SURROGATE
+        T, mu, xi = _d(T), _d(mu), _d(xi)
+        if plain_roots is None:
+            plain_roots = ctx.solve_batch(T, mu, xi, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=opts)["x"]
+        seeds = self.infer(T, mu, xi)
+        ctx.set_judge_mask(None, None, None)
+        q = ctx.solve_batch(T, mu, xi, policy=SEED_SURROGATE, seed=seeds, p_num=p_num, t_num=t_num, opts=opts)
+        m = ctx.solve_batch(T, mu, xi, policy=SEED_MULTI, p_num=p_num, t_num=t_num, opts=opts)
+
+        def differ(a, b):
+            return np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)), axis=1) > 1e-8
+
+        amb = differ(q["x"], m["x"]) | differ(_d(plain_roots), m["x"]) | (q["converged"] != 1) | (m["converged"] != 1)
+        if multi_root:
+            # also every probe where the HADRON and QUARK seeds reach distinct roots
+            from ._lib import PT_FIRST_ORDER, SEED_MULTIROOT3
+            r3 = ctx.solve_batch(T, mu, xi, policy=SEED_MULTIROOT3, seed=seeds, p_num=p_num, t_num=t_num, opts=opts)
+            amb |= (r3["flags"] & PT_FIRST_ORDER) != 0
+        nb = np.asarray(bins)
+        u = (np.stack([T, mu, xi], 1) - self.in_lo) / (self.in_hi - self.in_lo)
+        b = np.clip((u * nb).astype(int), 0, nb - 1)
+        mask = np.zeros(tuple(nb), dtype=np.uint8)
+        mask[b[amb, 0], b[amb, 1], b[amb, 2]] = 1
+        for _ in range(dilate):
+            grown = mask.copy()
+            for ax in range(3):
+                grown |= np.roll(mask, 1, ax) * (np.arange(nb[ax]) > 0).reshape([-1 if k == ax else 1 for k in range(3)])
+                grown |= np.roll(mask, -1, ax) * (np.arange(nb[ax]) < nb[ax] - 1).reshape(
+                    [-1 if k == ax else 1 for k in range(3)])
+            mask = grown.astype(np.uint8)
+        self.judge_mask = mask
+        ctx.set_judge_mask(mask, self.in_lo, self.in_hi)
+        return mask, amb
+
+
+def training_points(n=200_000, seed=4):
+    """(T, mu, xi) ~ U([20,300] MeV x [0,450] MeV x [-0.4,0.4]) from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(20.0, 300.0, n) / HBARC
+    mu = rng.uniform(0.0, 450.0, n) / HBARC
+    xi = rng.uniform(-0.4, 0.4, n)
+    return T, mu, xi

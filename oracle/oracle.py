@@ -20,7 +20,7 @@ _lib = None
 
 # policy codes (same values as include/pnjl_hip.h)
 SEED_DEFAULT, SEED_HADRON, SEED_QUARK, SEED_MULTI, SEED_EXPLICIT = 0, 1, 2, 3, 4
-SEED_CONTINUITY, SEED_PHASE_AWARE = 5, 6
+SEED_CONTINUITY, SEED_PHASE_AWARE, SEED_SURROGATE, SEED_MULTIROOT3 = 5, 6, 7, 8
 
 HBARC = 197.327
 
@@ -36,7 +36,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("xtol", C.c_double), ("ftol", C.c_double), ("residual_norm_max", C.c_double),
                 ("phi_tol", C.c_double), ("max_iter", C.c_int), ("trust_region_fallback", C.c_int),
                 ("auto_multiseed_fallback", C.c_int), ("surrogate_max_steps", C.c_int),
-                ("chain_len", C.c_int), ("reserved", C.c_int)]
+                ("chain_len", C.c_int), ("surrogate_judge", C.c_int)]
 
 
 _DP = C.POINTER(C.c_double)
@@ -55,7 +55,7 @@ def default_opts(**kw) -> SolveOpts:
     residual_norm_max=1e-6, iterations=1000 (NLsolve default), fallbacks on."""
     o = SolveOpts(xtol=1e-9, ftol=1e-9, residual_norm_max=1e-6, phi_tol=1e-8, max_iter=1000,
                   trust_region_fallback=1, auto_multiseed_fallback=1, surrogate_max_steps=8,
-                  chain_len=0, reserved=0)
+                  chain_len=0, surrogate_judge=0)
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -242,3 +242,20 @@ def set_boundary(rows, cep_rows):
     a, b, c = (np.ascontiguousarray(rows[:, i]) for i in range(3))
     d, e, f = (np.ascontiguousarray(cep[:, i]) for i in range(3))
     lib().oracle_set_boundary(len(a), _p(a), _p(b), _p(c), len(d), _p(d), _p(e), _p(f))
+
+
+def set_judge_mask(mask, lo=None, hi=None):
+    """Ambiguity mask (uint8 [nT, nmu, nxi] over the box [lo, hi]) of the
+    learned-seed policy's 3-seed Omega check; None clears it."""
+    L = lib()
+    L.oracle_set_judge_mask.argtypes = [C.c_int, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_uint8)]
+    if mask is None:
+        L.oracle_set_judge_mask(0, 0, 0, None, None, None)
+        return
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    rc = L.oracle_set_judge_mask(m.shape[0], m.shape[1], m.shape[2], _p(lo), _p(hi),
+                                 m.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != 0:
+        raise ValueError("oracle_set_judge_mask: bad geometry")

@@ -1104,6 +1104,160 @@ static void solve_with_seed(const prob_t* pb, const double* x0, const pnjl_solve
     out->steps = steps;
 }
 
+/* ------------------------------------------------------------------------- */
+/* learned-seed policies.  No counterpart in the reference: they compose its   */
+/* pieces -- a Newton run (nl_newton, NLsolve :newton) from the given seed     */
+/* capped at surrogate_max_steps, solve(DefaultSeed) (ImplicitSolver.jl:260-  */
+/* 328) as the fallback, and solve_multi's argmin-Omega choice (ImplicitSolver */
+/* .jl:532-559): over the learned root and the MultiSeed candidates inside    */
+/* the ambiguity mask (SURROGATE), over the learned, HADRON and HIGH_TEMP     */
+/* seeds (MULTIROOT3).                                                         */
+/* ------------------------------------------------------------------------- */
+static struct {
+    int dim[3];
+    double lo[3], hi[3];
+    uint8_t* mask;
+} g_judge;
+
+int oracle_set_judge_mask(int nT, int nmu, int nxi, const double* lo, const double* hi, const uint8_t* mask)
+{
+    free(g_judge.mask);
+    g_judge.mask = NULL;
+    if (!mask) return 0;
+    if (nT <= 0 || nmu <= 0 || nxi <= 0) return PNJL_ERR_ARG;
+    size_t nb = (size_t)nT * nmu * nxi;
+    g_judge.mask = (uint8_t*)malloc(nb);
+    memcpy(g_judge.mask, mask, nb);
+    g_judge.dim[0] = nT;
+    g_judge.dim[1] = nmu;
+    g_judge.dim[2] = nxi;
+    for (int k = 0; k < 3; k++) {
+        g_judge.lo[k] = lo[k];
+        g_judge.hi[k] = hi[k];
+    }
+    return 0;
+}
+
+/* bins of the mask over [lo, hi]; a point outside the box is always judged */
+static int judge_region(double T, double mu, double xi)
+{
+    const double v[3] = {T, mu, xi};
+    int64_t idx = 0;
+    for (int k = 0; k < 3; k++) {
+        double u = (v[k] - g_judge.lo[k]) / (g_judge.hi[k] - g_judge.lo[k]);
+        if (!(u >= 0.0 && u <= 1.0)) return 1;
+        int b = (int)(u * g_judge.dim[k]);
+        if (b >= g_judge.dim[k]) b = g_judge.dim[k] - 1;
+        idx = idx * g_judge.dim[k] + b;
+    }
+    return g_judge.mask[idx] != 0;
+}
+
+/* DualBranchScan._is_same_solution  DualBranchScan.jl:409-419 */
+static int same_root(const cand_t* a, const cand_t* b)
+{
+    return fabs(a->x[0] - b->x[0]) < 0.01 && fabs(a->x[3] - b->x[3]) < 0.01 &&
+           fabs(a->M[0] - b->M[0]) * 197.327 < 0.01 * 197.327;
+}
+
+static void solve_surrogate(const prob_t* pb, const double* seed, const pnjl_solve_opts* o, point_t* out)
+{
+    const int in_mask = g_judge.mask && judge_region(pb->T, pb->mu_vec[0], pb->xi);
+    if (in_mask && o->surrogate_judge == PNJL_JUDGE_REFERENCE) {
+        /* ambiguity region: solve(DefaultSeed) takes the point, learned seed unused */
+        double x0[5];
+        default_seed(0, pb->T, pb->mu_vec[0], x0);
+        solve_with_seed(pb, x0, o, out);
+        out->flags |= PNJL_PT_SURR_MASKED;
+        return;
+    }
+    /* 1. Newton from the learned seed, at most surrogate_max_steps steps, no fallback */
+    prob_t capped = *pb;
+    if (o->surrogate_max_steps < pb->max_iter) capped.max_iter = o->surrogate_max_steps;
+    pnjl_solve_opts oc = *o;
+    oc.trust_region_fallback = 0;
+    single_t s;
+    solve_single(&capped, seed, &oc, &s);
+    int steps = s.steps, judged;
+    if (!s.exception && s.converged) {
+        from_single(&s, out);
+        judged = 1;
+    } else {
+        /* 2. the learned seed is abandoned: solve(DefaultSeed) takes the point */
+        double x0[5];
+        default_seed(0, pb->T, pb->mu_vec[0], x0);
+        single_t p;
+        solve_single(pb, x0, o, &p);
+        steps += p.steps;
+        from_single(&p, out);
+        judged = !p.exception && p.converged;
+        if (!p.exception && !p.converged && o->auto_multiseed_fallback) {
+            point_t m;
+            if (solve_multi(pb, o, &m, &steps)) *out = m;
+        }
+        out->flags |= PNJL_PT_SURR_FALLBACK;
+    }
+    /* 3. inside the ambiguity mask: minimum Omega among this root and the
+     *    roots of the six MultiSeed candidates (get_all_seeds(MultiSeed())) */
+    if (judged && in_mask) {
+        double js[6][5];
+        multi_seeds(pb->T, pb->mu_vec[0], js);
+        point_t r0 = *out, best = *out;
+        int distinct = 0, moved = 0;
+        single_t jr[6];
+        /* HADRON and hint-QUARK first; the other four candidates only where
+         * one of them does not converge (its own MultiSeed fallback) */
+        solve_single(pb, js[0], o, &jr[0]);
+        solve_single(pb, js[1], o, &jr[1]);
+        int nk = 2;
+        for (int k = 0; k < 2; k++)
+            if (jr[k].exception || !jr[k].converged) nk = 6;
+        for (int k = 2; k < nk; k++) solve_single(pb, js[k], o, &jr[k]);
+        for (int k = 0; k < nk; k++) {
+            const single_t j = jr[k];
+            steps += j.steps;
+            if (j.exception || !j.converged) continue;
+            if (!same_root(&j.c, &r0.c)) distinct = 1;
+            if (j.c.omega < best.c.omega) {
+                from_single(&j, &best);
+                best.flags |= r0.flags & PNJL_PT_SURR_FALLBACK;
+                moved = 1;
+            }
+        }
+        *out = best;
+        if (distinct) out->flags |= PNJL_PT_FIRST_ORDER;
+        if (moved && !same_root(&best.c, &r0.c)) out->flags |= PNJL_PT_SURR_OMEGA;
+    }
+    out->steps = steps;
+}
+
+static void solve_multiroot3(const prob_t* pb, const double* seed, const pnjl_solve_opts* o, point_t* out)
+{
+    const double* sd[3] = {seed, HADRON_SEED_5, HIGH_TEMP_SEED_5};
+    single_t r[3];
+    int steps = 0, b = -1;
+    for (int k = 0; k < 3; k++) {
+        solve_single(pb, sd[k], o, &r[k]);
+        steps += r[k].steps;
+        if (!r[k].exception && r[k].converged && (b < 0 || r[k].c.omega < r[b].c.omega)) b = k;
+    }
+    if (b < 0) {
+        from_single(&r[0], out);
+        out->converged = 0;
+        out->flags |= PNJL_PT_NO_ROOT;
+    } else {
+        from_single(&r[b], out);
+        for (int i = 0; i < 3; i++)
+            for (int j = i + 1; j < 3; j++)
+                if (!r[i].exception && r[i].converged && !r[j].exception && r[j].converged &&
+                    !same_root(&r[i].c, &r[j].c))
+                    out->flags |= PNJL_PT_FIRST_ORDER;
+        if (r[0].exception || !r[0].converged) out->flags |= PNJL_PT_SURR_FALLBACK;
+        else if (b != 0 && !same_root(&r[0].c, &r[b].c)) out->flags |= PNJL_PT_SURR_OMEGA;
+    }
+    out->steps = steps;
+}
+
 static void solve_policy_point(const prob_t* pb, int policy, const double* seed, const pnjl_solve_opts* o,
                                point_t* out)
 {
@@ -1134,6 +1288,12 @@ static void solve_policy_point(const prob_t* pb, int policy, const double* seed,
         out->steps = steps;
         break;
     }
+    case PNJL_SEED_SURROGATE:
+        solve_surrogate(pb, seed, o, out);
+        break;
+    case PNJL_SEED_MULTIROOT3:
+        solve_multiroot3(pb, seed, o, out);
+        break;
     default:
         memset(out, 0, sizeof(*out));
         out->flags = PNJL_PT_EXCEPTION;
@@ -1382,7 +1542,10 @@ int oracle_solve_batch(int64_t n, const double* T, const double* mu, const doubl
         return 0;
     }
     if (policy != PNJL_SEED_DEFAULT && policy != PNJL_SEED_HADRON && policy != PNJL_SEED_QUARK &&
-        policy != PNJL_SEED_MULTI && policy != PNJL_SEED_EXPLICIT)
+        policy != PNJL_SEED_MULTI && policy != PNJL_SEED_EXPLICIT && policy != PNJL_SEED_SURROGATE &&
+        policy != PNJL_SEED_MULTIROOT3)
+        return PNJL_ERR_ARG;
+    if ((policy == PNJL_SEED_EXPLICIT || policy == PNJL_SEED_SURROGATE || policy == PNJL_SEED_MULTIROOT3) && !seed)
         return PNJL_ERR_ARG;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
     for (int64_t i = 0; i < n; i++) {
