@@ -1,15 +1,24 @@
 """Benchmark of the PNJL equilibrium-scan hot path on one MI355X.
 
-A "step" is one full scan of the workload's (T, mu, xi) grid through the
-persistent-kernel solver, with the grid already resident in HBM.  The value
-is points solved per second at |res| = ||grad_x P||_inf < 1e-10.
+A "step" is one full scan of the workload's (T, mu, xi) grid, with the grid
+already resident in HBM; the value is points solved per second at
+|res| = ||grad_x P||_inf < 1e-10 (converged, physical).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
-Workloads (grids as linspace including endpoints, MeV -> fm^-1 with 197.327):
-  c2_aniso_plain  64 x 64 x 9 (T 20-300, mu 0-450 MeV, xi -0.4..0.4 step 0.1),
-                  36,864 points, p = 64, t = 4, DefaultSeed per point
-                  (auto hint -> Newton -> trust region -> MultiSeed), iterations 40
+Workloads (grids are linspace with endpoints; MeV -> fm^-1 with 197.327):
+  c4_surrogate    (default) 256 x 256 x 17: T 20-300 MeV, mu 0-450 MeV,
+                  xi -0.4..0.4 step 0.05 = 1,114,112 points, p = 64, t = 4,
+                  iterations 40.  One step = MLP inference of the seeds
+                  (BF16 MFMA) + the learned-seed scan (8-step cutoff,
+                  DefaultSeed fallback, reference judge in the learned
+                  ambiguity mask).  Set-up, untimed and reported: 200,000
+                  plain roots at U([20,300] x [0,450] MeV x [-0.4,0.4])
+                  (default_rng(4)), Adam lr 1e-3 batch 4096 20,000 steps on
+                  the GPU, judge mask from those and 10^6 more probes
+                  (default_rng(5)).  The plain DefaultSeed scan of the same
+                  grid is timed with the same K/W and reported beside it.
+  c2_aniso_plain  64 x 64 x 9 (xi step 0.1) = 36,864 points, DefaultSeed.
 """
 import argparse
 import json
@@ -24,6 +33,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HB = 197.327
+METRIC = "PNJL gap points solved/s at |res|<1e-10: surrogate-seeded vs plain Newton"
 PEAK_FP64_TFLOPS = 78.6
 # FP64 FLOPs of one node x flavour evaluation of the fused integrand (value,
 # gradient, Hessian and thermodynamic sums), counted on the gfx950 ISA of the
@@ -31,44 +41,87 @@ PEAK_FP64_TFLOPS = 78.6
 C_NODE = 360.0
 
 
-def grid_c2():
-    xis = np.round(np.arange(-0.4, 0.41, 0.1), 10)
-    T, mu, xi = np.meshgrid(np.linspace(20, 300, 64), np.linspace(0, 450, 64), xis, indexing="ij")
+def grid(nT, nmu, xis):
+    T, mu, xi = np.meshgrid(np.linspace(20, 300, nT), np.linspace(0, 450, nmu), np.asarray(xis, float),
+                            indexing="ij")
     return T.ravel() / HB, mu.ravel() / HB, xi.ravel()
 
 
 WORKLOADS = {
-    "c2_aniso_plain": dict(grid=grid_c2, p_num=64, t_num=4, policy=0, max_iter=40,
+    "c4_surrogate": dict(grid=lambda: grid(256, 256, np.round(np.arange(-0.4, 0.4001, 0.05), 10)), p_num=64,
+                         t_num=4, max_iter=40, learned=True,
+                         desc="256x256x17 (T 20-300 MeV, mu 0-450 MeV, xi -0.4..0.4 step 0.05), p=64 t=4, "
+                              "learned seed (MLP 3-128-128-128-4, BF16 MFMA) -> Newton, 8-step cutoff, "
+                              "DefaultSeed fallback, reference judge; FP64; |res|<1e-10"),
+    "c2_aniso_plain": dict(grid=lambda: grid(64, 64, np.round(np.arange(-0.4, 0.41, 0.1), 10)), p_num=64, t_num=4,
+                           max_iter=40, learned=False,
                            desc="64x64x9 (T 20-300 MeV, mu 0-450 MeV, xi -0.4..0.4), p=64 t=4, "
                                 "DefaultSeed per point, FP64, |res|<1e-10"),
 }
 
 
-def cpu_baseline(w, T, mu, xi, n_sample=12000, threads=None):
-    """The oracle (plain-C restatement of the reference algorithm) on a seeded
-    sample of the same workload, OpenMP over points."""
+def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
+    """The oracle (plain-C restatement of the reference's DefaultSeed solve)
+    on a seeded sample of the same grid, OpenMP over points."""
     from oracle import oracle as O
     threads = threads or min(os.cpu_count() or 1, 16)
     rng = np.random.default_rng(20261202)
     idx = rng.choice(T.size, min(n_sample, T.size), replace=False)
     opts = O.default_opts(ftol=1e-10, max_iter=w["max_iter"])
     t0 = time.perf_counter()
-    r = O.solve_batch(T[idx], mu[idx], xi[idx], policy=w["policy"], p_num=w["p_num"], t_num=w["t_num"],
+    r = O.solve_batch(T[idx], mu[idx], xi[idx], policy=O.SEED_DEFAULT, p_num=w["p_num"], t_num=w["t_num"],
                       opts=opts, nthreads=threads)
     dt = time.perf_counter() - t0
     solved = int(np.sum((r["converged"] == 1) & (r["residual_norm"] < 1e-10)))
     return dict(value=solved / dt, unit="points/s", cores=threads, kind="port",
-                sample=f"{idx.size} seeded-random points of the same grid (rng 20261202), {threads} OpenMP threads, "
-                       f"{dt:.2f} s")
+                sample=f"{idx.size} seeded-random points of the same grid (rng 20261202), plain DefaultSeed "
+                       f"solve, {threads} OpenMP threads, {dt:.2f} s wall")
+
+
+def train_learned_seed(ctx, w, opts):
+    from julia_relaxtime_amd.surrogate import Surrogate, training_points
+    t0 = time.perf_counter()
+    T, mu, xi = training_points(200_000, seed=4)
+    tr = ctx.solve_batch(T, mu, xi, policy=0, p_num=w["p_num"], t_num=w["t_num"], opts=opts)
+    ok = tr["converged"] == 1
+    s = Surrogate(0, init_seed=4)
+    s.fit_output_scaling(tr["x"][ok])
+    loss = s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=20000, batch=4096, lr=1e-3, seed=4)
+    train_ms = s.last_ms
+    Tp, mup, xip = training_points(1_000_000, seed=5)
+    s.build_judge_mask(ctx, np.r_[T, Tp], np.r_[mu, mup], np.r_[xi, xip], p_num=w["p_num"], t_num=w["t_num"],
+                       opts=opts)
+    info = dict(train_points=int(ok.sum()), adam_steps=20000, batch=4096, lr=1e-3, train_kernel_ms=train_ms,
+                final_loss=float(np.mean(loss[-200:])), judge_mask_fraction=float(s.judge_mask.mean()),
+                setup_s=time.perf_counter() - t0)
+    return s, info
+
+
+def timed_steps(step, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    scan_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        scan_ms.append(step())
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, scan_ms
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2_aniso_plain", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c4_surrogate", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--launch", default=None,
+                    help="lanes,chunk,waves_per_cu[,tail_lanes] for both scans instead of autotuning "
+                         "(profiling runs)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -78,68 +131,110 @@ def main():
     import torch
 
     from julia_relaxtime_amd import _lib
-    from julia_relaxtime_amd.pnjl import load_phase_boundary
 
     w = WORKLOADS[args.workload]
     T, mu, xi = w["grid"]()
     n = T.size
     dev = torch.device("cuda", 0)
     ctx = _lib.Context(0)
-    rows, cep, _ = load_phase_boundary(0.0)
-    ctx.set_phase_boundary(rows, cep)
     opts = _lib.default_opts(ftol=1e-10, max_iter=w["max_iter"])
     Td, mud, xid = (torch.tensor(a, device=dev) for a in (T, mu, xi))
-    out = None
     stream = torch.cuda.current_stream(dev)
-
-    def step():
-        nonlocal out
-        out = ctx.solve_batch(Td, mud, xid, policy=w["policy"], p_num=w["p_num"], t_num=w["t_num"], opts=opts,
-                              out=out, stream=stream, sync=True)
-        return ctx.last_kernel_ms()
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    kms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kms.append(step())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-
-    conv = out["converged"].cpu().numpy()
-    res = out["residual_norm"].cpu().numpy()
-    passes = out["passes"].cpu().numpy().astype(np.float64)
-    solved = int(np.sum((conv == 1) & (res < 1e-10)))
-    ms_per_step = elapsed / args.steps * 1e3
     t_eff = np.where(xi == 0.0, 1, w["t_num"])
-    node_evals = float(np.sum(passes * w["p_num"] * t_eff))
-    flops = node_evals * C_NODE
-    kms_avg = float(np.mean(kms))
-    achieved = flops / (kms_avg * 1e-3) / 1e12
+
+    def scan_flops(out):
+        passes = out["passes"].cpu().numpy().astype(np.float64)
+        return float(np.sum(passes * w["p_num"] * t_eff)) * C_NODE
+
+    fixed = None
+    if args.launch:
+        v = [int(t) for t in args.launch.split(",")] + [0]
+        fixed = dict(lanes_per_point=v[0], chunk=v[1], waves_per_cu=v[2], tail_lanes_per_point=v[3])
+    # plain DefaultSeed scan, its launch geometry autotuned on the full grid
+    plain_cfg = fixed or ctx.autotune(T, mu, xi, policy=_lib.SEED_DEFAULT, p_num=w["p_num"], t_num=w["t_num"],
+                                      opts=opts)[0]
+    plain = {}
+
+    def plain_step():
+        ctx.set_launch_cfg(**plain_cfg)
+        plain["out"] = ctx.solve_batch(Td, mud, xid, policy=_lib.SEED_DEFAULT, p_num=w["p_num"], t_num=w["t_num"],
+                                       opts=opts, out=plain.get("out"), stream=stream, sync=True)
+        return float(np.sum(ctx.last_scan_ms()))
+
+    ms_plain, scan_plain = timed_steps(plain_step, args.steps, args.warmup)
+    P = {k: v.cpu().numpy() for k, v in plain["out"].items()}
+    solved_plain = int(np.sum((P["converged"] == 1) & (P["residual_norm"] < 1e-10)))
+    flops_plain = scan_flops(plain["out"])
+
+    extra = {}
+    if w["learned"]:
+        model, extra = train_learned_seed(ctx, w, opts)
+        ctx.set_judge_mask(model.judge_mask, model.in_lo, model.in_hi)
+        state = {"seeds": torch.empty((n, 5), dtype=torch.float64, device=dev)}
+        cfg = fixed or ctx.autotune(T, mu, xi, policy=_lib.SEED_SURROGATE, seed=model.infer(T, mu, xi),
+                                    p_num=w["p_num"], t_num=w["t_num"], opts=opts)[0]
+
+        def step():
+            ctx.set_launch_cfg(**cfg)
+            model.infer(Td, mud, xid, out=state["seeds"], stream=stream)
+            state["out"] = ctx.solve_batch(Td, mud, xid, policy=_lib.SEED_SURROGATE, seed=state["seeds"],
+                                           p_num=w["p_num"], t_num=w["t_num"], opts=opts, out=state.get("out"),
+                                           stream=stream, sync=True)
+            state["stage_ms"] = ctx.last_scan_ms()
+            return float(np.sum(state["stage_ms"]))
+
+        ms_step, scan_ms = timed_steps(step, args.steps, args.warmup)
+        Q = {k: v.cpu().numpy() for k, v in state["out"].items()}
+        flops = scan_flops(state["out"])
+        solved = int(np.sum((Q["converged"] == 1) & (Q["residual_norm"] < 1e-10)))
+        same = np.max(np.abs(Q["x"] - P["x"]) / np.maximum(1, np.abs(P["x"])), axis=1) < 1e-9
+        seed_err = np.abs(state["seeds"].cpu().numpy() - Q["x"]).max(axis=1)
+        extra.update(
+            plain_points_per_s=solved_plain / (ms_plain * 1e-3), plain_ms_per_step=ms_plain,
+            speedup_vs_plain=(solved / (ms_step * 1e-3)) / (solved_plain / (ms_plain * 1e-3)),
+            same_root_as_plain=float(same.mean()), points_not_on_plain_root=int((~same).sum()),
+            fallback_fraction=float(np.mean((Q["flags"] & _lib.PT_SURR_FALLBACK) != 0)),
+            judged_fraction=float(np.mean((Q["flags"] & _lib.PT_SURR_MASKED) != 0)),
+            seed_to_root_err_median=float(np.median(seed_err)), seed_to_root_err_p90=float(np.quantile(seed_err, 0.9)),
+            node_passes=int(Q["passes"].sum()), plain_node_passes=int(P["passes"].sum()),
+            plain_scan_kernel_ms=float(np.mean(scan_plain)), plain_launch=plain_cfg,
+            scan_stage_ms=[round(float(v), 3) for v in state["stage_ms"]],
+            plain_fp64_tflops=flops_plain / (np.mean(scan_plain) * 1e-3) / 1e12)
+        seed_policy = "SurrogateSeed(judge=reference)"
+    else:
+        ms_step, scan_ms, flops, solved = ms_plain, scan_plain, flops_plain, solved_plain
+        Q = P
+        seed_policy = "DefaultSeed"
+
+    scan_avg = float(np.mean(scan_ms))
+    achieved = flops / (scan_avg * 1e-3) / 1e12
     line = {
-        "metric": "PNJL gap points solved/s at |res|<1e-10",
-        "value": solved / (ms_per_step * 1e-3),
+        "metric": METRIC,
+        "value": solved / (ms_step * 1e-3),
         "unit": "points/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "points": n, "solved": solved, "p_num": w["p_num"],
-                   "t_num": w["t_num"], "seed_policy": "DefaultSeed", "detail": w["desc"],
-                   "launch": ctx.launch_cfg(), "kernel_ms": kms_avg,
-                   "mean_iterations": float(np.mean(out["iterations"].cpu().numpy())),
-                   "fallback_fraction": float(np.mean(out["flags"].cpu().numpy() != 0))},
+        "config": dict(workload=args.workload, points=n, solved=solved, p_num=w["p_num"], t_num=w["t_num"],
+                       seed_policy=seed_policy, detail=w["desc"], launch=ctx.launch_cfg(),
+                       mean_iterations=float(np.mean(Q["iterations"])), **extra),
+        # dominant kernel: the persistent scan kernel (all its launches in one step)
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "flops_per_launch": flops, "node_evals_per_launch": node_evals, "flops_per_node": C_NODE},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "pnjl::scan_kernel",
+                     "kernel_ms_per_step": scan_avg, "flops_per_step": flops, "flops_per_node": C_NODE},
     }
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["config"]["copy_probe_bytes"] = int(src.numel() * 4)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, T, mu, xi)
     print(json.dumps(line), flush=True)

@@ -224,16 +224,23 @@ typedef struct pnjl_launch_cfg {
     int lanes_per_point;  /* 1,2,4,8,16,32,64 -> 64/lanes points per wavefront */
     int chunk;            /* points taken from the work queue per pop        */
     int waves_per_cu;     /* persistent grid size = 256 CUs x waves_per_cu   */
-    int reserved;
+    int tail_lanes_per_point; /* lanes per point of the later pipeline stages
+                             (fallback attempts, MultiSeed candidates,
+                             judge runs: few, long-running items); 0 = same
+                             as lanes_per_point                         */
 } pnjl_launch_cfg;
 
 int pnjl_set_launch_cfg(pnjl_ctx* ctx, const pnjl_launch_cfg* cfg);
 int pnjl_get_launch_cfg(const pnjl_ctx* ctx, pnjl_launch_cfg* cfg);
 
-/* Autotuner: times the candidate launch geometries (lanes_per_point x chunk x
- * waves_per_cu) on the first `sample` points of the batch (host pointers as
- * in pnjl_solve_fixedmu_batch), keeps the fastest in the context and returns
- * it in *best (and its time per point in *ns_per_point).                   */
+/* Autotuner: times launch geometries on the first `sample` points of the
+ * batch (host pointers as in pnjl_solve_fixedmu_batch; each candidate warmed
+ * once, best of two timed runs) -- first every lanes_per_point (points per
+ * wavefront and nodes per lane) at chunk 1 and 12 waves/CU, then chunk
+ * {1,2,4} x waves/CU {8,12,16} for the two best lane counts, then the later
+ * stages' lanes {4,16,32} -- keeps the fastest in the context and returns it
+ * in *best (and its pipeline time per point in *ns_per_point).  No
+ * counterpart in the reference.                                            */
 int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* mu, const double* xi,
                   const double* seed, int seed_policy, int p_num, int t_num,
                   const pnjl_solve_opts* opts, pnjl_launch_cfg* best, double* ns_per_point);
@@ -241,6 +248,12 @@ int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* 
 /* Kernel time of the last solve/residual launch in milliseconds (HIP events
  * recorded around the kernel on the call's stream).                        */
 double pnjl_last_kernel_ms(const pnjl_ctx* ctx);
+
+/* Durations (ms, HIP events on the call's stream) of each persistent-kernel
+ * launch of the last solve -- one per pipeline stage that ran -- into
+ * ms[0..max-1]; returns the number of launches (waits for the last one).
+ * No counterpart in the reference: measurement of the GPU pipeline.        */
+int pnjl_last_scan_ms(pnjl_ctx* ctx, double* ms, int max);
 
 /* ------------------------------------------------------------------------ */
 /* learned seed (no counterpart in the reference: it feeds                  */

@@ -48,10 +48,11 @@ class SolveOpts(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [("lanes_per_point", C.c_int), ("chunk", C.c_int), ("waves_per_cu", C.c_int),
-                ("reserved", C.c_int)]
+                ("tail_lanes_per_point", C.c_int)]
 
     def as_dict(self):
-        return dict(lanes_per_point=self.lanes_per_point, chunk=self.chunk, waves_per_cu=self.waves_per_cu)
+        return dict(lanes_per_point=self.lanes_per_point, chunk=self.chunk, waves_per_cu=self.waves_per_cu,
+                    tail_lanes_per_point=self.tail_lanes_per_point)
 
 
 _DP = C.POINTER(C.c_double)
@@ -103,6 +104,8 @@ def load():
     L.pnjl_last_error.restype = C.c_char_p
     L.pnjl_last_kernel_ms.argtypes = [vp]
     L.pnjl_last_kernel_ms.restype = C.c_double
+    L.pnjl_last_scan_ms.argtypes = [vp, _DP, C.c_int]
+    L.pnjl_last_scan_ms.restype = C.c_int
     L.pnjl_gauleg.argtypes = [C.c_double, C.c_double, C.c_int, _DP, _DP]
     L.pnjl_nodes.argtypes = [vp, C.c_int, C.c_int, _DP, _DP, _DP]
     L.pnjl_set_phase_boundary.argtypes = [vp, C.c_int, _DP, _DP, _DP, C.c_int, _DP, _DP, _DP]
@@ -220,8 +223,8 @@ class Context:
         self._check(self._L.pnjl_set_judge_mask(self.h, m.shape[0], m.shape[1], m.shape[2], _dptr(lo), _dptr(hi),
                                                 m.ctypes.data_as(C.POINTER(C.c_uint8))), "pnjl_set_judge_mask")
 
-    def set_launch_cfg(self, lanes_per_point=4, chunk=4, waves_per_cu=8):
-        cfg = LaunchCfg(lanes_per_point, chunk, waves_per_cu, 0)
+    def set_launch_cfg(self, lanes_per_point=4, chunk=4, waves_per_cu=8, tail_lanes_per_point=0):
+        cfg = LaunchCfg(lanes_per_point, chunk, waves_per_cu, tail_lanes_per_point)
         self._check(self._L.pnjl_set_launch_cfg(self.h, C.byref(cfg)), "pnjl_set_launch_cfg")
 
     def launch_cfg(self):
@@ -231,6 +234,14 @@ class Context:
 
     def last_kernel_ms(self):
         return self._L.pnjl_last_kernel_ms(self.h)
+
+    def last_scan_ms(self):
+        """Durations (ms) of the persistent-kernel launches of the last solve."""
+        ms = np.zeros(8)
+        k = self._L.pnjl_last_scan_ms(self.h, _dptr(ms), ms.size)
+        if k < 0:
+            self._check(k, "pnjl_last_scan_ms")
+        return ms[:min(k, ms.size)].copy()
 
     # ------------------------------------------------------------------
     def solve_batch(self, T, mu, xi, policy=SEED_DEFAULT, seed=None, p_num=64, t_num=8, opts=None, out=None,

@@ -18,7 +18,7 @@ SOURCES = ["pnjl_lib.hip", "pnjl_mlp.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
-    # packed FP32 instructions stay off (they have given wrong results on this pool)
+    # packed FP32 instructions stay off (some of their forms have given wrong results)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 

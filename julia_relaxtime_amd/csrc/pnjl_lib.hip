@@ -19,6 +19,8 @@
 
 // device work counters of the staged pipeline (u64 words)
 constexpr int kQueueWords = 16;
+// persistent-kernel launches timed per solve (the staged pipeline has at most 6)
+constexpr int kMaxTimedScans = 8;
 
 struct pnjl_ctx {
     int device = 0;
@@ -40,6 +42,9 @@ struct pnjl_ctx {
     std::vector<size_t> stage_bytes;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    // begin/end events of each persistent-kernel launch of the last solve
+    hipEvent_t ev_scan[2 * kMaxTimedScans] = {};
+    int n_scans = 0;
 };
 
 namespace {
@@ -127,6 +132,19 @@ void build_meshes(int pn, int tn, std::vector<double>& p, std::vector<double>& c
         }
 }
 
+// exp_tab / log_tab tables (pnjl_device.h), in x87 extended precision and
+// rounded once to double
+void math_tables(double* t)
+{
+    for (int j = 0; j < 64; j++) t[pnjl::kExpTab + j] = (double)exp2l((long double)j / 64.0L);
+    for (int j = 0; j < 64; j++) {
+        const long double c = 1.0L + ((long double)j + 0.5L) / 64.0L;
+        const double invc = (double)(1.0L / c);
+        t[pnjl::kLogTab + 2 * j] = invc;
+        t[pnjl::kLogTab + 2 * j + 1] = (double)(-logl((long double)invc));
+    }
+}
+
 int get_tab(pnjl_ctx* ctx, int pn, int tn, double** out)
 {
     auto key = std::make_pair(pn, tn);
@@ -137,15 +155,17 @@ int get_tab(pnjl_ctx* ctx, int pn, int tn, double** out)
     }
     std::vector<double> p, c, C;
     build_meshes(pn, tn, p, c, C);
-    // device table: pp[pn] | cc[tn] | C[pn*tn] | Cs[pn]
-    std::vector<double> tab((size_t)pn + tn + (size_t)pn * tn + pn);
-    for (int j = 0; j < pn; j++) tab[j] = p[j] * p[j];
-    for (int l = 0; l < tn; l++) tab[pn + l] = c[(size_t)l * pn] * c[(size_t)l * pn];
-    for (size_t k = 0; k < (size_t)pn * tn; k++) tab[pn + tn + k] = C[k];
+    // device table: math tables[kMathTab] | pp[pn] | cc[tn] | C[pn*tn] | Cs[pn]
+    std::vector<double> tab(pnjl::kMathTab + (size_t)pn + tn + (size_t)pn * tn + pn);
+    math_tables(tab.data());
+    double* nt = tab.data() + pnjl::kMathTab;
+    for (int j = 0; j < pn; j++) nt[j] = p[j] * p[j];
+    for (int l = 0; l < tn; l++) nt[pn + l] = c[(size_t)l * pn] * c[(size_t)l * pn];
+    for (size_t k = 0; k < (size_t)pn * tn; k++) nt[pn + tn + k] = C[k];
     for (int j = 0; j < pn; j++) {
         double s = 0.0;
         for (int l = 0; l < tn; l++) s += C[(size_t)j + (size_t)l * pn];
-        tab[pn + tn + (size_t)pn * tn + j] = s;
+        nt[pn + tn + (size_t)pn * tn + j] = s;
     }
     double* d = nullptr;
     HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(double)));
@@ -186,9 +206,11 @@ int stage_buf(pnjl_ctx* ctx, size_t k, size_t bytes, void** out)
 
 bool valid_lpp(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64; }
 
-void launch_scan(const pnjl::ScanArgs& A, int lpp, int blocks, size_t lds, hipStream_t st)
+void launch_scan(pnjl_ctx* ctx, const pnjl::ScanArgs& A, int lpp, int blocks, size_t lds, hipStream_t st)
 {
     using namespace pnjl;
+    const int k = ctx->n_scans < kMaxTimedScans ? ctx->n_scans++ : -1;
+    if (k >= 0) (void)hipEventRecord(ctx->ev_scan[2 * k], st);
     switch (lpp) {
     case 1: hipLaunchKernelGGL(scan_kernel<1>, dim3(blocks), dim3(256), lds, st, A); break;
     case 2: hipLaunchKernelGGL(scan_kernel<2>, dim3(blocks), dim3(256), lds, st, A); break;
@@ -198,6 +220,7 @@ void launch_scan(const pnjl::ScanArgs& A, int lpp, int blocks, size_t lds, hipSt
     case 32: hipLaunchKernelGGL(scan_kernel<32>, dim3(blocks), dim3(256), lds, st, A); break;
     default: hipLaunchKernelGGL(scan_kernel<64>, dim3(blocks), dim3(256), lds, st, A); break;
     }
+    if (k >= 0) (void)hipEventRecord(ctx->ev_scan[2 * k + 1], st);
 }
 
 int check_policy(int p)
@@ -213,6 +236,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     double* tab = nullptr;
     int rc = get_tab(ctx, pn, tn, &tab);
     if (rc) return rc;
+    ctx->n_scans = 0;
     pnjl::ScanArgs A{};
     A.P = ctx->dp;
     A.n_points = n;
@@ -268,18 +292,20 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     // the later pipeline stages run the full persistent grid: size the
     // per-group scratch for the larger of the two grids
     const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / 4);
-    const int64_t cap_groups = std::max<int64_t>(groups, (int64_t)fullblocks * 256 / cfg.lanes_per_point);
+    const int tail = cfg.tail_lanes_per_point > 0 ? cfg.tail_lanes_per_point : cfg.lanes_per_point;
+    const int64_t cap_groups = std::max<int64_t>(
+        groups, (int64_t)fullblocks * 256 / std::min(cfg.lanes_per_point, tail));
     A.n_groups = cap_groups;
     rc = ensure_scratch(ctx, (size_t)cap_groups * pnjl::SCRATCH_FIELDS * sizeof(double));
     if (rc) return rc;
     A.scratch = ctx->scratch;
-    size_t lds = sizeof(double) * ((size_t)pn + tn + (size_t)pn * tn + pn);
+    size_t lds = sizeof(double) * (pnjl::kMathTab + (size_t)pn + tn + (size_t)pn * tn + pn);
     if (chains) {
         A.stage = pnjl::STAGE_NONE;
         hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, ctx->queue, kQueueWords);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
-        launch_scan(A, cfg.lanes_per_point, blocks, lds, stream);
+        launch_scan(ctx, A, cfg.lanes_per_point, blocks, lds, stream);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev1, stream));
         return PNJL_OK;
@@ -326,7 +352,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
         S.item_list = nullptr;
         S.queue = q + 2;
         S.rec_out = (double*)recM;
-        launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+        launch_scan(ctx, S, cfg.lanes_per_point, fullblocks, lds, stream);
     } else {
         if (surr) {
             S.stage = pnjl::STAGE_SURR;
@@ -350,7 +376,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
                     S.jb = (int32_t*)jb;
                 }
             }
-            launch_scan(S, cfg.lanes_per_point, blocks, lds, stream);
+            launch_scan(ctx, S, cfg.lanes_per_point, blocks, lds, stream);
             HIPCHK(ctx, hipGetLastError());
         }
         S.stage = pnjl::STAGE_PRIMARY;
@@ -361,7 +387,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
         S.rec_out = (double*)recP;
         S.next_list = (int32_t*)listM;
         S.next_count = q + 5;
-        launch_scan(S, cfg.lanes_per_point, surr ? fullblocks : blocks, lds, stream);
+        launch_scan(ctx, S, surr ? tail : cfg.lanes_per_point, surr ? fullblocks : blocks, lds, stream);
         HIPCHK(ctx, hipGetLastError());
         S.stage = pnjl::STAGE_MULTI;
         S.mult = 6;
@@ -373,7 +399,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
         S.next_count = nullptr;
         S.judge_mask = nullptr;
         S.jslot = nullptr;
-        launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+        launch_scan(ctx, S, tail, fullblocks, lds, stream);
         if (omega_judge) {
             HIPCHK(ctx, hipGetLastError());
             S.stage = pnjl::STAGE_JUDGE;
@@ -385,7 +411,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
             S.jb = (int32_t*)jb;
             S.judge2_list = (int32_t*)listJ2;
             S.judge2_count = q + 8;
-            launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+            launch_scan(ctx, S, tail, fullblocks, lds, stream);
             HIPCHK(ctx, hipGetLastError());
             S.stage = pnjl::STAGE_JUDGE2;
             S.mult = pnjl::JUDGE_SEEDS - 2;
@@ -396,7 +422,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
             S.jb = nullptr;
             S.judge2_list = nullptr;
             S.judge2_count = nullptr;
-            launch_scan(S, cfg.lanes_per_point, fullblocks, lds, stream);
+            launch_scan(ctx, S, tail, fullblocks, lds, stream);
         }
     }
     HIPCHK(ctx, hipGetLastError());
@@ -489,8 +515,10 @@ int pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->num_cu = prop.multiProcessorCount;
-    if (hipMalloc(&c->queue, kQueueWords * sizeof(unsigned long long)) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
-        hipEventCreate(&c->ev1) != hipSuccess) {
+    bool ev_ok = true;
+    for (int i = 0; i < 2 * kMaxTimedScans; i++) ev_ok = ev_ok && hipEventCreate(&c->ev_scan[i]) == hipSuccess;
+    if (!ev_ok || hipMalloc(&c->queue, kQueueWords * sizeof(unsigned long long)) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
         return PNJL_ERR_HIP;
     }
@@ -501,23 +529,39 @@ int pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out)
 void pnjl_ctx_destroy(pnjl_ctx* c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    hipDeviceSynchronize();
-    for (auto& kv : c->tabs) hipFree(kv.second);
+    // teardown: errors cannot be reported any more and are ignored
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : c->tabs) (void)hipFree(kv.second);
     for (void* b : c->stage)
-        if (b) hipFree(b);
-    if (c->scratch) hipFree(c->scratch);
-    if (c->queue) hipFree(c->queue);
-    if (c->bnd) hipFree(c->bnd);
-    if (c->jmask) hipFree(c->jmask);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
+        if (b) (void)hipFree(b);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->queue) (void)hipFree(c->queue);
+    if (c->bnd) (void)hipFree(c->bnd);
+    if (c->jmask) (void)hipFree(c->jmask);
+    for (int i = 0; i < 2 * kMaxTimedScans; i++)
+        if (c->ev_scan[i]) (void)hipEventDestroy(c->ev_scan[i]);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
     delete c;
 }
 
 const char* pnjl_last_error(const pnjl_ctx* c) { return c ? c->err.c_str() : "no context"; }
 
 double pnjl_last_kernel_ms(const pnjl_ctx* c) { return c ? c->last_ms : 0.0; }
+
+int pnjl_last_scan_ms(pnjl_ctx* c, double* ms, int max)
+{
+    if (!c) return PNJL_ERR_ARG;
+    const int n = c->n_scans;
+    if (n > 0) HIPCHK(c, hipEventSynchronize(c->ev_scan[2 * n - 1]));
+    for (int i = 0; i < n && i < max && ms; i++) {
+        float t = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev_scan[2 * i], c->ev_scan[2 * i + 1]));
+        ms[i] = t;
+    }
+    return n;
+}
 
 int pnjl_gauleg(double a, double b, int n, double* nodes, double* weights)
 {
@@ -620,6 +664,7 @@ int pnjl_set_judge_mask(pnjl_ctx* ctx, int nT, int nmu, int nxi, const double* l
 int pnjl_set_launch_cfg(pnjl_ctx* ctx, const pnjl_launch_cfg* cfg)
 {
     if (!ctx || !cfg || !valid_lpp(cfg->lanes_per_point) || cfg->chunk < 1 || cfg->waves_per_cu < 1 ||
+        (cfg->tail_lanes_per_point != 0 && !valid_lpp(cfg->tail_lanes_per_point)) ||
         cfg->waves_per_cu > 64)
         return set_err(ctx, PNJL_ERR_ARG, "pnjl_set_launch_cfg: invalid geometry");
     ctx->cfg = *cfg;
@@ -781,7 +826,7 @@ int pnjl_gap_residual_batch(pnjl_ctx* ctx, int64_t n, const double* x, const dou
     R.scratch = ctx->scratch;
     const int LPP = 16;
     int64_t blocks = (n * LPP + 255) / 256;
-    size_t lds = sizeof(double) * ((size_t)pn + tn + (size_t)pn * tn + pn);
+    size_t lds = sizeof(double) * (pnjl::kMathTab + (size_t)pn + tn + (size_t)pn * tn + pn);
     HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
     hipLaunchKernelGGL(pnjl::residual_kernel<LPP>, dim3((unsigned)blocks), dim3(256), lds, st, R);
     HIPCHK(ctx, hipGetLastError());
@@ -818,26 +863,46 @@ int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* 
     out.residual_norm = rs.data();
     out.iterations = it.data();
     out.converged = cv.data();
-    const int lpps[] = {1, 2, 4, 8, 16, 64};
-    const int chunks[] = {1, 4, 16};
-    const int wpcs[] = {4, 8, 16};
     pnjl_launch_cfg bcfg = ctx->cfg;
     double bms = 1e300;
-    for (int l : lpps)
-        for (int c : chunks)
-            for (int w : wpcs) {
-                pnjl_launch_cfg cfg{l, c, w, 0};
-                // warm, then time
-                int rc = solve_impl(ctx, sample, T, mu, xi, seed, policy, pn, tn, opts, &out, 0, nullptr, cfg);
-                if (rc) return rc;
-                rc = solve_impl(ctx, sample, T, mu, xi, seed, policy, pn, tn, opts, &out, 0, nullptr, cfg);
-                if (rc) return rc;
-                double t = ctx->last_ms;
-                if (t < bms) {
-                    bms = t;
-                    bcfg = cfg;
-                }
-            }
+    auto trial = [&](const pnjl_launch_cfg& cfg, double* t_out) -> int {
+        // warm once, then keep the better of two timed runs (HIP events
+        // around the pipeline)
+        double t = 1e300;
+        for (int rep = 0; rep < 3; rep++) {
+            int rc = solve_impl(ctx, sample, T, mu, xi, seed, policy, pn, tn, opts, &out, 0, nullptr, cfg);
+            if (rc) return rc;
+            if (rep > 0) t = std::min(t, ctx->last_ms);
+        }
+        if (t < bms) {
+            bms = t;
+            bcfg = cfg;
+        }
+        if (t_out) *t_out = t;
+        return PNJL_OK;
+    };
+    // 1) lanes per point (points per wavefront = 64 / lanes; nodes per lane =
+    //    p_num * t_eff / lanes) at queue chunk 1 and 12 waves/CU
+    const int lanes[] = {1, 2, 4, 8, 16, 32, 64};
+    double tl[7];
+    for (int i = 0; i < 7; i++)
+        if (int rc = trial(pnjl_launch_cfg{lanes[i], 1, 12, 0}, &tl[i])) return rc;
+    // 2) queue chunk and resident waves per CU for the two best lane counts
+    int i1 = 0, i2 = -1;
+    for (int i = 1; i < 7; i++)
+        if (tl[i] < tl[i1]) i1 = i;
+    for (int i = 0; i < 7; i++)
+        if (i != i1 && (i2 < 0 || tl[i] < tl[i2])) i2 = i;
+    for (int i : {i1, i2})
+        for (int c : {1, 2, 4})
+            for (int w : {8, 12, 16})
+                if (!(c == 1 && w == 12))
+                    if (int rc = trial(pnjl_launch_cfg{lanes[i], c, w, 0}, nullptr)) return rc;
+    // 3) lanes of the later (fallback / MultiSeed / judge) stages
+    const pnjl_launch_cfg b2 = bcfg;
+    for (int t : {4, 16, 32})
+        if (t != b2.lanes_per_point)
+            if (int rc = trial(pnjl_launch_cfg{b2.lanes_per_point, b2.chunk, b2.waves_per_cu, t}, nullptr)) return rc;
     ctx->cfg = bcfg;
     if (best) *best = bcfg;
     if (ns_per_point) *ns_per_point = bms * 1e6 / (double)sample;

@@ -640,13 +640,14 @@ int pnjl_surrogate_create(int device, uint64_t init_seed, pnjl_surrogate** out)
 void pnjl_surrogate_destroy(pnjl_surrogate* s)
 {
     if (!s) return;
-    hipSetDevice(s->device);
-    hipDeviceSynchronize();
+    // teardown: errors cannot be reported any more and are ignored
+    (void)hipSetDevice(s->device);
+    (void)hipDeviceSynchronize();
     for (void* p : {(void*)s->P, (void*)s->m, (void*)s->v, (void*)s->grad, (void*)s->wbuf, (void*)s->partial,
                     (void*)s->loss_partial, (void*)s->loss_dev, (void*)s->data, (void*)s->stage})
-        if (p) hipFree(p);
-    if (s->e0) hipEventDestroy(s->e0);
-    if (s->e1) hipEventDestroy(s->e1);
+        if (p) (void)hipFree(p);
+    if (s->e0) (void)hipEventDestroy(s->e0);
+    if (s->e1) (void)hipEventDestroy(s->e1);
     delete s;
 }
 

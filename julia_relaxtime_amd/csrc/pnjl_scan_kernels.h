@@ -231,17 +231,18 @@ template <int LPP>
 __device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, double M, double T, double mu,
                                              double xi, double Phi, double Phib, int gl, FlavSums& acc)
 {
-    const double* pp = lds;
-    const double* cc = lds + pn;
-    const double* Cm = lds + pn + tn;
+    const double* pp = lds + kMathTab;
+    const double* cc = pp + pn;
+    const double* Cm = pp + pn + tn;
     const double* Cs = Cm + pn * tn;
     const double invT = 1.0 / T;
+    const double K2mu = exp(-2.0 * mu * invT);
     const double M2 = M * M;
     flav_zero(acc);
     if (xi == 0.0) {
         // E does not depend on cos(theta): the cos nodes collapse into Cs
 #pragma unroll 1
-        for (int j = gl; j < pn; j += LPP) node_accumulate(pp[j], Cs[j], M2, mu, invT, Phi, Phib, acc);
+        for (int j = gl; j < pn; j += LPP) node_accumulate(lds, pp[j], Cs[j], M2, mu, invT, K2mu, Phi, Phib, acc);
     } else {
 #pragma unroll 1
         for (int l = 0; l < tn; l++) {
@@ -250,7 +251,7 @@ __device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, 
 #pragma unroll 1
             for (int j = gl; j < pn; j += LPP) {
                 double q = fma(xc, pp[j], pp[j]);
-                node_accumulate(q, Cl[j], M2, mu, invT, Phi, Phib, acc);
+                node_accumulate(lds, q, Cl[j], M2, mu, invT, K2mu, Phi, Phib, acc);
             }
         }
     }
@@ -258,8 +259,10 @@ __device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, 
 }
 
 // Node pass at state x: u, s, and d unless M_d == M_u (then d aliases u).
+// Not inlined: the node loop gets the whole register budget, and the caller's
+// live state is saved once per pass instead of competing with the loop.
 template <int LPP>
-__device__ __forceinline__ void node_pass(const DevParams& P, const double* lds, int pn, int tn, const Scr& s,
+__device__ __noinline__ void node_pass(const DevParams& P, const double* lds, int pn, int tn, const Scr& s,
                                           const double* x, double T, double mu, double xi, bool sym, int gl)
 {
     double M[3];
@@ -1335,11 +1338,14 @@ __device__ __forceinline__ bool next_point(const ScanArgs& A, St& st, int gl, bo
     return true;
 }
 
+// 3 waves per SIMD: every function fits 168 VGPRs; the node pass is a
+// separate function so that its loop keeps all of them (no spills), the
+// control path spills a little once per trip.
 template <int LPP>
-__global__ void __launch_bounds__(256) scan_kernel(ScanArgs A)
+__global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
 {
     extern __shared__ double lds[];
-    const int tab_len = A.pn + A.tn + A.pn * A.tn + A.pn;
+    const int tab_len = kMathTab + A.pn + A.tn + A.pn * A.tn + A.pn;
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) lds[i] = A.tab[i];
     __syncthreads();
 
@@ -1406,10 +1412,10 @@ struct ResArgs {
 };
 
 template <int LPP>
-__global__ void __launch_bounds__(256) residual_kernel(ResArgs R)
+__global__ void __launch_bounds__(256, 3) residual_kernel(ResArgs R)
 {
     extern __shared__ double lds[];
-    const int tab_len = R.pn + R.tn + R.pn * R.tn + R.pn;
+    const int tab_len = kMathTab + R.pn + R.tn + R.pn * R.tn + R.pn;
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) lds[i] = R.tab[i];
     __syncthreads();
     const int gl = threadIdx.x % LPP;

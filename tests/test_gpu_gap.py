@@ -84,7 +84,7 @@ def test_policy_parity_small_grid(gpu_ctx, oracle, policy, name):
     o = oracle.solve_batch(T, mu, xi, policy=policy, p_num=64, t_num=4, nthreads=8,
                            opts=oracle.default_opts(ftol=1e-10, max_iter=40))
     assert o["converged"].mean() > 0.9
-    ok = assert_parity(r, o)
+    ok = assert_parity(r, o, selection=(policy == 3))
     # integer by-products: identical except at points whose residual sits on the ftol edge.
     # MultiSeed: several candidates reach the same root and argmin Omega then
     # ties to rounding, so the winning candidate (and its count) may differ.
@@ -212,8 +212,8 @@ def test_launch_geometry_does_not_change_results(gpu_ctx):
         for o in outs[1:]:
             for k in ("x", "omega", "iterations", "converged", "entropy"):
                 assert np.array_equal(o[k], outs[0][k], equal_nan=True), k
-        for lpp in (1, 2, 8, 16, 64):
-            gpu_ctx.set_launch_cfg(lpp, 4, 8)
+        for lpp, tail in ((1, 0), (2, 0), (8, 0), (16, 0), (64, 0), (4, 64), (8, 32)):
+            gpu_ctx.set_launch_cfg(lpp, 4, 8, tail)
             o = gpu_ctx.solve_batch(T, mu, xi, policy=0, p_num=64, t_num=4, opts=opts)
             ok = outs[0]["converged"].astype(bool)
             assert np.array_equal(o["converged"], outs[0]["converged"])
