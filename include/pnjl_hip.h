@@ -291,6 +291,78 @@ int  pnjl_surrogate_loss_grad(pnjl_surrogate* s, int64_t n, const double* T, con
 int  pnjl_surrogate_infer(pnjl_surrogate* s, int64_t n, const double* T, const double* mu,
                           const double* xi, double* seeds, int flags, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* anisotropic one-loop polarization (src/relaxtime)                        */
+/* All momenta, energies, masses, chemical potentials and T in fm^-1.        */
+/* Results reproduce the reference's fixed quadrature (same segmentation,   */
+/* nodes and skip-non-finite rule), not the exact integrals.                 */
+/* ------------------------------------------------------------------------ */
+#define PNJL_CHANNEL_P 1          /* pseudoscalar (PolarizationCache.jl:32-35 codes) */
+#define PNJL_CHANNEL_S 2          /* scalar                                          */
+
+/* One polarization query: the arguments of
+ *   polarization_aniso(channel, k0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar,
+ *                      xi, A1_value, A2_value, num_s_quark)
+ *     src/relaxtime/PolarizationAniso.jl:22-24
+ * and, with width = 1, of polarization_with_width (:88-91) / the complex
+ * p0 = k0 + i gamma/2 of polarization_complex (:127-134).                  */
+typedef struct pnjl_pol_query {
+    int    channel;               /* PNJL_CHANNEL_P or PNJL_CHANNEL_S        */
+    int    num_s;                 /* num_s_quark (1 -> k0-symmetrised B0)     */
+    int    width;                 /* 0: polarization_aniso, 1: _with_width   */
+    int    reserved;
+    double k0, gamma, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, A1, A2;
+} pnjl_pol_query;
+
+/* Pi for n queries -> re[n], im[n].  Host pointers unless flags has
+ * PNJL_FLAG_DEVICE_PTRS.  A bad channel fails the call with PNJL_ERR_ARG
+ * (the reference throws ArgumentError, PolarizationAniso.jl:62).
+ * replaces: polarization_aniso / polarization_with_width / polarization_complex
+ *           src/relaxtime/PolarizationAniso.jl:22-134                      */
+int pnjl_polarization_batch(pnjl_ctx* ctx, int64_t n, const pnjl_pol_query* q, double* re, double* im,
+                            int flags, void* stream);
+
+/* B0(lambda, k, m1, mu1, m2, mu2, T; Phi, Phibar) for n rows of
+ * args[n*9] = (lambda, k, m1, mu1, m2, mu2, T, Phi, Phibar) (host pointers).
+ * replaces: B0  src/relaxtime/OneLoopIntegrals.jl:489-503                  */
+int pnjl_B0_batch(pnjl_ctx* ctx, int64_t n, const double* args, double* re, double* im);
+
+/* B0_correction(lambda, k, m1, m2, mu1, mu2, T, Phi, Phibar, xi) for n rows
+ * of args[n*10] in that order (host pointers).
+ * replaces: B0_correction  src/relaxtime/OneLoopIntegralsAniso.jl:396-413  */
+int pnjl_B0_correction_batch(pnjl_ctx* ctx, int64_t n, const double* args, double* re, double* im);
+
+/* A(m, mu, T, Phi, Phibar, nodes_p, weights_p) for n rows of
+ * args[n*5] = (m, mu, T, Phi, Phibar); with nc > 0 instead
+ * A_aniso(m, mu, T, Phi, Phibar, xi, nodes_p, weights_p, nodes_cos,
+ * weights_cos) with args[n*6] = (m, mu, T, Phi, Phibar, xi).  Host pointers.
+ * replaces: A  src/relaxtime/OneLoopIntegrals.jl:531-543
+ *           A_aniso  src/relaxtime/OneLoopIntegralsAniso.jl:446-467       */
+int pnjl_A_batch(pnjl_ctx* ctx, int64_t n, const double* args, int np, const double* nodes_p,
+                 const double* weights_p, int nc, const double* nodes_cos, const double* weights_cos,
+                 double* out);
+
+/* One (T, mu, xi) state of the polarization grid: the converged gap
+ * solution (masses and Polyakov loops) at that point.                      */
+typedef struct pnjl_pol_point {
+    double T, mu, xi, m_u, m_s, Phi, Phib;
+} pnjl_pol_point;
+
+/* The Pi(k0, k) grid of n points: for each point A_aniso(m_u) and
+ * A_aniso(m_s) on np momentum nodes (GL on [0, 10] fm^-1) x nc cos nodes
+ * (GL on [-1, 1]), then Pi for the pairs (u ubar, m_u m_u, num_s 0),
+ * (u sbar, m_u m_s, num_s 1), (s sbar, m_s m_s, num_s 2), both channels
+ * (P, S), mu1 = mu2 = mu, at every k0[i] x k[j]:
+ *   re/im[((pt*3 + pair)*2 + channel)*nk0*nk + i*nk + j]   (channel 0 = P).
+ * Host pointers unless PNJL_FLAG_DEVICE_PTRS.  The A and Pi kernels are
+ * captured once into a hipGraph per grid shape and replayed on later calls
+ * (inputs are copied into graph-owned buffers first).  No counterpart in
+ * the reference as one call: it batches the polarization_aniso calls of
+ * TotalPropagator.jl:440-535 / MesonMass.jl:50-216 over a whole grid.     */
+int pnjl_polarization_grid(pnjl_ctx* ctx, int npts, const pnjl_pol_point* pts, int nk0, const double* k0,
+                           int nk, const double* k, int np, int nc, double* re, double* im, int flags,
+                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,52 +15,12 @@
 #include <vector>
 
 #include "../../include/pnjl_hip.h"
+#include "pnjl_ctx.h"
 #include "pnjl_scan_kernels.h"
 
-// device work counters of the staged pipeline (u64 words)
-constexpr int kQueueWords = 16;
-// persistent-kernel launches timed per solve (the staged pipeline has at most 6)
-constexpr int kMaxTimedScans = 8;
+using namespace pnjl_host;
 
-struct pnjl_ctx {
-    int device = 0;
-    int num_cu = 256;
-    pnjl_model_params params{};
-    pnjl::DevParams dp{};
-    std::string err;
-    pnjl_launch_cfg cfg{4, 4, 8, 0};
-    std::map<std::pair<int, int>, double*> tabs;
-    double* scratch = nullptr;
-    size_t scratch_bytes = 0;
-    unsigned long long* queue = nullptr;
-    double* bnd = nullptr;
-    int nb = 0;
-    uint8_t* jmask = nullptr;
-    int jdim[3] = {0, 0, 0};
-    double jlo[3] = {0, 0, 0}, jhi[3] = {1, 1, 1};
-    std::vector<void*> stage;   // device staging buffers for host-pointer calls
-    std::vector<size_t> stage_bytes;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0;
-    // begin/end events of each persistent-kernel launch of the last solve
-    hipEvent_t ev_scan[2 * kMaxTimedScans] = {};
-    int n_scans = 0;
-};
-
-namespace {
-
-int set_err(pnjl_ctx* c, int code, const std::string& m)
-{
-    if (c) c->err = m;
-    return code;
-}
-
-#define HIPCHK(ctx, call)                                                                     \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return set_err(ctx, PNJL_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+namespace pnjl_host {
 
 // n-point Gauss-Legendre rule on [-1,1], ascending (Newton iteration on the
 // Legendre three-term recurrence from the Tricomi-type initial guesses).
@@ -456,7 +416,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     return PNJL_OK;
 }
 
-}  // namespace
+}  // namespace pnjl_host
 
 extern "C" {
 
@@ -539,6 +499,7 @@ void pnjl_ctx_destroy(pnjl_ctx* c)
     if (c->queue) (void)hipFree(c->queue);
     if (c->bnd) (void)hipFree(c->bnd);
     if (c->jmask) (void)hipFree(c->jmask);
+    polar_release(c);
     for (int i = 0; i < 2 * kMaxTimedScans; i++)
         if (c->ev_scan[i]) (void)hipEventDestroy(c->ev_scan[i]);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

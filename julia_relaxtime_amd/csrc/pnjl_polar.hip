@@ -1,0 +1,1109 @@
+// pnjl_polar.hip -- the anisotropic one-loop polarization Pi(k0, k) on gfx950.
+//
+// One GPU thread evaluates one B0 / B0_correction / polarization value with
+// the reference's fixed quadrature: the same breakpoints (poles of the k = 0
+// integrand, roots of the k > 0 logarithm, imaginary-part intervals), the
+// same 16/32-node power-clustered and 33-node double-exponential rules, the
+// same principal-value subtraction and the same skip-non-finite rule, so the
+// results reproduce the reference's numbers, not the exact integrals.
+// Quadrature tables sit in LDS.  For the grid call, the threads of a
+// wavefront share (point, pair, k0) and take 64 consecutive k, so their
+// segment structures differ only smoothly.
+//
+// Follows src/relaxtime/OneLoopIntegrals.jl, OneLoopIntegralsAniso.jl,
+// PolarizationAniso.jl, src/integration/IntervalQuadratureStrategies.jl and
+// src/QuarkDistribution{,_Aniso}.jl (file:line at each function).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "pnjl_ctx.h"
+
+using namespace pnjl_host;
+
+// No contraction of a*b + c into FMA here: the reference evaluates these
+// expressions unfused, and some of them cancel exactly (lambda^2 + m^2 - m'^2
+// with m = m'), where a fused product leaves a 1e-17 residue instead of 0.
+#pragma clang fp contract(off)
+
+namespace polar {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kEpsK = 1.0e-9;        // OneLoopIntegrals.jl:23
+constexpr double kEpsSeg = 1.0e-12;     // OneLoopIntegrals.jl:24
+constexpr double kPvGapRel = 1e-6;      // OneLoopIntegrals.jl:35
+constexpr double kAlpha = 0.35;         // IntervalQuadratureStrategies.jl:35
+constexpr double kDeH = 0.15;           // IntervalQuadratureStrategies.jl:36
+constexpr int kTabLen = 354;            // see tab_* offsets below
+
+enum { NONE = 0, LEFT = 1, RIGHT = 2, BOTH = 3 };
+enum { PLUS = 0, MINUS = 1 };           // B0 signs (f+(E), f+(-E) = 1 - f-(E))
+enum { QUARK = 0, ANTIQUARK = 1 };      // first-order anisotropy signs
+
+// table layout: x16 w16 x32 w32 | pl_t16 pl_tp16 pl_t32 pl_tp32 | pr_t16 pr_tp16 pr_t32 pr_tp32 | de_phi de_phip
+struct Tabs {
+    const double *x16, *w16, *x32, *w32;
+    const double *pl_t16, *pl_tp16, *pl_t32, *pl_tp32;
+    const double *pr_t16, *pr_tp16, *pr_t32, *pr_tp32;
+    const double *de_phi, *de_phip;
+};
+
+__device__ __forceinline__ Tabs make_tabs(const double* t)
+{
+    Tabs q;
+    q.x16 = t; q.w16 = t + 16; q.x32 = t + 32; q.w32 = t + 64;
+    q.pl_t16 = t + 96; q.pl_tp16 = t + 112; q.pl_t32 = t + 128; q.pl_tp32 = t + 160;
+    q.pr_t16 = t + 192; q.pr_tp16 = t + 208; q.pr_t32 = t + 224; q.pr_tp32 = t + 256;
+    q.de_phi = t + 288; q.de_phip = t + 321;
+    return q;
+}
+
+struct Phys {
+    double Lambda, Nc;
+};
+
+// Julia eps(x): spacing of doubles at |x|
+__device__ __forceinline__ double jeps(double x)
+{
+    x = fabs(x);
+    if (!(x >= 2.2250738585072014e-308)) return 4.9406564584124654e-324;
+    return ldexp(1.0, ilogb(x) - 52);
+}
+
+__device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
+__device__ __forceinline__ double clamp_e(double e) { return e < 1e-200 ? 1e-200 : (e > 1e200 ? 1e200 : e); }
+
+// PNJL distributions and their antiderivatives  QuarkDistribution.jl:14-99
+__device__ __forceinline__ double dist_ratio(double e, double c1, double c2)
+{
+    double e2 = e * e, e3 = e2 * e;
+    return (c1 * e + 2 * c2 * e2 + e3) / (1 + 3 * c1 * e + 3 * c2 * e2 + e3);
+}
+
+__device__ __forceinline__ double quark_f(double E, double mu, double T, double Phi, double Phib)
+{
+    return dist_ratio(clamp_e(exp(-(E - mu) * (1 / T))), Phi, Phib);
+}
+
+__device__ __forceinline__ double antiquark_f(double E, double mu, double T, double Phi, double Phib)
+{
+    return dist_ratio(clamp_e(exp(-(E + mu) * (1 / T))), Phib, Phi);
+}
+
+__device__ __forceinline__ double anti_deriv(double e, double c1, double c2, double T)
+{
+    double e2 = e * e, e3 = e2 * e;
+    double d = 1 + 3 * c1 * e + 3 * c2 * e2 + e3;
+    return -T * log(d <= 1e-300 ? 1e-300 : d) / 3;
+}
+
+// distribution_value_b0 / distribution_integral_b0  OneLoopIntegrals.jl:68-104
+__device__ __forceinline__ double dist_b0(int sg, double E, double mu, double T, double Phi, double Phib)
+{
+    return sg == PLUS ? quark_f(E, mu, T, Phi, Phib) : 1.0 - antiquark_f(E, mu, T, Phi, Phib);
+}
+
+__device__ double dist_int_b0(int sg, double a, double b, double mu, double T, double Phi, double Phib)
+{
+    const double beta = 1 / T;
+    if (sg == PLUS)
+        return anti_deriv(clamp_e(exp(-(b - mu) * beta)), Phi, Phib, T) -
+               anti_deriv(clamp_e(exp(-(a - mu) * beta)), Phi, Phib, T);
+    return (b - a) - (anti_deriv(clamp_e(exp(-(b + mu) * beta)), Phib, Phi, T) -
+                      anti_deriv(clamp_e(exp(-(a + mu) * beta)), Phib, Phi, T));
+}
+
+// energy_cutoff / internal_momentum  OneLoopIntegrals.jl:37-48
+__device__ __forceinline__ double e_cut(const Phys& P, double m)
+{
+    double mp = m > 0 ? m : 0.0;
+    return sqrt(mp * mp + P.Lambda * P.Lambda);
+}
+
+__device__ __forceinline__ double p_int(double E, double m)
+{
+    double mp = m > 0 ? m : 0.0;
+    return sqrt(E * E - mp * mp);
+}
+
+// integrate_hybrid_interval  IntervalQuadratureStrategies.jl:194-310
+template <class F>
+__device__ double rule_power(const F& f, double a, double b, const double* t, const double* tp, const double* W, int n)
+{
+    double len = b - a, total = 0.0;
+    for (int i = 0; i < n; i++) {
+        double x = a + len * t[i];
+        double w = W[i] * len * tp[i];
+        double v = f(x);
+        if (isfinite(v)) total += w * v;
+    }
+    return total;
+}
+
+template <class F>
+__device__ double hybrid(const F& f, const Tabs& q, double a, double b, int pos, int n)
+{
+    if (!(isfinite(a) && isfinite(b)) || b <= a) return 0.0;
+    if (pos == BOTH) {
+        double half = (b - a) / 2, center = (a + b) / 2, total = 0.0;
+        for (int i = 0; i < 33; i++) {
+            double x = center + half * q.de_phi[i];
+            double w = half * q.de_phip[i];
+            double v = f(x);
+            if (isfinite(v)) total += w * v;
+        }
+        return total;
+    }
+    if (pos == RIGHT)
+        return n == 16 ? rule_power(f, a, b, q.pr_t16, q.pr_tp16, q.w16, 16)
+                       : rule_power(f, a, b, q.pr_t32, q.pr_tp32, q.w32, 32);
+    return n == 16 ? rule_power(f, a, b, q.pl_t16, q.pl_tp16, q.w16, 16)
+                   : rule_power(f, a, b, q.pl_t32, q.pl_tp32, q.w32, 32);
+}
+
+// ---- B~0 at k = 0  OneLoopIntegrals.jl:117-224 ------------------------------
+struct B0K0 {
+    int sg;
+    double lam, m, dterm, mu, T, Phi, Phib;
+    __device__ double operator()(double E) const
+    {
+        double p = p_int(E, m);
+        double dist = dist_b0(sg, E, mu, T, Phi, Phib);
+        double den = lam * E + dterm;
+        if (!isfinite(p) || !isfinite(dist) || !isfinite(den)) return 0.0;
+        if (fabs(den) < kEpsSeg) return 0.0;
+        double v = p * dist / den;
+        return isfinite(v) ? v : 0.0;
+    }
+};
+
+struct B0Numer {
+    int sg;
+    double m, mu, T, Phi, Phib;
+    __device__ double operator()(double E) const
+    {
+        double p = p_int(E, m);
+        double dist = dist_b0(sg, E, mu, T, Phi, Phib);
+        if (!isfinite(p) || !isfinite(dist)) return 0.0;
+        double v = p * dist;
+        return isfinite(v) ? v : 0.0;
+    }
+};
+
+struct B0Regular {
+    B0Numer num;
+    double E0, f0, Emin, Emax;
+    __device__ double operator()(double E) const
+    {
+        double d = E - E0;
+        if (fabs(d) < 64 * jeps(E0)) {
+            double del = fmax(1e-7 * (Emax - Emin), 64 * jeps(E0));
+            double El = fmax(Emin, E0 - del), Er = fmin(Emax, E0 + del);
+            return Er > El ? (num(Er) - num(El)) / (Er - El) : 0.0;
+        }
+        return (num(E) - f0) / d;
+    }
+};
+
+__device__ void tilde_B0_k0(const Phys& P, const Tabs& q, int sg, double lam, double m_in, double mp_in, double mu,
+                            double T, double Phi, double Phib, double& re, double& im)
+{
+    const double m = m_in > 0 ? m_in : 0.0, mp = mp_in > 0 ? mp_in : 0.0;
+    const double Emin = m, Emax = e_cut(P, m);
+    const double dterm = (lam * lam + m * m - mp * mp) / 2.0;
+    const B0K0 f{sg, lam, m, dterm, mu, T, Phi, Phib};
+    // singularity_k_zero  :144-157
+    const double E0 = -dterm / lam;
+    bool has = false;
+    if (isfinite(E0)) {
+        double tol = 1e-8 * fmax(1.0, fabs(E0));
+        has = (E0 > Emin - tol) && (E0 < Emax + tol);
+    }
+    if (!has) {
+        re = hybrid(f, q, Emin, Emax, NONE, 16) * 2.0;
+        im = 0.0 / lam;
+        return;
+    }
+    const B0Numer num{sg, m, mu, T, Phi, Phib};
+    const double f0 = num(E0);
+    double pv;
+    const double etol = 1e-10 * fmax(1.0, Emax - Emin);
+    if (fabs(E0 - Emin) <= etol || fabs(Emax - E0) <= etol) {
+        double gap = fmax(kPvGapRel * (Emax - Emin), 64 * jeps(E0));
+        double left_end = fmax(Emin, E0 - gap), right_start = fmin(Emax, E0 + gap);
+        pv = 0.0;
+        if (left_end > Emin) pv += hybrid(f, q, Emin, left_end, NONE, 16);
+        if (right_start < Emax) pv += hybrid(f, q, right_start, Emax, NONE, 16);
+    } else {
+        const B0Regular g{num, E0, f0, Emin, Emax};
+        double regular = hybrid(g, q, Emin, Emax, NONE, 32);
+        double logterm = f0 * (log(fabs(Emax - E0)) - log(fabs(Emin - E0)));
+        pv = (regular + logterm) / lam;
+    }
+    const double p0 = p_int(E0, m);
+    re = pv * 2.0;
+    im = 2.0 * kPi * p0 * dist_b0(sg, E0, mu, T, Phi, Phib) / lam;
+}
+
+// ---- B~0 at k > 0  OneLoopIntegrals.jl:228-476 ------------------------------
+struct B0Kpos {
+    int sg;
+    double lam, k, m, mp, mu, T, Phi, Phib;
+    __device__ double operator()(double E) const
+    {
+        double p = p_int(E, m);
+        double dist = dist_b0(sg, E, mu, T, Phi, Phib);
+        double lpE = lam + E;
+        double common = lpE * lpE - mp * mp;
+        double pmk = p - k, ppk = p + k;
+        double num = common - pmk * pmk;
+        double den = common - ppk * ppk;
+        if (!isfinite(p) || !isfinite(dist) || !isfinite(num) || !isfinite(den)) return 0.0;
+        if (fabs(num) < kEpsSeg && fabs(den) < kEpsSeg) return 0.0;
+        if (fabs(den) < kEpsSeg) den = (den == 0.0) ? kEpsSeg : jsign(den) * kEpsSeg;
+        double ratio = fabs(num / den);
+        if (!isfinite(ratio)) return 0.0;
+        ratio = fmax(ratio, 1e-300);
+        double lt = log(ratio);
+        if (!isfinite(lt)) return 0.0;
+        double v = dist * lt;
+        return isfinite(v) ? v : 0.0;
+    }
+};
+
+// singularity_k_positive  :239-330 -> number of intervals in iv (0..2)
+__device__ int sing_kpos(double lam, double k, double m, double mp, double Emin, double Emax, double iv[2][2])
+{
+    int n = 0;
+    const double a = lam * lam - k * k;
+    const double d0 = k * k - lam * lam;
+    const double denom = 2.0 * d0;
+    if (fabs(a) < kEpsSeg) {
+        if (lam > 0.0 && mp > m) {
+            double m2d = mp * mp - m * m;
+            double Ec = m2d / (4.0 * lam) + lam * m * m / m2d;
+            if (Ec < Emax) {
+                iv[0][0] = fmax(Ec, Emin);
+                iv[0][1] = Emax;
+                n = 1;
+            }
+        }
+        return n;
+    }
+    const double A = lam * lam - k * k + m * m - mp * mp;
+    const double disc = A * A + 4.0 * m * m * d0;
+    if (disc < 0.0) return 0;
+    const double sq = sqrt(disc);
+    double E1 = (lam * A - k * sq) / denom, E2 = (lam * A + k * sq) / denom;
+    if (E1 > E2) {
+        double t = E1;
+        E1 = E2;
+        E2 = t;
+    }
+    if (E2 - E1 < kEpsSeg) return 0;
+    if (a > 0.0) {
+        if (E2 > Emin && E1 < Emax) {
+            double c1 = fmax(E1, Emin), c2 = fmin(E2, Emax);
+            if (c2 > c1) {
+                iv[0][0] = c1;
+                iv[0][1] = c2;
+                n = 1;
+            }
+        }
+        return n;
+    }
+    if (E1 > Emin) {
+        double c = fmin(E1, Emax);
+        if (c > Emin) {
+            iv[n][0] = Emin;
+            iv[n][1] = c;
+            n++;
+        }
+    }
+    if (E2 < Emax) {
+        double c = fmax(E2, Emin);
+        if (Emax > c) {
+            iv[n][0] = c;
+            iv[n][1] = Emax;
+            n++;
+        }
+    }
+    return n;
+}
+
+// integrate_piecewise_hybrid  :335-421
+template <class F>
+__device__ double piecewise(const F& f, const Tabs& q, double Emin, double Emax, int niv, const double iv[2][2])
+{
+    if (niv == 0) return hybrid(f, q, Emin, Emax, NONE, 16);
+    double E1 = __builtin_nan(""), E2 = __builtin_nan("");
+    if (niv == 1) {
+        E1 = iv[0][0];
+        E2 = iv[0][1];
+    } else {
+        const double a1 = iv[0][0], b1 = iv[0][1], a2 = iv[1][0], b2 = iv[1][1];
+        const double tol = 1e-12 * fmax(1.0, Emax - Emin);
+        if (fabs(a1 - Emin) <= tol) E1 = b1;
+        else if (fabs(a2 - Emin) <= tol) E1 = b2;
+        if (fabs(b1 - Emax) <= tol) E2 = a1;
+        else if (fabs(b2 - Emax) <= tol) E2 = a2;
+    }
+    if (isfinite(E1) && isfinite(E2)) {
+        double total = 0.0;
+        if (E1 > Emin) total += hybrid(f, q, Emin, E1, RIGHT, 32);
+        if (E2 > E1) total += hybrid(f, q, E1, E2, BOTH, 32);
+        if (Emax > E2) total += hybrid(f, q, E2, Emax, LEFT, 16);
+        return total;
+    }
+    // general path: sorted unique breakpoints (insertion sort of <= 6 values)
+    double pts[6];
+    int np = 0;
+    pts[np++] = Emin;
+    for (int i = 0; i < niv; i++) {
+        pts[np++] = iv[i][0];
+        pts[np++] = iv[i][1];
+    }
+    pts[np++] = Emax;
+    for (int i = 1; i < np; i++)
+        for (int j = i; j > 0 && pts[j] < pts[j - 1]; j--) {
+            double t = pts[j];
+            pts[j] = pts[j - 1];
+            pts[j - 1] = t;
+        }
+    int u = 0;
+    for (int i = 0; i < np; i++)
+        if (u == 0 || pts[i] != pts[u - 1]) pts[u++] = pts[i];
+    const int nseg = u - 1;
+    if (nseg <= 0) return 0.0;
+    if (nseg == 1) return hybrid(f, q, pts[0], pts[1], NONE, 16);
+    double total = 0.0;
+    for (int i = 1; i <= nseg; i++) {
+        double a = pts[i - 1], b = pts[i];
+        if (!(isfinite(a) && isfinite(b)) || b <= a) continue;
+        int pos = (i == 1) ? RIGHT : (i == nseg ? LEFT : BOTH);
+        total += hybrid(f, q, a, b, pos, i == nseg ? 16 : 32);
+    }
+    return total;
+}
+
+__device__ void tilde_B0_kpos(const Phys& P, const Tabs& q, int sg, double lam, double k, double m_in, double mp_in,
+                              double mu, double T, double Phi, double Phib, double& re, double& im)
+{
+    const double m = m_in > 0 ? m_in : 0.0, mp = mp_in > 0 ? mp_in : 0.0;
+    const double Emin = m, Emax = e_cut(P, m);
+    double iv[2][2];
+    const int niv = sing_kpos(lam, k, m, mp, Emin, Emax, iv);
+    const B0Kpos f{sg, lam, k, m, mp, mu, T, Phi, Phib};
+    const double real = piecewise(f, q, Emin, Emax, niv, iv);
+    double imag = 0.0;
+    if (niv > 0) {
+        for (int i = 0; i < niv; i++) imag += dist_int_b0(sg, iv[i][0], iv[i][1], mu, T, Phi, Phib);
+        imag *= kPi * jsign(lam);
+    }
+    re = real / k;
+    im = imag / k;
+}
+
+__device__ void tilde_B0(const Phys& P, const Tabs& q, int sg, double lam, double k, double m, double mp, double mu,
+                         double T, double Phi, double Phib, double& re, double& im)
+{
+    if (fabs(k) < kEpsK) tilde_B0_k0(P, q, sg, lam, m, mp, mu, T, Phi, Phib, re, im);
+    else tilde_B0_kpos(P, q, sg, lam, k, m, mp, mu, T, Phi, Phib, re, im);
+}
+
+// B0  OneLoopIntegrals.jl:489-503
+__device__ void B0(const Phys& P, const Tabs& q, double lam, double k, double m1, double mu1, double m2, double mu2,
+                   double T, double Phi, double Phib, double& re, double& im)
+{
+    double r, i;
+    re = 0.0;
+    im = 0.0;
+    tilde_B0(P, q, PLUS, -lam, k, m1, m2, mu1, T, Phi, Phib, r, i);
+    re = r;
+    im = i;
+    tilde_B0(P, q, MINUS, lam, k, m1, m2, mu1, T, Phi, Phib, r, i);
+    re -= r;
+    im -= i;
+    tilde_B0(P, q, PLUS, lam, k, m2, m1, mu2, T, Phi, Phib, r, i);
+    re += r;
+    im += i;
+    tilde_B0(P, q, MINUS, -lam, k, m2, m1, mu2, T, Phi, Phib, r, i);
+    re -= r;
+    im -= i;
+}
+
+// ---- first-order anisotropy  OneLoopIntegralsAniso.jl ---------------------
+// quark_df_dE / antiquark_df_dE / correction_cos_theta_coefficient
+//   QuarkDistribution_Aniso.jl:14-86
+__device__ double corr_coef(int sg, double p, double m, double mu, double T, double Phi, double Phib, double xi)
+{
+    const double E = sqrt(p * p + m * m);
+    const double coeff = 0.5 * xi * (p * p) / E;
+    const double beta = 1 / T;
+    const double c1 = sg == QUARK ? Phi : Phib, c2 = sg == QUARK ? Phib : Phi;
+    const double arg = sg == QUARK ? -(E - mu) * beta : -(E + mu) * beta;
+    const double e = clamp_e(exp(arg));
+    const double e2 = e * e, e3 = e2 * e;
+    const double num = c1 * e + 2 * c2 * e2 + e3;
+    const double den = 1 + 3 * c1 * e + 3 * c2 * e2 + e3;
+    const double dnum = -(c1 * e + 4 * c2 * e2 + 3 * e3);
+    const double dden = -(3 * c1 * e + 6 * c2 * e2 + 3 * e3);
+    double dfdx = dnum * den - num * dden;
+    dfdx /= den * den;
+    return coeff * (beta * dfdx);
+}
+
+// compute_coefficients  OneLoopIntegralsAniso.jl:50-61 (k > 0 branch)
+struct BcKpos {
+    int sg;
+    double lam, k, m, mp, mu, T, Phi, Phib, xi;
+    bool imag;
+    __device__ double operator()(double E) const
+    {
+        const double dc = lam * lam + 2 * lam * E + m * m - mp * mp - k * k;
+        const double p = p_int(E, m);
+        const double cx = 2 * p * k;
+        double tool;
+        if (imag) {
+            tool = kPi * dc * dc / (cx * cx * cx);
+        } else {
+            double t1 = -2 * dc / (cx * cx);
+            double t2 = dc * dc / (cx * cx * cx);
+            tool = t1 + t2 * log(fabs((cx + dc) / (cx - dc)));
+        }
+        return 2.0 * p * tool * corr_coef(sg, p, m, mu, T, Phi, Phib, xi);
+    }
+};
+
+// find_roots_AB  :207-272 -> number of roots (ascending, unique)
+__device__ int roots_AB(double lam, double k, double m, double mp, double Emin, double Emax, double r[2])
+{
+    const double C = lam * lam + m * m - mp * mp - k * k;
+    const double a = k * k - lam * lam, b = -lam * C, c = -(k * k * m * m + C * C / 4);
+    if (fabs(a) < 1e-14) {
+        if (fabs(b) < 1e-14) return 0;
+        double E = -c / b;
+        if (E >= Emin && E <= Emax && E >= m) {
+            double p = sqrt(fmax(0.0, E * E - m * m));
+            double A = 2 * k * p, B = lam * lam + 2 * lam * E + m * m - mp * mp - k * k;
+            if (fabs(A + B) < 1e-10 || fabs(A - B) < 1e-10) {
+                r[0] = E;
+                return 1;
+            }
+        }
+        return 0;
+    }
+    const double D = b * b - 4 * a * c;
+    if (D < 0) return 0;
+    const double sD = sqrt(D);
+    const double E12[2] = {(-b - sD) / (2 * a), (-b + sD) / (2 * a)};
+    int n = 0;
+    for (int i = 0; i < 2; i++) {
+        double E = E12[i];
+        if (E >= m && E >= Emin && E <= Emax) {
+            double p = sqrt(E * E - m * m);
+            double A = 2 * k * p, B = lam * lam + 2 * lam * E + m * m - mp * mp - k * k;
+            if (fabs(A + B) < 1e-10 || fabs(A - B) < 1e-10) r[n++] = E;
+        }
+    }
+    if (n == 2) {
+        if (r[0] > r[1]) {
+            double t = r[0];
+            r[0] = r[1];
+            r[1] = t;
+        }
+        if (r[0] == r[1]) n = 1;
+    }
+    return n;
+}
+
+// tilde_B0_correction_k_zero  :133-190
+__device__ void tilde_Bc_k0(const Phys& P, const Tabs& q, int sg, double lam, double m, double mp, double mu, double T,
+                            double Phi, double Phib, double xi, double& re, double& im)
+{
+    const double Emin = m, Emax = e_cut(P, m);
+    const double cE = 2.0 * lam;
+    const double dc = lam * lam + m * m - mp * mp;
+    bool has = false;
+    double E0 = 0.0;
+    if (cE != 0.0) {
+        double v = -dc / cE;
+        if (isfinite(v) && v >= Emin && v <= Emax) {
+            has = true;
+            E0 = v;
+        }
+    }
+    auto numer = [&](double E) {
+        double p = p_int(E, m);
+        return (4.0 / 3.0) * p * corr_coef(sg, p, m, mu, T, Phi, Phib, xi);
+    };
+    double sum = 0.0;
+    const double A0 = has ? numer(E0) : 0.0;
+    for (int i = 0; i < 32; i++) {
+        double E = ((Emax - Emin) * q.x32[i] + (Emax + Emin)) / 2.0;
+        double w = q.w32[i] * (Emax - Emin) / 2.0;
+        double den = cE * E + dc;
+        double v = numer(E) - A0;
+        sum += w * ((isfinite(den) && isfinite(v)) ? v / den : 0.0);
+    }
+    if (has) sum += A0 / cE * (log(fabs(cE * Emax + dc)) - log(fabs(cE * Emin + dc)));
+    re = sum;
+    // imag_integrand_k_zero  :98-112
+    const double Ep = -dc / cE;
+    if (!(Ep >= m && Ep <= e_cut(P, m))) {
+        im = 0.0;
+    } else {
+        const double pp = p_int(Ep, m);
+        im = kPi * (4.0 / 3.0) * pp / cE * corr_coef(sg, pp, m, mu, T, Phi, Phib, xi);
+    }
+}
+
+// tilde_B0_correction_k_positive  :307-375
+__device__ void tilde_Bc_kpos(const Phys& P, const Tabs& q, int sg, double lam, double k, double m, double mp,
+                              double mu, double T, double Phi, double Phib, double xi, double& re, double& im)
+{
+    const double Emin = m, Emax = e_cut(P, m);
+    double ivi[2][2];
+    const int nimag = sing_kpos(lam, k, m, mp, Emin, Emax, ivi);
+    double roots[2];
+    const int nr = roots_AB(lam, k, m, mp, Emin, Emax, roots);
+    // build_intervals_from_roots  :275-292
+    const double del = fmax(1e-10, 1e-8 * (Emax - Emin));
+    double ep[6];
+    int ne = 0;
+    ep[ne++] = Emin;
+    for (int i = 0; i < nr; i++) {
+        ep[ne++] = roots[i] - del;
+        ep[ne++] = roots[i] + del;
+    }
+    ep[ne++] = Emax;
+    double iv[3][2];
+    int niv = 0;
+    for (int i = 0; i + 1 < ne; i += 2)
+        if (ep[i + 1] > ep[i] + 1e-16) {
+            iv[niv][0] = ep[i];
+            iv[niv][1] = ep[i + 1];
+            niv++;
+        }
+    const BcKpos fr{sg, lam, k, m, mp, mu, T, Phi, Phib, xi, false};
+    double real = 0.0;
+    for (int i = 0; i < niv; i++) {
+        int pos;
+        if (niv == 1) pos = nr == 0 ? NONE : BOTH;
+        else if (i == 0) pos = RIGHT;
+        else if (i == niv - 1) pos = LEFT;
+        else pos = BOTH;
+        const int n = (nr == 0) ? 16 : (pos == LEFT ? 16 : 32);
+        real += hybrid(fr, q, iv[i][0], iv[i][1], pos, n);
+    }
+    double imag = 0.0;
+    if (nimag > 0) {
+        const BcKpos fi{sg, lam, k, m, mp, mu, T, Phi, Phib, xi, true};
+        for (int j = 0; j < nimag; j++) {
+            const double E1 = ivi[j][0], E2 = ivi[j][1];
+            const double half = (E2 - E1) / 2, center = (E1 + E2) / 2;
+            for (int i = 0; i < 16; i++) {
+                double x = center + half * q.x16[i];
+                double w = q.w16[i] * half;
+                double v = fi(x);
+                if (isfinite(v)) imag += w * v;
+            }
+        }
+        imag *= jsign(lam);
+    }
+    re = real;
+    im = imag;
+}
+
+__device__ void tilde_Bc(const Phys& P, const Tabs& q, int sg, double lam, double k, double m, double mp, double mu,
+                         double T, double Phi, double Phib, double xi, double& re, double& im)
+{
+    if (k > kEpsK) tilde_Bc_kpos(P, q, sg, lam, k, m, mp, mu, T, Phi, Phib, xi, re, im);
+    else tilde_Bc_k0(P, q, sg, lam, m, mp, mu, T, Phi, Phib, xi, re, im);
+}
+
+// B0_correction  OneLoopIntegralsAniso.jl:396-413 (all four terms added)
+__device__ void B0_corr(const Phys& P, const Tabs& q, double lam, double k, double m1, double m2, double mu1,
+                        double mu2, double T, double Phi, double Phib, double xi, double& re, double& im)
+{
+    double r, i;
+    tilde_Bc(P, q, QUARK, -lam, k, m1, m2, mu1, T, Phi, Phib, xi, r, i);
+    re = r;
+    im = i;
+    tilde_Bc(P, q, ANTIQUARK, lam, k, m1, m2, mu1, T, Phi, Phib, xi, r, i);
+    re += r;
+    im += i;
+    tilde_Bc(P, q, QUARK, lam, k, m2, m1, mu2, T, Phi, Phib, xi, r, i);
+    re += r;
+    im += i;
+    tilde_Bc(P, q, ANTIQUARK, -lam, k, m2, m1, mu2, T, Phi, Phib, xi, r, i);
+    re += r;
+    im += i;
+}
+
+// The symmetrised B0 (+ B0_correction) of polarization_aniso
+//   PolarizationAniso.jl:29-54
+__device__ void b0_total(const Phys& P, const Tabs& q, double k0, double k, double m1, double m2, double mu1,
+                         double mu2, double T, double Phi, double Phib, double xi, int num_s, double& br, double& bi)
+{
+    const double lam = k0 + mu1 - mu2;
+    B0(P, q, lam, k, m1, mu1, m2, mu2, T, Phi, Phib, br, bi);
+    const double lx = -k0 + mu1 - mu2;
+    if (num_s == 1) {
+        double r2, i2;
+        B0(P, q, lx, k, m1, mu1, m2, mu2, T, Phi, Phib, r2, i2);
+        br = 0.5 * (br + r2);
+        bi = 0.5 * (bi + i2);
+    }
+    if (fabs(xi) > kEpsSeg) {
+        double cr, ci;
+        B0_corr(P, q, lam, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, cr, ci);
+        if (num_s == 1) {
+            double r2, i2;
+            B0_corr(P, q, lx, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, r2, i2);
+            cr = 0.5 * (cr + r2);
+            ci = 0.5 * (ci + i2);
+        }
+        br += cr;
+        bi += ci;
+    }
+}
+
+// polarization_aniso / polarization_with_width  PolarizationAniso.jl:22-118
+__device__ void pi_value(const Phys& P, int channel, int width, double k0, double gamma, double k, double m1,
+                         double m2, double lam, double A12, double br, double bi, double& re, double& im)
+{
+    const double factor = -P.Nc / (8 * (kPi * kPi));
+    double real = A12, imag = 0.0;
+    double pref = k * k - lam * lam;
+    pref += channel == PNJL_CHANNEL_P ? (m1 - m2) * (m1 - m2) : (m1 + m2) * (m1 + m2);
+    if (width) {
+        pref += (gamma * gamma) / 4.0;
+        real += pref * br - gamma * lam * bi;
+        imag += pref * bi + gamma * lam * br;
+    } else {
+        real += pref * br;
+        imag += pref * bi;
+    }
+    re = factor * real;
+    im = factor * imag;
+}
+
+// A / A_aniso  OneLoopIntegrals.jl:507-543, OneLoopIntegralsAniso.jl:446-467
+__device__ double A_const(const Phys& P, double m)
+{
+    const double mp = m > 0 ? m : 0.0;
+    const double L = P.Lambda;
+    if (mp < 1e-14) return (L * L) / 2.0;
+    const double s = sqrt(L * L + mp * mp);
+    return (L * s - mp * mp * log((L + s) / mp)) / 2.0;
+}
+
+__device__ double A_value(const Phys& P, double m, double mu, double T, double Phi, double Phib, double xi, int np,
+                          const double* xp, const double* wp, int nc, const double* xc, const double* wc)
+{
+    double integral = 0.0;
+    if (nc <= 0) {
+        integral = -A_const(P, m);
+        for (int i = 0; i < np; i++) {
+            double p = xp[i], E = sqrt(p * p + m * m);
+            integral += wp[i] * (p * p) / E * (quark_f(E, mu, T, Phi, Phib) + antiquark_f(E, mu, T, Phi, Phib));
+        }
+        return 4.0 * integral;
+    }
+    for (int i = 0; i < np; i++) {
+        double p = xp[i], aq = 0.0, aa = 0.0;
+        double E = sqrt(p * p + m * m);
+        for (int j = 0; j < nc; j++) {
+            double pc = p * xc[j];
+            double Ea = sqrt(p * p + m * m + xi * (pc * pc));
+            aq += wc[j] * (p * p) / E * quark_f(Ea, mu, T, Phi, Phib);
+            aa += wc[j] * (p * p) / E * antiquark_f(Ea, mu, T, Phi, Phib);
+        }
+        integral += wp[i] * (aq + aa);
+    }
+    return 2.0 * integral - 4.0 * A_const(P, m);
+}
+
+// ---- kernels -----------------------------------------------------------------
+__device__ __forceinline__ void load_tabs(double* lds, const double* tab)
+{
+    for (int i = threadIdx.x; i < kTabLen; i += blockDim.x) lds[i] = tab[i];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) query_kernel(Phys P, const double* tab, int64_t n, const pnjl_pol_query* qs,
+                                                    double* re, double* im)
+{
+    __shared__ double lds[kTabLen];
+    load_tabs(lds, tab);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Tabs q = make_tabs(lds);
+    const pnjl_pol_query a = qs[i];
+    if (a.channel != PNJL_CHANNEL_P && a.channel != PNJL_CHANNEL_S) {
+        re[i] = im[i] = __builtin_nan("");   // unsupported channel
+        return;
+    }
+    double br, bi;
+    b0_total(P, q, a.k0, a.k, a.m1, a.m2, a.mu1, a.mu2, a.T, a.Phi, a.Phib, a.xi, a.num_s, br, bi);
+    pi_value(P, a.channel, a.width, a.k0, a.gamma, a.k, a.m1, a.m2, a.k0 + a.mu1 - a.mu2, a.A1 + a.A2, br, bi, re[i],
+             im[i]);
+}
+
+__global__ void __launch_bounds__(256) b0_kernel(Phys P, const double* tab, int64_t n, const double* args, int corr,
+                                                 double* re, double* im)
+{
+    __shared__ double lds[kTabLen];
+    load_tabs(lds, tab);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Tabs q = make_tabs(lds);
+    if (!corr) {
+        const double* a = args + i * 9;
+        B0(P, q, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], re[i], im[i]);
+    } else {
+        const double* a = args + i * 10;
+        B0_corr(P, q, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], re[i], im[i]);
+    }
+}
+
+__global__ void __launch_bounds__(256) a_kernel(Phys P, int64_t n, const double* args, int np, const double* xp,
+                                                const double* wp, int nc, const double* xc, const double* wc,
+                                                double* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int w = nc > 0 ? 6 : 5;
+    const double* a = args + i * w;
+    out[i] = A_value(P, a[0], a[1], a[2], a[3], a[4], nc > 0 ? a[5] : 0.0, np, xp, wp, nc, xc, wc);
+}
+
+// grid: A_aniso(m_u), A_aniso(m_s) per point -> A[pt*2 + f]
+__global__ void __launch_bounds__(256) grid_a_kernel(Phys P, int npts, const pnjl_pol_point* pts, int np,
+                                                     const double* xp, const double* wp, int nc, const double* xc,
+                                                     const double* wc, double* A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * npts) return;
+    const pnjl_pol_point s = pts[i >> 1];
+    const double m = (i & 1) ? s.m_s : s.m_u;
+    A[i] = A_value(P, m, s.mu, s.T, s.Phi, s.Phib, s.xi, np, xp, wp, nc, xc, wc);
+}
+
+// grid: one thread per (point, pair, k0, k); both channels written
+__global__ void __launch_bounds__(256) grid_pi_kernel(Phys P, const double* tab, int npts, const pnjl_pol_point* pts,
+                                                      const double* A, int nk0, const double* k0s, int nk,
+                                                      const double* ks, double* re, double* im)
+{
+    __shared__ double lds[kTabLen];
+    load_tabs(lds, tab);
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)nk0 * nk;
+    if (tid >= (int64_t)npts * 3 * plane) return;
+    const Tabs q = make_tabs(lds);
+    const int64_t j = tid % nk, i0 = (tid / nk) % nk0;
+    const int pair = (int)((tid / plane) % 3);
+    const int pt = (int)(tid / (3 * plane));
+    const pnjl_pol_point s = pts[pt];
+    const double m1 = pair == 2 ? s.m_s : s.m_u, m2 = pair == 0 ? s.m_u : s.m_s;
+    const double A12 = A[pt * 2 + (pair == 2 ? 1 : 0)] + A[pt * 2 + (pair == 0 ? 0 : 1)];
+    const double k0 = k0s[i0], k = ks[j];
+    double br, bi;
+    b0_total(P, q, k0, k, m1, m2, s.mu, s.mu, s.T, s.Phi, s.Phib, s.xi, pair, br, bi);
+    const int64_t base = ((int64_t)pt * 3 + pair) * 2 * plane + i0 * nk + j;
+    for (int c = 0; c < 2; c++) {
+        double r, m;
+        pi_value(P, c == 0 ? PNJL_CHANNEL_P : PNJL_CHANNEL_S, 0, k0, 0.0, k, m1, m2, k0, A12, br, bi, r, m);
+        re[base + c * plane] = r;
+        im[base + c * plane] = m;
+    }
+}
+
+}  // namespace polar
+
+// =============================================================================
+// host side
+// =============================================================================
+struct PolarGraph {
+    int npts = 0, nk0 = 0, nk = 0, np = 0, nc = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    pnjl_pol_point* pts = nullptr;
+    double *k0 = nullptr, *k = nullptr, *A = nullptr, *re = nullptr, *im = nullptr, *nodes = nullptr;
+};
+
+namespace pnjl_host {
+
+void polar_release(pnjl_ctx* c)
+{
+    if (c->pgraph) {
+        PolarGraph* g = c->pgraph;
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        for (void* p : {(void*)g->pts, (void*)g->k0, (void*)g->k, (void*)g->A, (void*)g->re, (void*)g->im,
+                        (void*)g->nodes})
+            if (p) (void)hipFree(p);
+        if (g->cap) (void)hipStreamDestroy(g->cap);
+        delete g;
+        c->pgraph = nullptr;
+    }
+    if (c->polar_tab) (void)hipFree(c->polar_tab);
+    c->polar_tab = nullptr;
+}
+
+}  // namespace pnjl_host
+
+namespace {
+
+polar::Phys phys(const pnjl_ctx* c) { return polar::Phys{c->params.Lambda, (double)c->params.N_color}; }
+
+// x16 w16 x32 w32 | power-left 16/32 (t, t') | power-right 16/32 | DE 33 (phi, phi')
+//   IntervalQuadratureStrategies.jl:56-85
+int polar_tab(pnjl_ctx* ctx, double** out)
+{
+    if (!ctx->polar_tab) {
+        std::vector<double> t(polar::kTabLen), x, w;
+        double* o = t.data();
+        std::vector<double> x16, w16, x32, w32;
+        gl_rule(16, x16, w16);
+        gl_rule(32, x32, w32);
+        auto put = [&](const std::vector<double>& v) {
+            memcpy(o, v.data(), v.size() * sizeof(double));
+            o += v.size();
+        };
+        put(x16);
+        put(w16);
+        put(x32);
+        put(w32);
+        const double ia = 1.0 / polar::kAlpha;
+        for (int side = 0; side < 2; side++)
+            for (const std::vector<double>* xs : {&x16, &x32}) {
+                const size_t n = xs->size();
+                std::vector<double> tt(n), tp(n);
+                for (size_t i = 0; i < n; i++) {
+                    double u = ((*xs)[i] + 1) / 2;
+                    if (side == 0) {
+                        tt[i] = pow(u, ia);
+                        tp[i] = ia * pow(u, ia - 1) / 2;
+                    } else {
+                        tt[i] = 1 - pow(1 - u, ia);
+                        tp[i] = ia * pow(1 - u, ia - 1) / 2;
+                    }
+                }
+                put(tt);
+                put(tp);
+            }
+        std::vector<double> phi(33), phip(33);
+        for (int k = -16; k <= 16; k++) {
+            double tk = k * polar::kDeH;
+            double s = polar::kPi / 2 * sinh(tk);
+            double sech = 1.0 / cosh(s);
+            phi[k + 16] = tanh(s);
+            phip[k + 16] = polar::kDeH * (polar::kPi / 2) * cosh(tk) * (sech * sech);
+        }
+        put(phi);
+        put(phip);
+        HIPCHK(ctx, hipMalloc(&ctx->polar_tab, t.size() * sizeof(double)));
+        HIPCHK(ctx, hipMemcpy(ctx->polar_tab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    *out = ctx->polar_tab;
+    return PNJL_OK;
+}
+
+int blocks_for(int64_t n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" {
+
+int pnjl_polarization_batch(pnjl_ctx* ctx, int64_t n, const pnjl_pol_query* q, double* re, double* im, int flags,
+                            void* stream)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || (n > 0 && (!q || !re || !im)))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_polarization_batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & PNJL_FLAG_DEVICE_PTRS;
+    if (!dev)
+        for (int64_t i = 0; i < n; i++)
+            if (q[i].channel != PNJL_CHANNEL_P && q[i].channel != PNJL_CHANNEL_S)
+                return set_err(ctx, PNJL_ERR_ARG, "pnjl_polarization_batch: unsupported channel (use P or S)");
+    double* tab = nullptr;
+    if (int rc = polar_tab(ctx, &tab)) return rc;
+    const pnjl_pol_query* dq = q;
+    double *dre = re, *dim = im;
+    if (!dev) {
+        void *a, *b, *c;
+        if (int rc = stage_buf(ctx, 60, (size_t)n * sizeof(pnjl_pol_query), &a)) return rc;
+        if (int rc = stage_buf(ctx, 61, (size_t)n * sizeof(double), &b)) return rc;
+        if (int rc = stage_buf(ctx, 62, (size_t)n * sizeof(double), &c)) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(a, q, (size_t)n * sizeof(pnjl_pol_query), hipMemcpyHostToDevice, st));
+        dq = (const pnjl_pol_query*)a;
+        dre = (double*)b;
+        dim = (double*)c;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(polar::query_kernel, dim3(blocks_for(n)), dim3(256), 0, st, phys(ctx), tab, n, dq, dre, dim);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
+    if (!dev) {
+        HIPCHK(ctx, hipMemcpyAsync(re, dre, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(im, dim, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    if (!(flags & PNJL_FLAG_ASYNC) || !dev) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    }
+    return PNJL_OK;
+}
+
+static int b0_common(pnjl_ctx* ctx, int64_t n, const double* args, int corr, double* re, double* im)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || (n > 0 && (!args || !re || !im))) return set_err(ctx, PNJL_ERR_ARG, "B0 batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    double* tab = nullptr;
+    if (int rc = polar_tab(ctx, &tab)) return rc;
+    const int w = corr ? 10 : 9;
+    void *a, *b, *c;
+    if (int rc = stage_buf(ctx, 60, (size_t)n * w * sizeof(double), &a)) return rc;
+    if (int rc = stage_buf(ctx, 61, (size_t)n * sizeof(double), &b)) return rc;
+    if (int rc = stage_buf(ctx, 62, (size_t)n * sizeof(double), &c)) return rc;
+    HIPCHK(ctx, hipMemcpy(a, args, (size_t)n * w * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(polar::b0_kernel, dim3(blocks_for(n)), dim3(256), 0, 0, phys(ctx), tab, n, (const double*)a,
+                       corr, (double*)b, (double*)c);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpy(re, b, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(im, c, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return PNJL_OK;
+}
+
+int pnjl_B0_batch(pnjl_ctx* ctx, int64_t n, const double* args, double* re, double* im)
+{
+    return b0_common(ctx, n, args, 0, re, im);
+}
+
+int pnjl_B0_correction_batch(pnjl_ctx* ctx, int64_t n, const double* args, double* re, double* im)
+{
+    return b0_common(ctx, n, args, 1, re, im);
+}
+
+int pnjl_A_batch(pnjl_ctx* ctx, int64_t n, const double* args, int np, const double* nodes_p, const double* weights_p,
+                 int nc, const double* nodes_cos, const double* weights_cos, double* out)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || np <= 0 || np > 65536 || nc < 0 || nc > 4096 || (n > 0 && (!args || !out)) || !nodes_p ||
+        !weights_p || (nc > 0 && (!nodes_cos || !weights_cos)))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_A_batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int w = nc > 0 ? 6 : 5;
+    void *a, *nd, *o;
+    if (int rc = stage_buf(ctx, 60, (size_t)n * w * sizeof(double), &a)) return rc;
+    if (int rc = stage_buf(ctx, 61, (size_t)(2 * np + 2 * nc) * sizeof(double), &nd)) return rc;
+    if (int rc = stage_buf(ctx, 62, (size_t)n * sizeof(double), &o)) return rc;
+    double* d = (double*)nd;
+    HIPCHK(ctx, hipMemcpy(a, args, (size_t)n * w * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d, nodes_p, np * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d + np, weights_p, np * sizeof(double), hipMemcpyHostToDevice));
+    if (nc > 0) {
+        HIPCHK(ctx, hipMemcpy(d + 2 * np, nodes_cos, nc * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(d + 2 * np + nc, weights_cos, nc * sizeof(double), hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(polar::a_kernel, dim3(blocks_for(n)), dim3(256), 0, 0, phys(ctx), n, (const double*)a, np, d,
+                       d + np, nc, d + 2 * np, d + 2 * np + nc, (double*)o);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpy(out, o, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return PNJL_OK;
+}
+
+int pnjl_polarization_grid(pnjl_ctx* ctx, int npts, const pnjl_pol_point* pts, int nk0, const double* k0, int nk,
+                           const double* k, int np, int nc, double* re, double* im, int flags, void* stream)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (npts <= 0 || nk0 <= 0 || nk <= 0 || np <= 0 || np > 4096 || nc <= 0 || nc > 64 || !pts || !k0 || !k ||
+        !re || !im)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_polarization_grid: bad argument");
+    const int64_t nout = (int64_t)npts * 6 * nk0 * nk;
+    if (nout > ((int64_t)1 << 31)) return set_err(ctx, PNJL_ERR_ARG, "pnjl_polarization_grid: grid too large");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    double* tab = nullptr;
+    if (int rc = polar_tab(ctx, &tab)) return rc;
+    PolarGraph* g = ctx->pgraph;
+    if (g && (g->npts != npts || g->nk0 != nk0 || g->nk != nk || g->np != np || g->nc != nc)) {
+        polar_release(ctx);   // a new shape: rebuild the graph (tables are rebuilt too)
+        if (int rc = polar_tab(ctx, &tab)) return rc;
+        g = nullptr;
+    }
+    if (!g) {
+        g = new PolarGraph();
+        ctx->pgraph = g;
+        g->npts = npts;
+        g->nk0 = nk0;
+        g->nk = nk;
+        g->np = np;
+        g->nc = nc;
+        HIPCHK(ctx, hipStreamCreate(&g->cap));
+        HIPCHK(ctx, hipMalloc(&g->pts, (size_t)npts * sizeof(pnjl_pol_point)));
+        HIPCHK(ctx, hipMalloc(&g->k0, (size_t)nk0 * sizeof(double)));
+        HIPCHK(ctx, hipMalloc(&g->k, (size_t)nk * sizeof(double)));
+        HIPCHK(ctx, hipMalloc(&g->A, (size_t)npts * 2 * sizeof(double)));
+        HIPCHK(ctx, hipMalloc(&g->re, (size_t)nout * sizeof(double)));
+        HIPCHK(ctx, hipMalloc(&g->im, (size_t)nout * sizeof(double)));
+        // momentum nodes GL on [0, 10] fm^-1, cos nodes GL on [-1, 1]
+        std::vector<double> xp, wp, xc, wc;
+        gl_rule(np, xp, wp);
+        for (int i = 0; i < np; i++) {
+            xp[i] = 5.0 * xp[i] + 5.0;
+            wp[i] = 5.0 * wp[i];
+        }
+        gl_rule(nc, xc, wc);
+        std::vector<double> nodes;
+        nodes.insert(nodes.end(), xp.begin(), xp.end());
+        nodes.insert(nodes.end(), wp.begin(), wp.end());
+        nodes.insert(nodes.end(), xc.begin(), xc.end());
+        nodes.insert(nodes.end(), wc.begin(), wc.end());
+        HIPCHK(ctx, hipMalloc(&g->nodes, nodes.size() * sizeof(double)));
+        HIPCHK(ctx, hipMemcpy(g->nodes, nodes.data(), nodes.size() * sizeof(double), hipMemcpyHostToDevice));
+        // capture the A and Pi kernels once; later calls replay the graph
+        HIPCHK(ctx, hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(polar::grid_a_kernel, dim3(blocks_for(2 * (int64_t)npts)), dim3(256), 0, g->cap,
+                           phys(ctx), npts, (const pnjl_pol_point*)g->pts, np, g->nodes, g->nodes + np, nc,
+                           g->nodes + 2 * np, g->nodes + 2 * np + nc, g->A);
+        hipLaunchKernelGGL(polar::grid_pi_kernel, dim3(blocks_for(nout / 2)), dim3(256), 0, g->cap, phys(ctx), tab,
+                           npts, (const pnjl_pol_point*)g->pts, (const double*)g->A, nk0, (const double*)g->k0, nk,
+                           (const double*)g->k, g->re, g->im);
+        HIPCHK(ctx, hipStreamEndCapture(g->cap, &g->graph));
+        HIPCHK(ctx, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : g->cap;
+    const bool dev = flags & PNJL_FLAG_DEVICE_PTRS;
+    const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind outk = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIPCHK(ctx, hipMemcpyAsync(g->pts, pts, (size_t)npts * sizeof(pnjl_pol_point), in, st));
+    HIPCHK(ctx, hipMemcpyAsync(g->k0, k0, (size_t)nk0 * sizeof(double), in, st));
+    HIPCHK(ctx, hipMemcpyAsync(g->k, k, (size_t)nk * sizeof(double), in, st));
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
+    HIPCHK(ctx, hipGraphLaunch(g->exec, st));
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
+    HIPCHK(ctx, hipMemcpyAsync(re, g->re, (size_t)nout * sizeof(double), outk, st));
+    HIPCHK(ctx, hipMemcpyAsync(im, g->im, (size_t)nout * sizeof(double), outk, st));
+    if (!(flags & PNJL_FLAG_ASYNC) || !dev) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    }
+    return PNJL_OK;
+}
+
+}  // extern "C"

@@ -259,3 +259,119 @@ def set_judge_mask(mask, lo=None, hi=None):
                                  m.ctypes.data_as(C.POINTER(C.c_uint8)))
     if rc != 0:
         raise ValueError("oracle_set_judge_mask: bad geometry")
+
+
+# ---------------------------------------------------------------------------
+# anisotropic one-loop polarization (pnjl_polar_oracle.c)
+# ---------------------------------------------------------------------------
+CHANNEL_P, CHANNEL_S = 1, 2
+
+
+class PolQuery(C.Structure):
+    _fields_ = [("channel", C.c_int), ("num_s", C.c_int), ("width", C.c_int), ("reserved", C.c_int),
+                ("k0", C.c_double), ("gamma", C.c_double), ("k", C.c_double), ("m1", C.c_double),
+                ("m2", C.c_double), ("mu1", C.c_double), ("mu2", C.c_double), ("T", C.c_double),
+                ("Phi", C.c_double), ("Phib", C.c_double), ("xi", C.c_double), ("A1", C.c_double),
+                ("A2", C.c_double)]
+
+
+POL_FIELDS = [f for f, _ in PolQuery._fields_ if f != "reserved"]
+
+
+def pol_queries(**cols):
+    """Array of PolQuery from equal-length columns (names as in PolQuery)."""
+    n = max(np.size(v) for v in cols.values())
+    q = (PolQuery * n)()
+    for f in POL_FIELDS:
+        v = np.broadcast_to(np.asarray(cols.get(f, 0)), (n,))
+        for i in range(n):
+            setattr(q[i], f, int(v[i]) if f in ("channel", "num_s", "width") else float(v[i]))
+    return q
+
+
+def _polar_lib():
+    L = lib()
+    if not getattr(L, "_polar_ready", False):
+        d2 = [C.c_double] * 2
+        L.oracle_B0.argtypes = [C.c_double] * 9 + [_DP, _DP]
+        L.oracle_tilde_B0.argtypes = [C.c_int] + [C.c_double] * 8 + [_DP, _DP]
+        L.oracle_B0_correction.argtypes = [C.c_double] * 10 + [_DP, _DP]
+        L.oracle_A_const.argtypes = [C.c_double]
+        L.oracle_A_const.restype = C.c_double
+        L.oracle_A.argtypes = [C.c_double] * 5 + [C.c_int, _DP, _DP]
+        L.oracle_A.restype = C.c_double
+        L.oracle_A_correction.argtypes = [C.c_double] * 6 + [C.c_int, _DP, _DP]
+        L.oracle_A_correction.restype = C.c_double
+        L.oracle_A_aniso.argtypes = [C.c_double] * 6 + [C.c_int, _DP, _DP, C.c_int, _DP, _DP]
+        L.oracle_A_aniso.restype = C.c_double
+        L.oracle_polarization.argtypes = [C.c_int, C.c_int] + [C.c_double] * 13 + [C.c_int, _DP, _DP]
+        L.oracle_polarization.restype = C.c_int
+        L.oracle_polarization_batch.argtypes = [C.c_int64, C.POINTER(PolQuery), _DP, _DP, C.c_int]
+        L.oracle_polarization_batch.restype = C.c_int
+        L.oracle_polar_tables.argtypes = [_DP]
+        del d2
+        L._polar_ready = True
+    return L
+
+
+def _pair(fn, *args):
+    re, im = C.c_double(), C.c_double()
+    fn(*args, C.byref(re), C.byref(im))
+    return re.value, im.value
+
+
+def B0(lam, k, m1, mu1, m2, mu2, T, Phi=0.0, Phib=0.0):
+    return _pair(_polar_lib().oracle_B0, lam, k, m1, mu1, m2, mu2, T, Phi, Phib)
+
+
+def tilde_B0(plus, lam, k, m, mp, mu, T, Phi=0.0, Phib=0.0):
+    return _pair(_polar_lib().oracle_tilde_B0, 0 if plus else 1, lam, k, m, mp, mu, T, Phi, Phib)
+
+
+def B0_correction(lam, k, m1, m2, mu1, mu2, T, Phi, Phib, xi):
+    return _pair(_polar_lib().oracle_B0_correction, lam, k, m1, m2, mu1, mu2, T, Phi, Phib, xi)
+
+
+def A_const(m):
+    return _polar_lib().oracle_A_const(m)
+
+
+def A(m, mu, T, Phi, Phib, nodes, weights):
+    x, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (nodes, weights))
+    return _polar_lib().oracle_A(m, mu, T, Phi, Phib, x.size, _p(x), _p(w))
+
+
+def A_correction(m, mu, T, Phi, Phib, xi, nodes, weights):
+    x, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (nodes, weights))
+    return _polar_lib().oracle_A_correction(m, mu, T, Phi, Phib, xi, x.size, _p(x), _p(w))
+
+
+def A_aniso(m, mu, T, Phi, Phib, xi, nodes_p, weights_p, nodes_c, weights_c):
+    xp, wp, xc, wc = (np.ascontiguousarray(a, dtype=np.float64) for a in (nodes_p, weights_p, nodes_c, weights_c))
+    return _polar_lib().oracle_A_aniso(m, mu, T, Phi, Phib, xi, xp.size, _p(xp), _p(wp), xc.size, _p(xc), _p(wc))
+
+
+def polarization(channel, k0, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, A1, A2, num_s, gamma=None):
+    """polarization_aniso (gamma None) or polarization_with_width."""
+    re, im = C.c_double(), C.c_double()
+    rc = _polar_lib().oracle_polarization(channel, 0 if gamma is None else 1, k0, 0.0 if gamma is None else gamma, k,
+                                          m1, m2, mu1, mu2, T, Phi, Phib, xi, A1, A2, num_s, C.byref(re),
+                                          C.byref(im))
+    if rc != 0:
+        raise ValueError("Unsupported channel")
+    return re.value, im.value
+
+
+def polarization_batch(q, nthreads=8):
+    n = len(q)
+    re, im = np.zeros(n), np.zeros(n)
+    rc = _polar_lib().oracle_polarization_batch(n, q, _p(re), _p(im), nthreads)
+    if rc != 0:
+        raise ValueError("Unsupported channel")
+    return re, im
+
+
+def polar_tables():
+    t = np.zeros(16 * 2 + 32 * 2 + (16 * 2 + 32 * 2) * 2 + 66)
+    _polar_lib().oracle_polar_tables(_p(t))
+    return t
