@@ -36,9 +36,11 @@ HB = 197.327
 METRIC = "PNJL gap points solved/s at |res|<1e-10: surrogate-seeded vs plain Newton"
 PEAK_FP64_TFLOPS = 78.6
 # FP64 FLOPs of one node x flavour evaluation of the fused integrand (value,
-# gradient, Hessian and thermodynamic sums), counted on the gfx950 ISA of the
-# node loop: 94 add + 65 mul + 98 FMA (x2) + 5 rcp/rsq = 360.
-C_NODE = 360.0
+# gradient, Hessian and thermodynamic sums) as the device routine executes it:
+# FP64 FMA = 2, add/mul/transcendental = 1, measured with rocprofv3 FP64 VALU
+# counters on the residual kernel at p = 64 and p = 32
+# (python3 -m julia_relaxtime_amd.flop_probe): 198.1 + 2.0.
+C_NODE = 200.0
 
 
 def grid(nT, nmu, xis):

@@ -1,0 +1,127 @@
+"""
+    PNJLHip
+
+Julia binding of libpnjl_hip.so (include/pnjl_hip.h) for the reference's
+`PNJL.solve` / `polarization_aniso` call sites: a thin `ccall` layer with the
+reference's argument meaning, so a driver switches backends with one keyword
+(`backend=:hip`).  No Julia toolchain exists in the build image; this module
+is the binding a maintainer adds to the reference and is not executed here
+(the Python mirror in julia_relaxtime_amd/ is the tested host).
+
+Replaces:
+  PNJL.solve(::FixedMu, T_fm, mu_fm; ...)          src/pnjl/solver/ImplicitSolver.jl:211-328
+  polarization_aniso(channel, k0, k_norm, ...)      src/relaxtime/PolarizationAniso.jl:22-24
+"""
+module PNJLHip
+
+const LIB = get(ENV, "PNJL_HIP_LIB", joinpath(@__DIR__, "..", "libpnjl_hip.so"))
+
+# policies / flags (include/pnjl_hip.h)
+const SEED_DEFAULT, SEED_HADRON, SEED_QUARK, SEED_MULTI, SEED_EXPLICIT = 0, 1, 2, 3, 4
+const SEED_CONTINUITY, SEED_PHASE_AWARE, SEED_SURROGATE, SEED_MULTIROOT3 = 5, 6, 7, 8
+const CHANNEL_P, CHANNEL_S = 1, 2
+
+struct SolveOpts
+    xtol::Float64
+    ftol::Float64
+    residual_norm_max::Float64
+    phi_tol::Float64
+    max_iter::Cint
+    trust_region_fallback::Cint
+    auto_multiseed_fallback::Cint
+    surrogate_max_steps::Cint
+    chain_len::Cint
+    surrogate_judge::Cint
+end
+
+struct SolveOut
+    x::Ptr{Float64}; omega::Ptr{Float64}; pressure::Ptr{Float64}; rho_norm::Ptr{Float64}
+    entropy::Ptr{Float64}; energy::Ptr{Float64}; masses::Ptr{Float64}; n_q::Ptr{Float64}
+    n_qbar::Ptr{Float64}; residual_norm::Ptr{Float64}; iterations::Ptr{Cint}; converged::Ptr{Cint}
+    flags::Ptr{Cint}; newton_steps::Ptr{Cint}; passes::Ptr{Cint}
+end
+
+struct PolQuery
+    channel::Cint; num_s::Cint; width::Cint; reserved::Cint
+    k0::Float64; gamma::Float64; k::Float64; m1::Float64; m2::Float64; mu1::Float64; mu2::Float64
+    T::Float64; Phi::Float64; Phib::Float64; xi::Float64; A1::Float64; A2::Float64
+end
+
+mutable struct Context
+    h::Ptr{Cvoid}
+end
+
+function check(ctx::Context, rc::Cint, what)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:pnjl_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx.h))
+    error("$what failed with status $rc: $msg")
+end
+
+"""One context per device and host thread (calls on a context are serialised)."""
+function Context(device::Integer=0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pnjl_ctx_create, LIB), Cint, (Ptr{Cvoid}, Cint, Ref{Ptr{Cvoid}}), C_NULL, device, h)
+    rc == 0 || error("pnjl_ctx_create failed with status $rc")
+    ctx = Context(h[])
+    finalizer(c -> ccall((:pnjl_ctx_destroy, LIB), Cvoid, (Ptr{Cvoid},), c.h), ctx)
+    return ctx
+end
+
+const DEFAULT_CTX = Ref{Union{Nothing,Context}}(nothing)
+default_context() = something(DEFAULT_CTX[], (DEFAULT_CTX[] = Context(0)))
+
+"""
+    solve_batch(T_fm, mu_fm; xi, policy, seeds, p_num=64, t_num=8, iterations=1000, ...)
+
+Batched `PNJL.solve(FixedMu(), T, mu; ...)` in one persistent-kernel launch.
+Returns a NamedTuple of arrays (SolverResult fields, point-major).
+"""
+function solve_batch(T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{Float64};
+                     xi::AbstractVector{Float64}=zeros(length(T_fm)), policy::Integer=SEED_DEFAULT,
+                     seeds::Union{Nothing,Matrix{Float64}}=nothing, p_num::Integer=64, t_num::Integer=8,
+                     iterations::Integer=1000, trust_region_fallback::Bool=true,
+                     auto_multiseed_fallback::Bool=true, residual_norm_max::Float64=1e-6,
+                     xtol::Float64=1e-9, ftol::Float64=1e-9, chain_len::Integer=0,
+                     surrogate_max_steps::Integer=8, surrogate_judge::Integer=0,
+                     ctx::Context=default_context())
+    n = length(T_fm)
+    opts = SolveOpts(xtol, ftol, residual_norm_max, 1e-8, iterations, trust_region_fallback,
+                     auto_multiseed_fallback, surrogate_max_steps, chain_len, surrogate_judge)
+    x = zeros(5, n); om = zeros(n); P = zeros(n); rho = zeros(n); s = zeros(n); e = zeros(n)
+    M = zeros(3, n); nq = zeros(3, n); nqb = zeros(3, n); res = zeros(n)
+    it = zeros(Cint, n); conv = zeros(Cint, n); fl = zeros(Cint, n); st = zeros(Cint, n); ps = zeros(Cint, n)
+    GC.@preserve T_fm mu_fm xi seeds x om P rho s e M nq nqb res it conv fl st ps begin
+        out = SolveOut(pointer(x), pointer(om), pointer(P), pointer(rho), pointer(s), pointer(e), pointer(M),
+                       pointer(nq), pointer(nqb), pointer(res), pointer(it), pointer(conv), pointer(fl),
+                       pointer(st), pointer(ps))
+        sp = seeds === nothing ? Ptr{Float64}(C_NULL) : pointer(seeds)   # 5 x n, column-major = point-major
+        rc = ccall((:pnjl_solve_fixedmu_batch, LIB), Cint,
+                   (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Cint, Cint,
+                    Ref{SolveOpts}, Ref{SolveOut}, Cint, Ptr{Cvoid}),
+                   ctx.h, n, T_fm, mu_fm, xi, sp, policy, p_num, t_num, opts, out, 0, C_NULL)
+        check(ctx, rc, "pnjl_solve_fixedmu_batch")
+    end
+    return (solution=x, omega=om, pressure=P, rho_norm=rho, entropy=s, energy=e, masses=M, n_q=nq,
+            n_qbar=nqb, residual_norm=res, iterations=it, converged=conv .!= 0, flags=fl)
+end
+
+"""
+    polarization_aniso(channel, k0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi, A1, A2, num_s)
+
+Same signature and meaning as the reference (channel :P or :S); computed on the GPU.
+"""
+function polarization_aniso(channel::Symbol, k0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi,
+                            A1_value, A2_value, num_s_quark::Int; ctx::Context=default_context())
+    ch = channel === :P ? CHANNEL_P : channel === :S ? CHANNEL_S :
+         throw(ArgumentError("Unsupported channel: $channel. Use :P or :S."))
+    q = Ref(PolQuery(ch, num_s_quark, 0, 0, k0, 0.0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi,
+                     A1_value, A2_value))
+    re = Ref(0.0); im = Ref(0.0)
+    rc = ccall((:pnjl_polarization_batch, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Ref{PolQuery}, Ref{Float64}, Ref{Float64}, Cint, Ptr{Cvoid}),
+               ctx.h, 1, q, re, im, 0, C_NULL)
+    check(ctx, rc, "pnjl_polarization_batch")
+    return re[], im[]
+end
+
+end # module PNJLHip
