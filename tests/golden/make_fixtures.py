@@ -9,6 +9,11 @@ Sources (paths relative to the reference root):
   data/outputs/results/relaxtime/gap_transport_scan*.csv  (run_gap_transport_scan.jl,
       PhaseAwareContinuitySeed chains, p12/t6, full Float64 precision)
   data/outputs/results/pnjl/tmu_scan.csv               (TmuScan, 6 decimals)
+  data/reference/pnjl/trho_scan_xi{0.0,0.2,0.4}.csv    (TrhoScan, p24/t8, 6 decimals): the
+      rows at T = 50, 80, 110 MeV, rho > 0, converged -- FixedRho states, which are FixedMu
+      roots at mu = mu_avg on all three branches of the first-order S-curve
+  data/reference/pnjl/spinodals.csv, boundary.csv      (calculate_phase_structure.jl: S-curve
+      spinodals and the Maxwell-construction transition line, xi = 0, 0.2, 0.4)
 """
 import csv
 import os
@@ -46,6 +51,21 @@ def main(ref):
           ["T_MeV", "mu_MeV", "xi", "pressure_fm4", "rho", "entropy_fm3", "energy_fm4", "phi_u", "phi_d", "phi_s",
            "Phi1", "Phi2", "iterations", "converged"],
           rows_of(os.path.join(ref, "data/outputs/results/pnjl/tmu_scan.csv")))
+    trho = []
+    for xi in ["0.0", "0.2", "0.4"]:
+        for r in rows_of(os.path.join(ref, f"data/reference/pnjl/trho_scan_xi{xi}.csv")):
+            if (r["converged"] == "true" and float(r["rho"]) > 0
+                    and round(float(r["T_MeV"])) in (50, 80, 110)):
+                trho.append(r)
+    write("trho_scan_T50_80_110.csv",
+          ["T_MeV", "rho", "xi", "mu_avg_MeV", "pressure_fm4", "entropy_fm3", "energy_fm4", "phi_u", "phi_d",
+           "phi_s", "Phi1", "Phi2", "M_u_MeV", "M_s_MeV"], trho)
+    write("spinodals.csv",
+          ["xi", "T_MeV", "mu_spinodal_hadron_MeV", "mu_spinodal_quark_MeV", "rho_spinodal_hadron",
+           "rho_spinodal_quark"],
+          rows_of(os.path.join(ref, "data/reference/pnjl/spinodals.csv")))
+    write("phase_boundary.csv", ["xi", "T_MeV", "mu_transition_MeV", "rho_hadron", "rho_quark"],
+          rows_of(os.path.join(ref, "data/reference/pnjl/boundary.csv")))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,31 @@
+"""GPU multi-root scan (MULTIROOT3: learned-seed candidate + HADRON + QUARK
+seeds, minimum-Omega root, first-order flag by DualBranchScan's sameness
+test) against the reference's first-order line and spinodals, and against
+the oracle point by point."""
+import numpy as np
+import pytest
+
+from test_oracle_learned import c5_lines, check_c5_lines
+
+pytestmark = pytest.mark.gpu
+
+
+def test_multiroot3_lines_reproduce_reference_phase_structure(gpu_ctx, oracle):
+    """36 mu-lines x 512 points (xi = 0, 0.2, 0.4; T 50-115 MeV) at p24/t8:
+    the selected root switches branch at the reference's Maxwell
+    mu_transition and the spinodal interval is flagged first-order; every
+    line matches the oracle (1e-10, selection ties allowed), flags included."""
+    from julia_relaxtime_amd import _lib
+    from test_gpu_gap import assert_parity
+    lines, T, mu, xi = c5_lines(oracle)
+    assert len(lines) >= 30
+    opts = _lib.default_opts(ftol=1e-10, max_iter=40)
+    plain = gpu_ctx.solve_batch(T, mu, xi, policy=_lib.SEED_DEFAULT, p_num=24, t_num=8, opts=opts)
+    r = gpu_ctx.solve_batch(T, mu, xi, policy=_lib.SEED_MULTIROOT3, seed=plain["x"], p_num=24, t_num=8, opts=opts)
+    check_c5_lines(lines, r)
+    assert np.all(r["residual_norm"] < 1e-10)
+    o = oracle.solve_batch(T, mu, xi, policy=oracle.SEED_MULTIROOT3, seed=plain["x"], p_num=24, t_num=8,
+                           opts=oracle.default_opts(ftol=1e-10, max_iter=40), nthreads=8)
+    assert_parity(r, o, selection=True)
+    fo_r, fo_o = (r["flags"] & _lib.PT_FIRST_ORDER) != 0, (o["flags"] & _lib.PT_FIRST_ORDER) != 0
+    assert np.mean(fo_r == fo_o) > 0.999

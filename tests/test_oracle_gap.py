@@ -287,6 +287,103 @@ def test_tmu_scan_coarse_values(oracle):
     assert np.max(np.abs(r["pressure"][ok] - P[ok])) < 5e-6
 
 
+def test_trho_scan_states_are_fixedmu_roots(oracle):
+    """trho_scan_xi{0.0,0.2,0.4}.csv (TrhoScan, p24/t8, 6 decimals) at T = 50,
+    80, 110 MeV: each FixedRho state is a FixedMu root at mu = mu_avg, on the
+    hadron, unstable and quark branches of the S-curve alike; P, rho, s and
+    epsilon of the root match the stored columns."""
+    rows = golden_rows("trho_scan_T50_80_110.csv")
+    f = lambda k: np.array([float(r[k]) for r in rows])
+    T, mu, xi = f("T_MeV") / HB, f("mu_avg_MeV") / HB, f("xi")
+    x = np.stack([f(k) for k in ("phi_u", "phi_d", "phi_s", "Phi1", "Phi2")], 1)
+    r = oracle.solve_batch(T, mu, xi, policy=oracle.SEED_EXPLICIT, seed=x, p_num=24, t_num=8, nthreads=8,
+                           opts=oracle.default_opts(ftol=1e-12))
+    assert r["converged"].all()
+    dx = np.abs(r["x"] - x).max(1)
+    assert np.median(dx) < 1e-6 and dx.max() < 5e-5     # 6-decimal mu, ill-conditioned near spinodals
+    assert np.max(np.abs(r["pressure"] - f("pressure_fm4"))) < 1e-6
+    assert np.max(np.abs(r["rho_norm"] - f("rho"))) < 1e-4
+    assert np.max(np.abs(r["entropy"] - f("entropy_fm3"))) < 1e-5
+    assert np.max(np.abs(r["energy"] - f("energy_fm4"))) < 1e-4
+    assert np.max(np.abs(r["masses"][:, 0] * HB - f("M_u_MeV"))) < 1e-2
+    # the unstable branch is covered: d mu / d rho < 0 between the spinodals
+    for key in {(a, b) for a, b in zip(f("xi"), f("T_MeV"))}:
+        sel = (f("xi") == key[0]) & (f("T_MeV") == key[1])
+        o = np.argsort(f("rho")[sel])
+        if key[1] <= 80:
+            assert np.any(np.diff(f("mu_avg_MeV")[sel][o]) < 0)
+
+
+def _T_cep(xi):
+    """T_CEP (MeV) of the package's cep.csv, the reference's data/reference/pnjl/cep.csv."""
+    import julia_relaxtime_amd.pnjl as P
+    return P.load_phase_boundary(xi)[2][2]
+
+
+def test_maxwell_transition_matches_reference_boundary(oracle):
+    """boundary.csv (Maxwell construction on the T-rho S-curves): at mu_transition
+    the hadron and quark roots have equal Omega to within 0.03 MeV of mu, and
+    their densities match the stored rho_hadron / rho_quark."""
+    rows = [r for r in golden_rows("phase_boundary.csv")]
+    checked = 0
+    for r in rows:
+        xi, Tm, mt = float(r["xi"]), float(r["T_MeV"]), float(r["mu_transition_MeV"])
+        if Tm > _T_cep(xi) - 10:
+            continue
+        mus = np.array([mt - 0.2, mt, mt + 0.2]) / HB
+        Ts = np.full(3, Tm / HB)
+        h = oracle.solve_batch(Ts, mus, xi, policy=oracle.SEED_HADRON, p_num=24, t_num=8)
+        q = oracle.solve_batch(Ts, mus, xi, policy=oracle.SEED_QUARK, p_num=24, t_num=8)
+        assert h["converged"].all() and q["converged"].all()
+        assert np.all(h["masses"][:, 0] - q["masses"][:, 0] > 50 / HB)      # two distinct branches
+        d = h["omega"] - q["omega"]
+        assert d[0] < 0 < d[2]                                             # hadron stable below, quark above
+        mu_star = mus[1] - d[1] * (mus[2] - mus[0]) / (d[2] - d[0])
+        assert abs(mu_star * HB - mt) < 0.03, (xi, Tm, mu_star * HB - mt)
+        assert abs(h["rho_norm"][1] - float(r["rho_hadron"])) < 0.01
+        assert abs(q["rho_norm"][1] - float(r["rho_quark"])) < 0.01
+        checked += 1
+    assert checked >= 30
+
+
+def _branch_end(oracle, T, xi, mus, first_policy):
+    """Follows one branch by continuation along mus; returns the first mu at
+    which the root leaves it (|delta M_u| > 20 MeV or no convergence)."""
+    x, M_prev = None, None
+    for m in mus:
+        if x is None:
+            s = oracle.solve_batch(np.array([T]), np.array([m]), xi, policy=first_policy, p_num=24, t_num=8)
+        else:
+            s = oracle.solve_batch(np.array([T]), np.array([m]), xi, policy=oracle.SEED_EXPLICIT, seed=x[None],
+                                   p_num=24, t_num=8, opts=oracle.default_opts(auto_multiseed_fallback=0))
+        M = s["masses"][0, 0] * HB
+        if not s["converged"][0] or (M_prev is not None and abs(M - M_prev) > 20):
+            return m
+        x, M_prev = s["x"][0], M
+    return None
+
+
+@pytest.mark.parametrize("xi", [0.0, 0.2, 0.4])
+def test_spinodals_match_reference(oracle, xi):
+    """spinodals.csv: following the hadron branch up in mu (the quark branch
+    down) by continuation, the branch ends within 0.1 MeV of the stored
+    spinodal, for T at least 15 MeV below the CEP."""
+    rows = [r for r in golden_rows("spinodals.csv") if float(r["xi"]) == xi]
+    Tc = _T_cep(xi)
+    rows = [r for r in rows if float(r["T_MeV"]) <= Tc - 15][::3]
+    assert len(rows) >= 4
+    step = 0.05
+    for r in rows:
+        T = float(r["T_MeV"]) / HB
+        mh, mq = float(r["mu_spinodal_hadron_MeV"]), float(r["mu_spinodal_quark_MeV"])
+        up = np.arange(mh - 1.5, mh + 1.5, step) / HB
+        end_h = _branch_end(oracle, T, xi, up, oracle.SEED_HADRON)
+        down = np.arange(mq + 1.5, mq - 1.5, -step) / HB
+        end_q = _branch_end(oracle, T, xi, down, oracle.SEED_QUARK)
+        assert end_h is not None and abs(end_h * HB - mh) < 2 * step + 0.02, (xi, r["T_MeV"], end_h * HB, mh)
+        assert end_q is not None and abs(end_q * HB - mq) < 2 * step + 0.02, (xi, r["T_MeV"], end_q * HB, mq)
+
+
 def test_seed_policy_semantics(oracle):
     """DefaultSeed auto hint / VERY_HIGH_TEMP switch, MultiSeed argmin Omega."""
     T = np.array([100, 160, 100, 320, 50]) / HB

@@ -111,3 +111,59 @@ def test_omega_judge_never_keeps_a_higher_omega_root(oracle, case):
         assert np.all(r["omega"][c] <= d["omega"][c])
     moved = ok & ~np.all(r["x"] == nomask["x"], axis=1)
     assert moved.any()
+
+
+def c5_lines(oracle, every=1):
+    """mu-lines of the multi-root scan at the temperatures of the reference's
+    Maxwell table (boundary.csv), T at least 15 MeV below the CEP: mu =
+    linspace(200, 450, 512) MeV (the spacing of the 512-point scan), p24/t8 as
+    the table.  Returns (lines, T, mu, xi) with lines = [(xi, T_MeV, mu_tr,
+    mu_spinodal_quark, mu_spinodal_hadron)]."""
+    import julia_relaxtime_amd.pnjl as P
+    from conftest import golden_rows
+    sp = {(float(r["xi"]), float(r["T_MeV"])): r for r in golden_rows("spinodals.csv")}
+    lines = []
+    for r in golden_rows("phase_boundary.csv"):
+        xi, Tm = float(r["xi"]), float(r["T_MeV"])
+        if Tm > P.load_phase_boundary(xi)[2][2] - 15 or (xi, Tm) not in sp:
+            continue
+        s = sp[(xi, Tm)]
+        lines.append((xi, Tm, float(r["mu_transition_MeV"]), float(s["mu_spinodal_quark_MeV"]),
+                      float(s["mu_spinodal_hadron_MeV"])))
+    lines = lines[::every]
+    mus = np.linspace(200, 450, 512)
+    T = np.concatenate([np.full(512, l[1]) for l in lines]) / HB
+    mu = np.tile(mus, len(lines)) / HB
+    xi = np.concatenate([np.full(512, l[0]) for l in lines])
+    return lines, T, mu, xi
+
+
+def check_c5_lines(lines, r):
+    """On every line: all points converge; the min-Omega root jumps (|dM_u| >
+    50 MeV) exactly once, within one grid step of the reference's Maxwell
+    mu_transition; every point inside the reference's spinodal interval
+    (1 MeV margin) carries the first-order flag."""
+    mus = np.linspace(200, 450, 512)
+    for k, (xi, Tm, mt, mq, mh) in enumerate(lines):
+        s = slice(512 * k, 512 * (k + 1))
+        assert np.all(r["converged"][s] == 1), (xi, Tm)
+        dM = np.abs(np.diff(r["masses"][s, 0])) * HB
+        assert np.sum(dM > 50) == 1, (xi, Tm)
+        j = int(np.argmax(dM))
+        assert mus[j] - 0.05 <= mt <= mus[j + 1] + 0.05, (xi, Tm, mus[j], mt)
+        inside = (mus > mq + 1) & (mus < mh - 1)
+        assert np.all((r["flags"][s][inside] & 0x40) != 0), (xi, Tm)
+
+
+def test_multiroot3_lines_switch_at_reference_transition(oracle):
+    """MULTIROOT3 (learned seed + HADRON + QUARK, min Omega) along mu at fixed
+    T reproduces the reference's first-order line (Maxwell construction on
+    the T-rho S-curves) and flags its coexistence region.  The learned-seed
+    candidate is the DefaultSeed root here, as any seed is admissible."""
+    lines, T, mu, xi = c5_lines(oracle, every=4)
+    assert len(lines) >= 8 and {l[0] for l in lines} == {0.0, 0.2, 0.4}
+    opts = oracle.default_opts(ftol=1e-10, max_iter=40)
+    plain = oracle.solve_batch(T, mu, xi, policy=oracle.SEED_DEFAULT, p_num=24, t_num=8, opts=opts, nthreads=8)
+    r = oracle.solve_batch(T, mu, xi, policy=oracle.SEED_MULTIROOT3, seed=plain["x"], p_num=24, t_num=8,
+                           opts=opts, nthreads=8)
+    check_c5_lines(lines, r)
