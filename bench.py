@@ -18,9 +18,28 @@ Workloads (grids are linspace with endpoints; MeV -> fm^-1 with 197.327):
                   the GPU, judge mask from those and 10^6 more probes
                   (default_rng(5)).  The plain DefaultSeed scan of the same
                   grid is timed with the same K/W and reported beside it.
+  c5_multiroot    512 x 512 x 5: T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2
+                  step 0.1 = 1,310,720 points, p = 64, t = 4, iterations 40.
+                  One step = MLP inference + MULTIROOT3 (learned seed, HADRON
+                  and QUARK seeds, minimum-Omega root, first-order flag);
+                  learned seed trained as for c4_surrogate.  The plain
+                  minimum-Omega policy, MultiSeed (6 seeds), is timed beside it.
+  c1_continuity   32 x 32 at xi = 0 (T 50-300 MeV, mu 0-400 MeV) = 1,024 points,
+                  p = 64, t = 8, one ContinuitySeed chain in scan order (T
+                  outer, mu inner, no reset; first seed DefaultSeed): a
+                  latency-bound serial chain, one lane group.
   c2_aniso_plain  64 x 64 x 9 (xi step 0.1) = 36,864 points, DefaultSeed.
+  c3_polarization 256 equilibrium points drawn without replacement
+                  (default_rng(20261202)) from the converged 32 x 32 (xi = 0,
+                  T 50-300 MeV, mu 0-400 MeV, p = 64, t = 8) and 64 x 64 x 9
+                  grids; Pi for {P, S} x {u ubar, u sbar, s sbar} on k0 =
+                  linspace(0, 4, 128) x k = linspace(0, 3, 64) fm^-1 =
+                  12,582,912 complex values; A_u, A_s with 64 momentum nodes
+                  on [0, 10] (4 cos nodes when xi != 0).  One step = one
+                  replay of the captured grid graph (A kernel + Pi kernel).
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -39,45 +58,152 @@ PEAK_FP64_TFLOPS = 78.6
 # gradient, Hessian and thermodynamic sums) as the device routine executes it:
 # FP64 FMA = 2, add/mul/transcendental = 1, measured with rocprofv3 FP64 VALU
 # counters on the residual kernel at p = 64 and p = 32
-# (python3 -m julia_relaxtime_amd.flop_probe): 198.1 + 2.0.
-C_NODE = 200.0
+# (python3 -m julia_relaxtime_amd.flop_probe): 176.0 + 2.0.
+C_NODE = 178.0
 
 
-def grid(nT, nmu, xis):
-    T, mu, xi = np.meshgrid(np.linspace(20, 300, nT), np.linspace(0, 450, nmu), np.asarray(xis, float),
+def grid(nT, nmu, xis, T_range=(20, 300), mu_range=(0, 450)):
+    T, mu, xi = np.meshgrid(np.linspace(*T_range, nT), np.linspace(*mu_range, nmu), np.asarray(xis, float),
                             indexing="ij")
     return T.ravel() / HB, mu.ravel() / HB, xi.ravel()
 
 
 WORKLOADS = {
     "c4_surrogate": dict(grid=lambda: grid(256, 256, np.round(np.arange(-0.4, 0.4001, 0.05), 10)), p_num=64,
-                         t_num=4, max_iter=40, learned=True,
+                         t_num=4, max_iter=40, learned=True, policy="SURROGATE", plain_policy="DEFAULT",
                          desc="256x256x17 (T 20-300 MeV, mu 0-450 MeV, xi -0.4..0.4 step 0.05), p=64 t=4, "
                               "learned seed (MLP 3-128-128-128-4, BF16 MFMA) -> Newton, 8-step cutoff, "
                               "DefaultSeed fallback, reference judge; FP64; |res|<1e-10"),
     "c2_aniso_plain": dict(grid=lambda: grid(64, 64, np.round(np.arange(-0.4, 0.41, 0.1), 10)), p_num=64, t_num=4,
-                           max_iter=40, learned=False,
+                           max_iter=40, learned=False, plain_policy="DEFAULT",
                            desc="64x64x9 (T 20-300 MeV, mu 0-450 MeV, xi -0.4..0.4), p=64 t=4, "
                                 "DefaultSeed per point, FP64, |res|<1e-10"),
+    "c1_continuity": dict(grid=lambda: grid(32, 32, [0.0], (50, 300), (0, 400)), p_num=64, t_num=8, max_iter=1000,
+                          learned=False, plain_policy="CONTINUITY", chain=True,
+                          desc="32x32 at xi=0 (T 50-300 MeV, mu 0-400 MeV), p=64 t=8, one ContinuitySeed chain "
+                               "(T outer, mu inner, no reset), FP64, |res|<1e-10"),
+    "c3_polarization": dict(polar=True, desc="256 points x {P,S} x {uu,us,ss} x 128 k0 x 64 k = 12,582,912 complex "
+                                              "Pi (polarization_aniso), hipGraph replay, FP64"),
+    "c5_multiroot": dict(grid=lambda: grid(512, 512, [-0.2, -0.1, 0.0, 0.1, 0.2], (10, 250), (200, 450)), p_num=64,
+                         t_num=4, max_iter=40, learned=True, policy="MULTIROOT3", plain_policy="MULTI",
+                         desc="512x512x5 (T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2 step 0.1), p=64 t=4, "
+                              "MULTIROOT3: learned seed + HADRON + QUARK Newton solves, minimum Omega, "
+                              "first-order flag; FP64; |res|<1e-10"),
 }
 
 
+C3_METRIC = "anisotropic one-loop polarization values/s (12,582,912-value Pi grid)"
+# FP64 FLOPs per complex Pi value of this grid, data-dependent and therefore
+# counted once: rocprofv3 FP64 VALU counters (FMA = 2) on 131,072 seeded cells
+# of the same grid evaluated one per wavefront, so that inactive lanes of
+# divergent waves are not counted (python3 -m julia_relaxtime_amd.pi_flop_probe):
+# 86,571 per cell (one symmetrised B0 + correction and one Pi), two Pi values
+# per cell.  The A kernel (512 values) is negligible.
+C3_FLOPS_PER_VALUE = 43286.0
+
+
+def c3_points(ctx, opts):
+    """The 256 equilibrium points of the Pi workload (T, mu, xi, M_u, M_s, Phi, Phib)."""
+    from julia_relaxtime_amd.workloads import pi_grid_points
+    return pi_grid_points(ctx, opts)
+
+
+def c3_cpu_baseline(pts, k0, k, n_sample=2_000_000, threads=None):
+    """The oracle's polarization_aniso (C restatement, OpenMP over values) on
+    a seeded sample of the grid's Pi values, A from the oracle as well."""
+    from oracle import oracle as O
+    threads = threads or min(os.cpu_count() or 1, 16)
+    xp, wp = O.gauleg(0.0, 10.0, 64)
+    xc, wc = np.polynomial.legendre.leggauss(4)
+    t0 = time.perf_counter()
+    A = np.array([[O.A_aniso(m, p[1], p[0], p[5], p[6], p[2], xp, wp, xc, wc) for m in (p[3], p[4])] for p in pts])
+    rng = np.random.default_rng(20261202)
+    n = n_sample
+    pt, pair, ch = rng.integers(0, len(pts), n), rng.integers(0, 3, n), rng.integers(1, 3, n)
+    i0, j = rng.integers(0, k0.size, n), rng.integers(0, k.size, n)
+    P = pts[pt]
+    m1 = np.where(pair == 2, P[:, 4], P[:, 3])
+    m2 = np.where(pair == 0, P[:, 3], P[:, 4])
+    A1 = np.where(pair == 2, A[pt, 1], A[pt, 0])
+    A2 = np.where(pair == 0, A[pt, 0], A[pt, 1])
+    qa = np.zeros(n, dtype=np.dtype([(f, np.int32 if t is C.c_int else np.float64) for f, t in O.PolQuery._fields_]))
+    for f, v in dict(channel=ch, num_s=pair, k0=k0[i0], k=k[j], m1=m1, m2=m2, mu1=P[:, 1], mu2=P[:, 1], T=P[:, 0],
+                     Phi=P[:, 5], Phib=P[:, 6], xi=P[:, 2], A1=A1, A2=A2).items():
+        qa[f] = v
+    q = (O.PolQuery * n).from_buffer(qa)
+    t1 = time.perf_counter()
+    O.polarization_batch(q, nthreads=threads)
+    dt = time.perf_counter() - t1
+    return dict(value=n / dt, unit="Pi values/s", cores=threads, kind="port",
+                sample=f"{n} seeded-random values of the same grid (rng 20261202), oracle polarization_aniso, "
+                       f"{threads} OpenMP threads, {dt:.2f} s wall (+{t1 - t0:.2f} s for 512 A values)")
+
+
+def run_polar(args, w):
+    import torch
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import polarization as PZ
+    dev = torch.device("cuda", 0)
+    ctx = _lib.Context(0)
+    pts = c3_points(ctx, _lib.default_opts(ftol=1e-10, max_iter=40))
+    from julia_relaxtime_amd.workloads import PI_K, PI_K0
+    k0, k = PI_K0, PI_K
+    ptd = torch.tensor(pts, device=dev)
+    k0d, kd = torch.tensor(k0, device=dev), torch.tensor(k, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    state = {}
+    pctx = PZ._ctx(0)
+
+    def step():
+        state["out"] = PZ.polarization_grid(ptd, k0d, kd, out=state.get("out"), stream=stream)
+        return pctx.last_kernel_ms()
+
+    ms_step, graph_ms = timed_steps(step, args.steps, args.warmup)
+    re, im = (t.cpu().numpy() for t in state["out"])
+    nval = re.size
+    graph_avg = float(np.mean(graph_ms))
+    flops = None if C3_FLOPS_PER_VALUE is None else C3_FLOPS_PER_VALUE * nval
+    achieved = None if flops is None else flops / (graph_avg * 1e-3) / 1e12
+    line = {
+        "metric": C3_METRIC, "value": nval / (ms_step * 1e-3), "unit": "Pi values/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(workload=args.workload, points=len(pts), values=nval, detail=w["desc"],
+                       nan_imag_values=int(np.isnan(im).sum()), nan_real_values=int(np.isnan(re).sum()),
+                       inf_values=int(np.isinf(re).sum() + np.isinf(im).sum()), graph_ms_per_step=graph_avg),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": None if achieved is None else achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "polar::grid_pi_kernel (+ grid_a_kernel)", "kernel_ms_per_step": graph_avg,
+                     "flops_per_value": C3_FLOPS_PER_VALUE},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = c3_cpu_baseline(pts, k0, k)
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
-    """The oracle (plain-C restatement of the reference's DefaultSeed solve)
-    on a seeded sample of the same grid, OpenMP over points."""
+    """The oracle (plain-C restatement of the reference's solve with the
+    workload's plain policy, DefaultSeed or MultiSeed) on a seeded sample of
+    the same grid, OpenMP over points."""
     from oracle import oracle as O
     threads = threads or min(os.cpu_count() or 1, 16)
     rng = np.random.default_rng(20261202)
     idx = rng.choice(T.size, min(n_sample, T.size), replace=False)
     opts = O.default_opts(ftol=1e-10, max_iter=w["max_iter"])
+    policy = getattr(O, "SEED_" + w["plain_policy"])
+    if w.get("chain"):
+        # a chain is sequential: the whole chain, one thread
+        threads, idx = 1, np.arange(T.size)
+        opts = O.default_opts(ftol=1e-10, max_iter=w["max_iter"], chain_len=T.size)
     t0 = time.perf_counter()
-    r = O.solve_batch(T[idx], mu[idx], xi[idx], policy=O.SEED_DEFAULT, p_num=w["p_num"], t_num=w["t_num"],
+    r = O.solve_batch(T[idx], mu[idx], xi[idx], policy=policy, p_num=w["p_num"], t_num=w["t_num"],
                       opts=opts, nthreads=threads)
     dt = time.perf_counter() - t0
     solved = int(np.sum((r["converged"] == 1) & (r["residual_norm"] < 1e-10)))
+    name = {"DEFAULT": "DefaultSeed", "MULTI": "MultiSeed", "CONTINUITY": "ContinuitySeed"}[w["plain_policy"]]
+    what = "the whole chain" if w.get("chain") else f"{idx.size} seeded-random points of the same grid (rng 20261202)"
     return dict(value=solved / dt, unit="points/s", cores=threads, kind="port",
-                sample=f"{idx.size} seeded-random points of the same grid (rng 20261202), plain DefaultSeed "
-                       f"solve, {threads} OpenMP threads, {dt:.2f} s wall")
+                sample=f"{what}, plain {name} solve, {threads} OpenMP threads, {dt:.2f} s wall")
 
 
 def train_learned_seed(ctx, w, opts):
@@ -135,11 +261,13 @@ def main():
     from julia_relaxtime_amd import _lib
 
     w = WORKLOADS[args.workload]
+    if w.get("polar"):
+        return run_polar(args, w)
     T, mu, xi = w["grid"]()
     n = T.size
     dev = torch.device("cuda", 0)
     ctx = _lib.Context(0)
-    opts = _lib.default_opts(ftol=1e-10, max_iter=w["max_iter"])
+    opts = _lib.default_opts(ftol=1e-10, max_iter=w["max_iter"], chain_len=n if w.get("chain") else 0)
     Td, mud, xid = (torch.tensor(a, device=dev) for a in (T, mu, xi))
     stream = torch.cuda.current_stream(dev)
     t_eff = np.where(xi == 0.0, 1, w["t_num"])
@@ -152,14 +280,16 @@ def main():
     if args.launch:
         v = [int(t) for t in args.launch.split(",")] + [0]
         fixed = dict(lanes_per_point=v[0], chunk=v[1], waves_per_cu=v[2], tail_lanes_per_point=v[3])
-    # plain DefaultSeed scan, its launch geometry autotuned on the full grid
-    plain_cfg = fixed or ctx.autotune(T, mu, xi, policy=_lib.SEED_DEFAULT, p_num=w["p_num"], t_num=w["t_num"],
+    # the plain scan (DefaultSeed, or MultiSeed for minimum-Omega workloads),
+    # its launch geometry autotuned on the full grid
+    plain_policy = getattr(_lib, "SEED_" + w["plain_policy"])
+    plain_cfg = fixed or ctx.autotune(T, mu, xi, policy=plain_policy, p_num=w["p_num"], t_num=w["t_num"],
                                       opts=opts)[0]
     plain = {}
 
     def plain_step():
         ctx.set_launch_cfg(**plain_cfg)
-        plain["out"] = ctx.solve_batch(Td, mud, xid, policy=_lib.SEED_DEFAULT, p_num=w["p_num"], t_num=w["t_num"],
+        plain["out"] = ctx.solve_batch(Td, mud, xid, policy=plain_policy, p_num=w["p_num"], t_num=w["t_num"],
                                        opts=opts, out=plain.get("out"), stream=stream, sync=True)
         return float(np.sum(ctx.last_scan_ms()))
 
@@ -173,13 +303,14 @@ def main():
         model, extra = train_learned_seed(ctx, w, opts)
         ctx.set_judge_mask(model.judge_mask, model.in_lo, model.in_hi)
         state = {"seeds": torch.empty((n, 5), dtype=torch.float64, device=dev)}
-        cfg = fixed or ctx.autotune(T, mu, xi, policy=_lib.SEED_SURROGATE, seed=model.infer(T, mu, xi),
+        policy = getattr(_lib, "SEED_" + w["policy"])
+        cfg = fixed or ctx.autotune(T, mu, xi, policy=policy, seed=model.infer(T, mu, xi),
                                     p_num=w["p_num"], t_num=w["t_num"], opts=opts)[0]
 
         def step():
             ctx.set_launch_cfg(**cfg)
             model.infer(Td, mud, xid, out=state["seeds"], stream=stream)
-            state["out"] = ctx.solve_batch(Td, mud, xid, policy=_lib.SEED_SURROGATE, seed=state["seeds"],
+            state["out"] = ctx.solve_batch(Td, mud, xid, policy=policy, seed=state["seeds"],
                                            p_num=w["p_num"], t_num=w["t_num"], opts=opts, out=state.get("out"),
                                            stream=stream, sync=True)
             state["stage_ms"] = ctx.last_scan_ms()
@@ -202,11 +333,20 @@ def main():
             plain_scan_kernel_ms=float(np.mean(scan_plain)), plain_launch=plain_cfg,
             scan_stage_ms=[round(float(v), 3) for v in state["stage_ms"]],
             plain_fp64_tflops=flops_plain / (np.mean(scan_plain) * 1e-3) / 1e12)
-        seed_policy = "SurrogateSeed(judge=reference)"
+        if w["policy"] == "MULTIROOT3":
+            eo = (Q["omega"] - P["omega"]) / np.abs(P["omega"])
+            extra.update(
+                plain_policy="MultiSeed", first_order_fraction=float(np.mean((Q["flags"] & _lib.PT_FIRST_ORDER) != 0)),
+                omega_at_most_multiseed=float(np.mean(eo <= 1e-12)), multiseed_lower_omega=int(np.sum(eo > 1e-12)))
+            for k in ("fallback_fraction", "judged_fraction"):
+                extra.pop(k)
+            seed_policy = "MULTIROOT3 (learned seed + HADRON + QUARK, min Omega)"
+        else:
+            seed_policy = "SurrogateSeed(judge=reference)"
     else:
         ms_step, scan_ms, flops, solved = ms_plain, scan_plain, flops_plain, solved_plain
         Q = P
-        seed_policy = "DefaultSeed"
+        seed_policy = {"DEFAULT": "DefaultSeed", "CONTINUITY": "ContinuitySeed (one chain)"}[w["plain_policy"]]
 
     scan_avg = float(np.mean(scan_ms))
     achieved = flops / (scan_avg * 1e-3) / 1e12

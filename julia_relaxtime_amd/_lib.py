@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_PKG, "libpnjl_hip.so")
 
 # status codes
 PNJL_OK, PNJL_ERR_ARG, PNJL_ERR_HIP, PNJL_ERR_NO_DEVICE, PNJL_ERR_CHANNEL, PNJL_ERR_STATE = range(6)
-FLAG_DEVICE_PTRS, FLAG_ASYNC = 0x1, 0x2
+FLAG_DEVICE_PTRS, FLAG_ASYNC, FLAG_ONE_PER_WAVE = 0x1, 0x2, 0x4
 
 # seed policies
 SEED_DEFAULT, SEED_HADRON, SEED_QUARK, SEED_MULTI, SEED_EXPLICIT = 0, 1, 2, 3, 4

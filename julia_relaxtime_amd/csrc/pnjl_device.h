@@ -22,6 +22,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// The node functions evaluate exactly the written operations: every fused
+// multiply-add is an explicit fma(), nothing is contracted by the compiler.
+// Instances of the node loop compiled in different contexts (the checked and
+// unchecked loops) then round identically, and results do not depend on which
+// states share a wavefront.
+#define NODE_MATH_EXACT _Pragma("clang fp contract(off)")
+
 namespace pnjl {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -69,6 +76,7 @@ __device__ __forceinline__ void flav_zero(FlavSums& s)
 // handling -- the node energies E^2 = p^2 + M^2 are positive and normal.
 __device__ __forceinline__ void rsqrt_sqrt(double x, double& rs, double& sq)
 {
+    NODE_MATH_EXACT
     double y = __builtin_amdgcn_rsq(x);
     double g = x * y, h = 0.5 * y;
     double e = fma(-g, h, 0.5);
@@ -85,6 +93,7 @@ __device__ __forceinline__ void rsqrt_sqrt(double x, double& rs, double& sq)
 // steps r <- r + r (1 - d r), within about 1 ulp.
 __device__ __forceinline__ double rcp_nr(double d)
 {
+    NODE_MATH_EXACT
     double r = __builtin_amdgcn_rcp(d);
     double e = fma(-d, r, 1.0);
     r = fma(r, e, r);
@@ -92,17 +101,18 @@ __device__ __forceinline__ double rcp_nr(double d)
     return fma(r, e, r);
 }
 
-// exp(x) for -800 <= x <= 0 after clamping (Tang's table-driven method, ACM
+// exp(x) for -800 <= x <= 700 after clamping (Tang's table-driven method, ACM
 // TOMS 15(2), 1989): x = (64 k + j) ln2/64 + r with |r| <= ln2/128 (Cody-Waite
 // reduction with a split ln2/64), exp(x) = 2^k 2^(j/64) e^r and e^r from its
 // degree-5 Taylor polynomial (truncation below 4e-17).  About 1 ulp; results
 // below the double range flush through ldexp.
 __device__ __forceinline__ double exp_tab(double x, const double* tab)
 {
+    NODE_MATH_EXACT
     const double k64_ln2 = 0x1.71547652b82fep+6;     // 64 / ln 2
     const double ln2_64_hi = 0x1.62e42fee00000p-7;   // ln2/64, 21 trailing zero bits
     const double ln2_64_lo = 0x1.a39ef35793c76p-39;
-    x = fmax(x, -800.0);
+    x = fmin(fmax(x, -800.0), 700.0);
     const double nd = rint(x * k64_ln2);
     double r = fma(nd, -ln2_64_hi, x);
     r = fma(nd, -ln2_64_lo, r);
@@ -122,6 +132,7 @@ __device__ __forceinline__ double exp_tab(double x, const double* tab)
 // error about 1e-16 for the arguments used here.
 __device__ __forceinline__ double log_tab(double x, const double* tab)
 {
+    NODE_MATH_EXACT
     const double ln2_hi = 0x1.62e42fee00000p-1;
     const double ln2_lo = 0x1.a39ef35793c76p-33;
     int e;
@@ -139,15 +150,22 @@ __device__ __forceinline__ double log_tab(double x, const double* tab)
     return fma(ed, ln2_hi, t.y) + (r + lo);
 }
 
-// Slow path for one sign when its log argument is below the 1e-16 floor (or
-// not positive): returns the floored value and zero derivatives except the
-// 3 from the LSE shift, exactly like log(max(term, 1e-16)) under AD.
+// Per-sign terms of ln F(a) = ln(1 + 3 c1 e^a + 3 c2 e^2a + e^3a) in the
+// normalisation the node loop accumulates (the factors 3 and 9 of the
+// derivatives are applied once per flavour sum):
+//   v = ln F,  gq = (dlnF/da) / 3,  h = (d2lnF/da2) / 3,
+//   k1 = (d2lnF/da dc1) / 3,  k2 = (d2lnF/da dc2) / 3,
+//   r1 = e^a / F,  r2 = e^2a / F  (dlnF/dc_i = 3 r_i),  f = distribution.
 struct Sign1 {
-    double v, g, aa, ac1, ac2, r1, r2, f;
+    double v, gq, h, k1, k2, r1, r2, f;
 };
 
+// Slow path for one sign: the reference's shifted form for a > 0 and its
+// 1e-16 floor on the log argument -- a floored value has zero derivatives
+// except the 3 of the shift, exactly like log(max(term, 1e-16)) under AD.
 __device__ __forceinline__ void lnF_slow(const double* tab, double a, double c1, double c2, Sign1& o)
 {
+    NODE_MATH_EXACT
     double t = exp_tab(-fabs(a), tab);
     double t2 = t * t, t3 = t2 * t;
     bool pos = a > 0.0;
@@ -156,8 +174,8 @@ __device__ __forceinline__ void lnF_slow(const double* tab, double a, double c1,
     double shift = pos ? 3.0 * a : 0.0;
     if (!(D >= kPolyakovEps)) {
         o.v = shift + kLogPolyakovEps;
-        o.g = pos ? 3.0 : 0.0;
-        o.aa = o.ac1 = o.ac2 = o.r1 = o.r2 = 0.0;
+        o.gq = pos ? 1.0 : 0.0;
+        o.h = o.k1 = o.k2 = o.r1 = o.r2 = 0.0;
         // distribution (used for densities) keeps its closed form
         o.f = (c1 * n1 + 2.0 * c2 * n2 + n3) / D;
         return;
@@ -166,79 +184,98 @@ __device__ __forceinline__ void lnF_slow(const double* tab, double a, double c1,
     double r1 = n1 * iD, r2 = n2 * iD, r3 = n3 * iD;
     double f = c1 * r1 + 2.0 * c2 * r2 + r3;
     o.v = shift + log_tab(D, tab);
-    o.g = 3.0 * f;
-    o.aa = 3.0 * (c1 * r1 + 4.0 * c2 * r2 + 3.0 * r3) - 9.0 * f * f;
-    o.ac1 = 3.0 * r1 * (1.0 - 3.0 * f);
-    o.ac2 = 3.0 * r2 * (2.0 - 3.0 * f);
+    o.gq = f;
+    o.h = (c1 * r1 + 4.0 * c2 * r2 + 3.0 * r3) - 3.0 * f * f;
+    o.k1 = r1 * (1.0 - 3.0 * f);
+    o.k2 = r2 * (2.0 - 3.0 * f);
     o.r1 = r1;
     o.r2 = r2;
     o.f = f;
 }
 
-// One node of one flavour: adds C-weighted contributions to `s`.
-// K2mu = exp(-2 mu / T): for mu >= 0 and a <= 0 both exponents are
-// non-positive and exp(-|b|) = exp(-|a|) K2mu, so one exp serves both signs.
-// q = p^2 (1 + xi cos^2), M2 = M^2, lmu = mu/T, invT = 1/T, tmu = 2 mu / T.
-__device__ __forceinline__ void node_accumulate(const double* tab, double q, double C, double M2, double mu,
+// Largest exponent taken on the unscaled fast path: e^(3a) stays below 1e131,
+// so the product of both signs' arguments is finite.
+constexpr double kFastExpMax = 100.0;
+
+// True when no node of a flavour can leave the fast path.  With Phi, Phib >=
+// -d the argument D = 1 + 3 Phi t + 3 Phib t^2 + t^3 is >= (1 - 6d) for t <= 1
+// and D / t^3 >= (1 - 6d) for t >= 1, so at d = 0.1 every log argument is
+// >= 0.4 max(1, e^3a), far from the 1e-16 floor; and a = (mu - E)/T < mu/T,
+// b = -(E + mu)/T < -mu/T bound both exponents by |mu|/T.
+__device__ __forceinline__ bool nodes_all_fast(double Phi, double Phib, double muT)
+{
+    return Phi >= -0.1 && Phib >= -0.1 && fabs(muT) <= kFastExpMax;
+}
+
+// One node of one flavour: adds C-weighted raw terms to `s` (scaled to the
+// FlavSums definitions by flav_finish).
+// The fast path works with the unscaled arguments D = 1 + 3 c1 t + 3 c2 t^2 +
+// t^3, t = e^a (resp. e^b), for either sign of the exponents: ln F is the same
+// function as the reference's shifted form 3a + ln(e^-3a + ...) for a > 0,
+// and the ratios t^k / D are scale-free, so no branch on sign(a) is needed
+// and e^b = e^a exp(-2 mu / T) = e^a K2mu costs one multiply.  The
+// reference's 1e-16 floor applies to the argument it takes the log of, which
+// for a > 0 is D e^-3a: with kChecked the fast path requires
+// D >= 1e-16 max(1, t^3) for both signs and a, b <= kFastExpMax, and
+// everything else goes to the per-sign shifted slow path; without kChecked
+// the caller has established nodes_all_fast.
+// q = p^2 (1 + xi cos^2), M2 = M^2, muT = mu/T, invT = 1/T, K2mu = exp(-2 mu/T).
+template <bool kChecked>
+__device__ __forceinline__ void node_accumulate(const double* tab, double q, double C, double M2, double muT,
                                                 double invT, double K2mu, double Phi, double Phib, FlavSums& s)
 {
+    NODE_MATH_EXACT
     double E, invE;
     rsqrt_sqrt(q + M2, invE, E);
-    double a = (mu - E) * invT;
-    double b = -(E + mu) * invT;
+    const double a = fma(-E, invT, muT);
+    const double b = fma(-E, invT, -muT);
 
-    // quark: (c1,c2) = (Phi, Phib), argument a
-    double ta = exp_tab(-fabs(a), tab);
-    double ta2 = ta * ta, ta3 = ta2 * ta;
-    bool pa = a > 0.0;
-    double a0 = pa ? ta3 : 1.0, a1 = pa ? ta2 : ta, a2 = pa ? ta : ta2, a3 = pa ? 1.0 : ta3;
-    double Dp = a0 + 3.0 * (Phi * a1 + Phib * a2) + a3;
-    // antiquark: (c1,c2) = (Phib, Phi), argument b = a - 2 mu / T
-    double tb;
-    if (a <= 0.0 && K2mu <= 1.0)   // K2mu <= 1 <=> mu >= 0, then b <= 0 too
-        tb = ta * K2mu;
-    else
-        tb = exp_tab(-fabs(b), tab);
-    double tb2 = tb * tb, tb3 = tb2 * tb;
-    bool pb = b > 0.0;
-    double b0 = pb ? tb3 : 1.0, b1 = pb ? tb2 : tb, b2 = pb ? tb : tb2, b3 = pb ? 1.0 : tb3;
-    double Dm = b0 + 3.0 * (Phib * b1 + Phi * b2) + b3;
+    // quark: (c1, c2) = (Phi, Phib), t = e^a; antiquark: (Phib, Phi), e^b
+    const double ta = exp_tab(a, tab);
+    const double tb = ta * K2mu;
+    const double ta2 = ta * ta, ta3 = ta2 * ta;
+    const double tb2 = tb * tb, tb3 = tb2 * tb;
+    const double Dp = fma(3.0, fma(Phi, ta, Phib * ta2), 1.0) + ta3;
+    const double Dm = fma(3.0, fma(Phib, tb, Phi * tb2), 1.0) + tb3;
 
-    double Q, gp, gm, aap, aam, ac1p, ac2p, ac1m, ac2m, r1p, r2p, r1m, r2m, fp, fm;
-    if (Dp >= kPolyakovEps && Dm >= kPolyakovEps) {
-        double DD = Dp * Dm;
-        double iDD = rcp_nr(DD);
-        double iDp = Dm * iDD, iDm = Dp * iDD;
-        Q = log_tab(DD, tab) + (pa ? 3.0 * a : 0.0) + (pb ? 3.0 * b : 0.0);
-        r1p = a1 * iDp;
-        r2p = a2 * iDp;
-        double r3p = a3 * iDp;
-        r1m = b1 * iDm;
-        r2m = b2 * iDm;
-        double r3m = b3 * iDm;
-        fp = Phi * r1p + 2.0 * Phib * r2p + r3p;
-        fm = Phib * r1m + 2.0 * Phi * r2m + r3m;
-        gp = 3.0 * fp;
-        gm = 3.0 * fm;
-        aap = 3.0 * (Phi * r1p + 4.0 * Phib * r2p + 3.0 * r3p) - gp * gp;
-        aam = 3.0 * (Phib * r1m + 4.0 * Phi * r2m + 3.0 * r3m) - gm * gm;
-        ac1p = 3.0 * r1p * (1.0 - gp);
-        ac2p = 3.0 * r2p * (2.0 - gp);
-        ac1m = 3.0 * r1m * (1.0 - gm);
-        ac2m = 3.0 * r2m * (2.0 - gm);
+    double Q, gs, hs, k1p, k2p, k1m, k2m, r1p, r2p, r1m, r2m, fp, fm;
+    bool fast = true;
+    if (kChecked)
+        fast = Dp >= kPolyakovEps * fmax(1.0, ta3) && Dm >= kPolyakovEps * fmax(1.0, tb3) &&
+               fmax(a, b) <= kFastExpMax;
+    if (fast) {
+        const double DD = Dp * Dm;
+        const double iDD = rcp_nr(DD);
+        const double iDp = Dm * iDD, iDm = Dp * iDD;
+        Q = log_tab(DD, tab);
+        r1p = ta * iDp;
+        r2p = ta2 * iDp;
+        const double r3p = ta3 * iDp;
+        r1m = tb * iDm;
+        r2m = tb2 * iDm;
+        const double r3m = tb3 * iDm;
+        const double u1p = Phi * r1p, u1m = Phib * r1m;
+        fp = fma(2.0 * Phib, r2p, u1p) + r3p;
+        fm = fma(2.0 * Phi, r2m, u1m) + r3m;
+        const double hp = fma(-3.0 * fp, fp, fma(3.0, r3p, fma(4.0 * Phib, r2p, u1p)));
+        const double hm = fma(-3.0 * fm, fm, fma(3.0, r3m, fma(4.0 * Phi, r2m, u1m)));
+        k1p = r1p * fma(-3.0, fp, 1.0);
+        k2p = r2p * fma(-3.0, fp, 2.0);
+        k1m = r1m * fma(-3.0, fm, 1.0);
+        k2m = r2m * fma(-3.0, fm, 2.0);
+        gs = fp + fm;
+        hs = hp + hm;
     } else {
         Sign1 sp, sm;
         lnF_slow(tab, a, Phi, Phib, sp);
         lnF_slow(tab, b, Phib, Phi, sm);
         Q = sp.v + sm.v;
-        gp = sp.g;
-        gm = sm.g;
-        aap = sp.aa;
-        aam = sm.aa;
-        ac1p = sp.ac1;
-        ac2p = sp.ac2;
-        ac1m = sm.ac1;
-        ac2m = sm.ac2;
+        gs = sp.gq + sm.gq;
+        hs = sp.h + sm.h;
+        k1p = sp.k1;
+        k2p = sp.k2;
+        k1m = sm.k1;
+        k2m = sm.k2;
         r1p = sp.r1;
         r2p = sp.r2;
         r1m = sm.r1;
@@ -246,24 +283,37 @@ __device__ __forceinline__ void node_accumulate(const double* tab, double q, dou
         fp = sp.f;
         fm = sm.f;
     }
-    double QE = -(gp + gm) * invT;
-    double QEE = (aap + aam) * (invT * invT);
-    double QEP = -(ac1p + ac2m) * invT;
-    double QEPb = -(ac2p + ac1m) * invT;
-    double CinvE = C * invE;
+    const double CinvE = C * invE;
     s.S = fma(C, Q, s.S);
-    s.A1 = fma(CinvE, QE, s.A1);
-    s.A2 = fma(CinvE * invE, QEE - QE * invE, s.A2);
-    s.SP = fma(C, 3.0 * (r1p + r2m), s.SP);
-    s.SPb = fma(C, 3.0 * (r2p + r1m), s.SPb);
-    s.B1 = fma(CinvE, QEP, s.B1);
-    s.B2 = fma(CinvE, QEPb, s.B2);
-    s.SPP = fma(C, -9.0 * (r1p * r1p + r2m * r2m), s.SPP);
-    s.SPPb = fma(C, -9.0 * (r1p * r2p + r1m * r2m), s.SPPb);
-    s.SPbPb = fma(C, -9.0 * (r2p * r2p + r1m * r1m), s.SPbPb);
+    s.A1 = fma(CinvE, gs, s.A1);
+    s.A2 = fma(CinvE * invE, fma(hs, invT, gs * invE), s.A2);
+    s.B1 = fma(CinvE, k1p + k2m, s.B1);
+    s.B2 = fma(CinvE, k2p + k1m, s.B2);
+    s.SP = fma(C, r1p + r2m, s.SP);
+    s.SPb = fma(C, r2p + r1m, s.SPb);
+    s.SPP = fma(C, fma(r1p, r1p, r2m * r2m), s.SPP);
+    s.SPPb = fma(C, fma(r1p, r2p, r1m * r2m), s.SPPb);
+    s.SPbPb = fma(C, fma(r2p, r2p, r1m * r1m), s.SPbPb);
     s.Sfp = fma(C, fp, s.Sfp);
     s.Sfm = fma(C, fm, s.Sfm);
-    s.Sfab = fma(C, fp * a + fm * b, s.Sfab);
+    s.Sfab = fma(C, fma(fp, a, fm * b), s.Sfab);
+}
+
+// Scales the raw node sums to the FlavSums definitions: the derivative
+// factors Q_E = -3 gs/T, Q_EE = 3 hs/T^2, Q_EPhi = -3 (k1p + k2m)/T,
+// Q_Phi = 3 (r1p + r2m), Q_PhiPhi = -9 (r1p^2 + r2m^2), ...
+__device__ __forceinline__ void flav_finish(FlavSums& s, double invT)
+{
+    const double t3 = 3.0 * invT;
+    s.A1 *= -t3;
+    s.A2 *= t3;
+    s.B1 *= -t3;
+    s.B2 *= -t3;
+    s.SP *= 3.0;
+    s.SPb *= 3.0;
+    s.SPP *= -9.0;
+    s.SPPb *= -9.0;
+    s.SPbPb *= -9.0;
 }
 
 // vacuum_integral (src/pnjl/core/Integrals.jl:123-132) and its M-derivatives.

@@ -415,25 +415,29 @@ __device__ void tilde_B0(const Phys& P, const Tabs& q, int sg, double lam, doubl
     else tilde_B0_kpos(P, q, sg, lam, k, m, mp, mu, T, Phi, Phib, re, im);
 }
 
-// B0  OneLoopIntegrals.jl:489-503
+// B0  OneLoopIntegrals.jl:489-503: B~+(-l; m1, m2, mu1) - B~-(l; m1, m2, mu1)
+// + B~+(l; m2, m1, mu2) - B~-(-l; m2, m1, mu2), summed in this order.  One
+// loop over the four terms keeps a single copy of tilde_B0 in the kernel.
 __device__ void B0(const Phys& P, const Tabs& q, double lam, double k, double m1, double mu1, double m2, double mu2,
                    double T, double Phi, double Phib, double& re, double& im)
 {
-    double r, i;
     re = 0.0;
     im = 0.0;
-    tilde_B0(P, q, PLUS, -lam, k, m1, m2, mu1, T, Phi, Phib, r, i);
-    re = r;
-    im = i;
-    tilde_B0(P, q, MINUS, lam, k, m1, m2, mu1, T, Phi, Phib, r, i);
-    re -= r;
-    im -= i;
-    tilde_B0(P, q, PLUS, lam, k, m2, m1, mu2, T, Phi, Phib, r, i);
-    re += r;
-    im += i;
-    tilde_B0(P, q, MINUS, -lam, k, m2, m1, mu2, T, Phi, Phib, r, i);
-    re -= r;
-    im -= i;
+#pragma unroll 1
+    for (int t = 0; t < 4; t++) {
+        const bool swap = t >= 2;
+        const double l = (t == 0 || t == 3) ? -lam : lam;
+        double r, i;
+        tilde_B0(P, q, (t & 1) ? MINUS : PLUS, l, k, swap ? m2 : m1, swap ? m1 : m2, swap ? mu2 : mu1, T, Phi, Phib,
+                 r, i);
+        if (t & 1) {
+            re -= r;
+            im -= i;
+        } else {
+            re += r;
+            im += i;
+        }
+    }
 }
 
 // ---- first-order anisotropy  OneLoopIntegralsAniso.jl ---------------------
@@ -626,23 +630,24 @@ __device__ void tilde_Bc(const Phys& P, const Tabs& q, int sg, double lam, doubl
     else tilde_Bc_k0(P, q, sg, lam, m, mp, mu, T, Phi, Phib, xi, re, im);
 }
 
-// B0_correction  OneLoopIntegralsAniso.jl:396-413 (all four terms added)
+// B0_correction  OneLoopIntegralsAniso.jl:396-413 (all four terms added):
+// dB~q(-l; m1, m2, mu1) + dB~qbar(l; m1, m2, mu1) + dB~q(l; m2, m1, mu2)
+// + dB~qbar(-l; m2, m1, mu2).
 __device__ void B0_corr(const Phys& P, const Tabs& q, double lam, double k, double m1, double m2, double mu1,
                         double mu2, double T, double Phi, double Phib, double xi, double& re, double& im)
 {
-    double r, i;
-    tilde_Bc(P, q, QUARK, -lam, k, m1, m2, mu1, T, Phi, Phib, xi, r, i);
-    re = r;
-    im = i;
-    tilde_Bc(P, q, ANTIQUARK, lam, k, m1, m2, mu1, T, Phi, Phib, xi, r, i);
-    re += r;
-    im += i;
-    tilde_Bc(P, q, QUARK, lam, k, m2, m1, mu2, T, Phi, Phib, xi, r, i);
-    re += r;
-    im += i;
-    tilde_Bc(P, q, ANTIQUARK, -lam, k, m2, m1, mu2, T, Phi, Phib, xi, r, i);
-    re += r;
-    im += i;
+    re = 0.0;
+    im = 0.0;
+#pragma unroll 1
+    for (int t = 0; t < 4; t++) {
+        const bool swap = t >= 2;
+        const double l = (t == 0 || t == 3) ? -lam : lam;
+        double r, i;
+        tilde_Bc(P, q, (t & 1) ? ANTIQUARK : QUARK, l, k, swap ? m2 : m1, swap ? m1 : m2, swap ? mu2 : mu1, T, Phi,
+                 Phib, xi, r, i);
+        re += r;
+        im += i;
+    }
 }
 
 // The symmetrised B0 (+ B0_correction) of polarization_aniso
@@ -650,26 +655,24 @@ __device__ void B0_corr(const Phys& P, const Tabs& q, double lam, double k, doub
 __device__ void b0_total(const Phys& P, const Tabs& q, double k0, double k, double m1, double m2, double mu1,
                          double mu2, double T, double Phi, double Phib, double xi, int num_s, double& br, double& bi)
 {
-    const double lam = k0 + mu1 - mu2;
-    B0(P, q, lam, k, m1, mu1, m2, mu2, T, Phi, Phib, br, bi);
-    const double lx = -k0 + mu1 - mu2;
-    if (num_s == 1) {
-        double r2, i2;
-        B0(P, q, lx, k, m1, mu1, m2, mu2, T, Phi, Phib, r2, i2);
-        br = 0.5 * (br + r2);
-        bi = 0.5 * (bi + i2);
+    // lambda = k0 + mu1 - mu2; num_s = 1 averages with lambda' = -k0 + mu1 - mu2
+    const int nv = num_s == 1 ? 2 : 1;
+    double vr[2], vi[2];
+#pragma unroll 1
+    for (int v = 0; v < nv; v++) {
+        const double l = (v == 0 ? k0 : -k0) + mu1 - mu2;
+        B0(P, q, l, k, m1, mu1, m2, mu2, T, Phi, Phib, vr[v], vi[v]);
     }
+    br = nv == 2 ? 0.5 * (vr[0] + vr[1]) : vr[0];
+    bi = nv == 2 ? 0.5 * (vi[0] + vi[1]) : vi[0];
     if (fabs(xi) > kEpsSeg) {
-        double cr, ci;
-        B0_corr(P, q, lam, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, cr, ci);
-        if (num_s == 1) {
-            double r2, i2;
-            B0_corr(P, q, lx, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, r2, i2);
-            cr = 0.5 * (cr + r2);
-            ci = 0.5 * (ci + i2);
+#pragma unroll 1
+        for (int v = 0; v < nv; v++) {
+            const double l = (v == 0 ? k0 : -k0) + mu1 - mu2;
+            B0_corr(P, q, l, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, vr[v], vi[v]);
         }
-        br += cr;
-        bi += ci;
+        br += nv == 2 ? 0.5 * (vr[0] + vr[1]) : vr[0];
+        bi += nv == 2 ? 0.5 * (vi[0] + vi[1]) : vi[0];
     }
 }
 
@@ -736,12 +739,15 @@ __device__ __forceinline__ void load_tabs(double* lds, const double* tab)
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) query_kernel(Phys P, const double* tab, int64_t n, const pnjl_pol_query* qs,
-                                                    double* re, double* im)
+// query i = thread / stride (stride 1, or 64: one query per wavefront)
+__global__ void __launch_bounds__(256, 3) query_kernel(Phys P, const double* tab, int64_t n, const pnjl_pol_query* qs,
+                                                    double* re, double* im, int stride)
 {
     __shared__ double lds[kTabLen];
     load_tabs(lds, tab);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t % stride) return;
+    const int64_t i = t / stride;
     if (i >= n) return;
     const Tabs q = make_tabs(lds);
     const pnjl_pol_query a = qs[i];
@@ -755,7 +761,7 @@ __global__ void __launch_bounds__(256) query_kernel(Phys P, const double* tab, i
              im[i]);
 }
 
-__global__ void __launch_bounds__(256) b0_kernel(Phys P, const double* tab, int64_t n, const double* args, int corr,
+__global__ void __launch_bounds__(256, 3) b0_kernel(Phys P, const double* tab, int64_t n, const double* args, int corr,
                                                  double* re, double* im)
 {
     __shared__ double lds[kTabLen];
@@ -795,8 +801,10 @@ __global__ void __launch_bounds__(256) grid_a_kernel(Phys P, int npts, const pnj
     A[i] = A_value(P, m, s.mu, s.T, s.Phi, s.Phib, s.xi, np, xp, wp, nc, xc, wc);
 }
 
-// grid: one thread per (point, pair, k0, k); both channels written
-__global__ void __launch_bounds__(256) grid_pi_kernel(Phys P, const double* tab, int npts, const pnjl_pol_point* pts,
+// grid: one thread per (point, pair, k0, k); both channels written.
+// The quadrature kernels are held to 3 waves per SIMD (168 VGPRs, a few
+// spills): latency hiding pays more than the spill traffic costs.
+__global__ void __launch_bounds__(256, 3) grid_pi_kernel(Phys P, const double* tab, int npts, const pnjl_pol_point* pts,
                                                       const double* A, int nk0, const double* k0s, int nk,
                                                       const double* ks, double* re, double* im)
 {
@@ -951,7 +959,9 @@ int pnjl_polarization_batch(pnjl_ctx* ctx, int64_t n, const pnjl_pol_query* q, d
         dim = (double*)c;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
-    hipLaunchKernelGGL(polar::query_kernel, dim3(blocks_for(n)), dim3(256), 0, st, phys(ctx), tab, n, dq, dre, dim);
+    const int stride = (flags & PNJL_FLAG_ONE_PER_WAVE) ? 64 : 1;
+    hipLaunchKernelGGL(polar::query_kernel, dim3(blocks_for(n * stride)), dim3(256), 0, st, phys(ctx), tab, n, dq, dre,
+                       dim, stride);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
     if (!dev) {

@@ -227,6 +227,31 @@ __device__ __forceinline__ void store_flav(const Scr& s, int f, const FlavSums& 
     s.at(b + SF_Sfab) = a.Sfab;
 }
 
+// This lane's share of the node sums of one flavour: nodes j = gl, gl + LPP, ...
+template <int LPP, bool kChecked>
+__device__ __forceinline__ void node_loop(const double* lds, const double* pp, const double* cc, const double* Cm,
+                                          const double* Cs, int pn, int tn, double M2, double muT, double invT,
+                                          double K2mu, double xi, double Phi, double Phib, int gl, FlavSums& acc)
+{
+    if (xi == 0.0) {
+        // E does not depend on cos(theta): the cos nodes collapse into Cs
+#pragma unroll 1
+        for (int j = gl; j < pn; j += LPP)
+            node_accumulate<kChecked>(lds, pp[j], Cs[j], M2, muT, invT, K2mu, Phi, Phib, acc);
+    } else {
+#pragma unroll 1
+        for (int l = 0; l < tn; l++) {
+            double xc = xi * cc[l];
+            const double* Cl = Cm + l * pn;
+#pragma unroll 1
+            for (int j = gl; j < pn; j += LPP) {
+                double q = fma(xc, pp[j], pp[j]);
+                node_accumulate<kChecked>(lds, q, Cl[j], M2, muT, invT, K2mu, Phi, Phib, acc);
+            }
+        }
+    }
+}
+
 template <int LPP>
 __device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, double M, double T, double mu,
                                              double xi, double Phi, double Phib, int gl, FlavSums& acc)
@@ -236,26 +261,19 @@ __device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, 
     const double* Cm = pp + pn + tn;
     const double* Cs = Cm + pn * tn;
     const double invT = 1.0 / T;
-    const double K2mu = exp(-2.0 * mu * invT);
+    const double muT = mu * invT;
+    const double K2mu = exp(-2.0 * muT);
     const double M2 = M * M;
     flav_zero(acc);
-    if (xi == 0.0) {
-        // E does not depend on cos(theta): the cos nodes collapse into Cs
-#pragma unroll 1
-        for (int j = gl; j < pn; j += LPP) node_accumulate(lds, pp[j], Cs[j], M2, mu, invT, K2mu, Phi, Phib, acc);
-    } else {
-#pragma unroll 1
-        for (int l = 0; l < tn; l++) {
-            double xc = xi * cc[l];
-            const double* Cl = Cm + l * pn;
-#pragma unroll 1
-            for (int j = gl; j < pn; j += LPP) {
-                double q = fma(xc, pp[j], pp[j]);
-                node_accumulate(lds, q, Cl[j], M2, mu, invT, K2mu, Phi, Phib, acc);
-            }
-        }
-    }
+    // one loop per wave: a state that needs the checked loop (a negative
+    // Polyakov loop during an iteration, |mu|/T > 100) takes its whole wave
+    // with it, instead of the wave running both loops one after the other
+    if (__any(!nodes_all_fast(Phi, Phib, muT)))
+        node_loop<LPP, true>(lds, pp, cc, Cm, Cs, pn, tn, M2, muT, invT, K2mu, xi, Phi, Phib, gl, acc);
+    else
+        node_loop<LPP, false>(lds, pp, cc, Cm, Cs, pn, tn, M2, muT, invT, K2mu, xi, Phi, Phib, gl, acc);
     reduce_flav<LPP>(acc);
+    flav_finish(acc, invT);
 }
 
 // Node pass at state x: u, s, and d unless M_d == M_u (then d aliases u).

@@ -106,8 +106,10 @@ def make_queries(channel, k0, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, A1, A2, num
     return q
 
 
-def polarization_batch(q, device=0):
-    """Pi for a structured array of queries (QUERY_DTYPE) -> (re, im) numpy."""
+def polarization_batch(q, device=0, one_per_wave=False):
+    """Pi for a structured array of queries (QUERY_DTYPE) -> (re, im) numpy.
+    one_per_wave: one query per wavefront (FLOP counting with hardware
+    counters; 64x slower)."""
     ctx = _ctx(device)
     q = np.ascontiguousarray(q, dtype=QUERY_DTYPE)
     bad = ~np.isin(q["channel"], (CHANNEL_P, CHANNEL_S))
@@ -115,7 +117,8 @@ def polarization_batch(q, device=0):
         raise ValueError(f"Unsupported channel: {q['channel'][bad][0]}. Use :P or :S.")
     n = q.size
     re, im = np.zeros(n), np.zeros(n)
-    rc = ctx._L.pnjl_polarization_batch(ctx.h, n, q.ctypes.data, re.ctypes.data, im.ctypes.data, 0, None)
+    flags = _lib.FLAG_ONE_PER_WAVE if one_per_wave else 0
+    rc = ctx._L.pnjl_polarization_batch(ctx.h, n, q.ctypes.data, re.ctypes.data, im.ctypes.data, flags, None)
     ctx._check(rc, "pnjl_polarization_batch")
     return re, im
 

@@ -34,12 +34,15 @@ def assert_parity(r, o, max_chaotic=0.03, rtol=1e-10, selection=False):
     selection=True (policies that keep the minimum-Omega root of several
     runs): two runs that stop within ftol of the same root tie in Omega to
     rounding, and either may be kept; such a point (Omega equal to 1e-12,
-    x to 1e-8, a different winning run) counts as the same root."""
+    x to 1e-8, a different winning run) counts as the same root, and its
+    thermodynamics (first derivatives of P, steep in x at low T) are held to
+    1e-6 instead of 1e-9."""
     err = np.max(np.abs(r["x"] - o["x"]) / np.maximum(1.0, np.abs(o["x"])), axis=1)
     eo = np.abs(r["omega"] - o["omega"]) / np.abs(o["omega"])
     bad = (r["converged"] != o["converged"]) | ((o["converged"] == 1) & ((err > rtol) | (eo > rtol)))
+    tie = np.zeros_like(bad)
     if selection:
-        tie = (eo < 1e-12) & (err < 1e-8) & (r["iterations"] != o["iterations"])
+        tie = (eo < 1e-12) & (err < 1e-8) & (r["iterations"] != o["iterations"]) & bad
         bad &= ~tie
     chaotic = (np.maximum(r["iterations"], o["iterations"]) > 25) | (r["flags"] != o["flags"])
     worst = np.argsort(-np.where(np.isnan(err), np.inf, err))[:5]
@@ -48,9 +51,13 @@ def assert_parity(r, o, max_chaotic=0.03, rtol=1e-10, selection=False):
     assert not np.any(bad & ~chaotic), f"non-chaotic mismatches: {info}"
     assert chaotic[bad].size <= max_chaotic * r["x"].shape[0], f"too many mismatches: {info}"
     ok = (o["converged"] == 1) & ~bad
+    tol = np.where(tie, 1e-6, 1e-9)
     for k in ("pressure", "entropy", "energy", "rho_norm"):
-        assert np.all(np.abs(r[k][ok] - o[k][ok]) <= 1e-9 * np.maximum(1, np.abs(o[k][ok]))), k
-    assert np.all(np.abs(r["n_q"][ok] - o["n_q"][ok]) <= 1e-9)
+        d = np.abs(r[k] - o[k]) / np.maximum(1, np.abs(o[k]))
+        w = np.flatnonzero(ok & (d > tol))
+        assert w.size == 0, (k, [(int(i), float(d[i]), float(err[i]), bool(tie[i])) for i in w[:5]])
+    dn = np.max(np.abs(r["n_q"] - o["n_q"]), axis=1)
+    assert np.all(dn[ok] <= tol[ok])
     return ok
 
 
@@ -72,6 +79,29 @@ def test_residual_and_jacobian_match_oracle(gpu_ctx, oracle):
             assert np.max(np.abs(F[i] + g)) <= 1e-12 * scale, i
             hs = max(1.0, np.abs(H).max())
             assert np.max(np.abs(J[i] + H)) <= 1e-12 * hs, i
+
+
+def test_residual_at_floor_and_extreme_exponents(gpu_ctx, oracle):
+    """The integrand's edge branches: negative Polyakov loops that push a
+    log argument under the reference's 1e-16 floor, exponents (mu - E)/T up to
+    150 (T = 3 MeV, mu = 450 MeV) beyond the unscaled fast path, mu < 0
+    (exp(-2 mu/T) > 1) and xi != 0 -- value, gradient and Hessian as the
+    oracle's shifted-log restatement.  A negative loop can leave a log
+    argument D just above the floor, where the ratios t^k/D and their squares
+    in the Hessian amplify rounding (max |H| reaches ~1e4 here): the Hessian
+    is held to 1e-9 of its largest entry, a wrong branch would miss by O(1)."""
+    import itertools
+    rows = list(itertools.product([3, 5, 20, 150], [-300, 0, 450], [-1.2, -0.4, 0.3], [0.05, 0.9], [0.0, 0.3]))
+    T = np.array([r[0] for r in rows]) / HB
+    mu = np.array([r[1] for r in rows]) / HB
+    xi = np.array([r[4] for r in rows], dtype=float)
+    x = np.array([[-0.5, -0.5, -1.9, r[2], r[3]] for r in rows])
+    F, J, om = gpu_ctx.residual_batch(x, T, mu, xi, 64, 4)
+    for i in range(len(rows)):
+        o, g, H = oracle.omega_derivs(x[i], mu[i], T[i], xi[i], 64, 4)
+        assert abs(om[i] - o) <= 1e-13 * max(1.0, abs(o)), rows[i]
+        assert np.max(np.abs(F[i] + g)) <= 1e-11 * max(1.0, np.abs(g).max()), rows[i]
+        assert np.max(np.abs(J[i] + H)) <= 1e-9 * max(1.0, np.abs(H).max()), rows[i]
 
 
 @pytest.mark.parametrize("policy,name", [(0, "default"), (3, "multi"), (1, "hadron"), (2, "quark")])
