@@ -178,11 +178,37 @@ __host__ __device__ constexpr int sidx(int i, int j)
 // ---------------------------------------------------------------------------
 // node pass
 // ---------------------------------------------------------------------------
+// v from the lane selected by the DPP pattern CTRL (within its row of 16)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+enum DppCtrl {
+    DPP_QUAD_XOR1 = 0xB1,       // quad_perm [1, 0, 3, 2]
+    DPP_QUAD_XOR2 = 0x4E,       // quad_perm [2, 3, 0, 1]
+    DPP_ROW_MIRROR = 0x140,     // lane i <- 15 - i within the row
+    DPP_ROW_HALF_MIRROR = 0x141 // lane i <- 7 - i within each half row
+};
+
+// Sum over the LPP lanes of a group (aligned, LPP a power of two), returned
+// in every lane of the group.  Steps across rows use lane shuffles; within a
+// row, DPP: mirror (16), half mirror (8), then the quad butterfly.  Every
+// step adds a value and its partner's, so all lanes of the group end with
+// the same pairwise sums in the same tree, i.e. bit-identical totals.
 template <int LPP>
 __device__ __forceinline__ double group_sum(double v)
 {
-#pragma unroll
-    for (int off = LPP / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, LPP);
+    if (LPP >= 64) v += __shfl_xor(v, 32, 64);
+    if (LPP >= 32) v += __shfl_xor(v, 16, 64);
+    if (LPP >= 16) v += dpp_f64<DPP_ROW_MIRROR>(v);
+    if (LPP >= 8) v += dpp_f64<DPP_ROW_HALF_MIRROR>(v);
+    if (LPP >= 2) v += dpp_f64<DPP_QUAD_XOR1>(v);
+    if (LPP >= 4) v += dpp_f64<DPP_QUAD_XOR2>(v);
     return v;
 }
 
