@@ -343,6 +343,12 @@ def main():
             seed_policy = "MULTIROOT3 (learned seed + HADRON + QUARK, min Omega)"
         else:
             seed_policy = "SurrogateSeed(judge=reference)"
+        # implicit thermodynamic derivatives at every root of the step (one
+        # pass per point after the solve; kernel time, outside the timed step)
+        okr = Q["converged"] == 1
+        ctx.thermo_derivatives_batch(Q["x"][okr], T[okr], mu[okr], xi[okr], w["p_num"], w["t_num"])
+        extra.update(thermo_derivatives_kernel_ms=ctx.last_kernel_ms(),
+                     thermo_derivatives_points_per_s=float(okr.sum()) / (ctx.last_kernel_ms() * 1e-3))
     else:
         ms_step, scan_ms, flops, solved = ms_plain, scan_plain, flops_plain, solved_plain
         Q = P

@@ -194,6 +194,27 @@ int pnjl_solve_fixedmu_batch(pnjl_ctx* ctx, int64_t n,
                              const pnjl_solve_out* out,
                              int flags, void* stream);
 
+/* Implicit thermodynamic derivatives at given roots x[n*5] of the FixedMu gap
+ * equations (solve first; xi and the node counts as in the solve): out[n*36],
+ * per point, in this order --
+ *   0 pressure, 1 energy, 2 rho (sum_i rho_i / 3), 3 rho_norm, 4 entropy,
+ *   5 dP_dT, 6 dP_dmu, 7 dEpsilon_dT, 8 dEpsilon_dmu, 9 dn_dT, 10 dn_dmu,
+ *   11 dP_depsilon_n, 12 dP_dn_epsilon, 13-15 masses (u, d, s; bare masses
+ *   5.5 and 140.0 MeV as the reference's derivative module writes them),
+ *   16-18 dM_dT, 19-21 dM_dmu (quark mu), 22 v_n_sq, 23 dmuB_dT_sigma,
+ *   24-28 dx_dT, 29-33 dx_dmu, 34 ds_dT, 35 ds_dmu.
+ * dx/dtheta = -(d2 Omega/dx2)^-1 d2 Omega/dx dtheta (implicit function
+ * theorem); all-NaN row where that Hessian is singular.  Host pointers unless
+ * PNJL_FLAG_DEVICE_PTRS.
+ * replaces: mass_derivatives (order 1), thermo_derivatives, dP_dT / dP_dmu
+ *           (order 1, 2) and bulk_viscosity_coefficients
+ *           src/pnjl/derivatives/ThermoDerivatives.jl:132-466              */
+#define PNJL_THERMO_DERIV_FIELDS 36
+int pnjl_thermo_derivatives_batch(pnjl_ctx* ctx, int64_t n,
+                                  const double* T, const double* mu, const double* xi,
+                                  const double* x, int p_num, int t_num,
+                                  double* out, int flags, void* stream);
+
 /* Residual F = grad_x P (5), Jacobian J = dF/dx (5x5, row-major) and Omega at
  * given states -- the building block of the Newton loop.
  * replaces: gap_conditions  src/pnjl/solver/Conditions.jl:72-81 and the

@@ -25,6 +25,22 @@ SEED_CONTINUITY, SEED_PHASE_AWARE, SEED_SURROGATE, SEED_MULTIROOT3 = 5, 6, 7, 8
 PT_TR_USED, PT_MULTISEED, PT_EXCEPTION, PT_NO_ROOT = 0x01, 0x02, 0x04, 0x08
 PT_SURR_FALLBACK, PT_SURR_OMEGA, PT_FIRST_ORDER, PT_CHAIN_RESET = 0x10, 0x20, 0x40, 0x80
 PT_SURR_MASKED = 0x100
+
+# pnjl_thermo_derivatives_batch: per-point output columns (name, width)
+THERMO_DERIV_FIELDS = 36
+THERMO_DERIV_LAYOUT = [("pressure", 1), ("energy", 1), ("rho", 1), ("rho_norm", 1), ("entropy", 1), ("dP_dT", 1),
+                       ("dP_dmu", 1), ("dEpsilon_dT", 1), ("dEpsilon_dmu", 1), ("dn_dT", 1), ("dn_dmu", 1),
+                       ("dP_depsilon_n", 1), ("dP_dn_epsilon", 1), ("masses", 3), ("dM_dT", 3), ("dM_dmu", 3),
+                       ("v_n_sq", 1), ("dmuB_dT_sigma", 1), ("dx_dT", 5), ("dx_dmu", 5), ("ds_dT", 1), ("ds_dmu", 1)]
+
+
+def unpack_thermo_derivs(out):
+    """(n, 36) -> dict of named columns ((n,) or (n, w))."""
+    d, k = {}, 0
+    for name, w in THERMO_DERIV_LAYOUT:
+        d[name] = out[:, k] if w == 1 else out[:, k:k + w]
+        k += w
+    return d
 JUDGE_REFERENCE, JUDGE_OMEGA = 0, 1
 
 
@@ -116,11 +132,12 @@ def load():
                                            C.POINTER(SolveOpts), C.POINTER(SolveOut), C.c_int, vp]
     L.pnjl_gap_residual_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp,
                                           C.c_int, vp]
+    L.pnjl_thermo_derivatives_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, vp, C.c_int, vp]
     L.pnjl_autotune.argtypes = [vp, C.c_int64, _DP, _DP, _DP, _DP, C.c_int, C.c_int, C.c_int,
                                 C.POINTER(SolveOpts), C.POINTER(LaunchCfg), C.POINTER(C.c_double)]
     for name in ("pnjl_ctx_create", "pnjl_gauleg", "pnjl_nodes", "pnjl_set_phase_boundary", "pnjl_set_judge_mask",
                  "pnjl_set_launch_cfg", "pnjl_get_launch_cfg", "pnjl_solve_fixedmu_batch",
-                 "pnjl_gap_residual_batch", "pnjl_autotune"):
+                 "pnjl_gap_residual_batch", "pnjl_thermo_derivatives_batch", "pnjl_autotune"):
         getattr(L, name).restype = C.c_int
     _register_extra(L)
     _lib = L
@@ -300,6 +317,18 @@ class Context:
                                              om.ctypes.data, 0, None)
         self._check(rc, "pnjl_gap_residual_batch")
         return F, J, om
+
+    def thermo_derivatives_batch(self, x, T, mu, xi, p_num=64, t_num=8):
+        """Implicit thermodynamic derivatives at the roots x (n, 5) -> dict of
+        the PNJL_THERMO_DERIV_FIELDS columns (THERMO_DERIV_LAYOUT)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 5)
+        n = x.shape[0]
+        T, mu, xi = (np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (T, mu, xi))
+        out = np.zeros((n, THERMO_DERIV_FIELDS))
+        rc = self._L.pnjl_thermo_derivatives_batch(self.h, n, T.ctypes.data, mu.ctypes.data, xi.ctypes.data,
+                                                   x.ctypes.data, p_num, t_num, out.ctypes.data, 0, None)
+        self._check(rc, "pnjl_thermo_derivatives_batch")
+        return unpack_thermo_derivs(out)
 
     def autotune(self, T, mu, xi, policy=SEED_DEFAULT, seed=None, p_num=64, t_num=8, opts=None, sample=None):
         T = np.ascontiguousarray(T, dtype=np.float64).ravel()

@@ -14,7 +14,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpnjl_hip.so")
 
 # one translation unit per feature; all linked into one shared library
-SOURCES = ["pnjl_lib.hip", "pnjl_mlp.hip", "pnjl_polar.hip"]
+SOURCES = ["pnjl_lib.hip", "pnjl_mlp.hip", "pnjl_polar.hip", "pnjl_deriv.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

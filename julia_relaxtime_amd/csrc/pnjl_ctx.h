@@ -60,6 +60,8 @@ inline int set_err(pnjl_ctx* c, int code, const std::string& m)
             return pnjl_host::set_err(ctx, PNJL_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// node table of (p_num, t_num) on the device (math tables | p^2 | cos^2 | C | Cs), cached per context
+int get_tab(pnjl_ctx* ctx, int pn, int tn, double** out);
 // n-point Gauss-Legendre rule on [-1,1], ascending
 void gl_rule(int n, std::vector<double>& x, std::vector<double>& w);
 // device staging buffer k of at least `bytes` (grown on demand, owned by the context)

@@ -106,6 +106,30 @@ function solve_batch(T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{Float6
 end
 
 """
+    thermo_derivatives_batch(T_fm, mu_fm, x; xi, p_num=64, t_num=8)
+
+Implicit thermodynamic derivatives at the roots `x` (5 x n, from `solve_batch`):
+a 36 x n matrix, columns as documented for `pnjl_thermo_derivatives_batch`
+(pressure, energy, rho, rho_norm, entropy, dP_dT, ..., v_n_sq, dmuB_dT_sigma,
+dx_dT, dx_dmu, ds_dT, ds_dmu).  Replaces ThermoDerivatives.jl:132-466
+(mass_derivatives order 1, thermo_derivatives, bulk_viscosity_coefficients).
+"""
+function thermo_derivatives_batch(T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{Float64},
+                                  x::Matrix{Float64}; xi::AbstractVector{Float64}=zeros(length(T_fm)),
+                                  p_num::Integer=64, t_num::Integer=8, ctx::Context=default_context())
+    n = length(T_fm)
+    out = zeros(36, n)
+    GC.@preserve T_fm mu_fm xi x out begin
+        rc = ccall((:pnjl_thermo_derivatives_batch, LIB), Cint,
+                   (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Cint,
+                    Ptr{Float64}, Cint, Ptr{Cvoid}),
+                   ctx.h, n, T_fm, mu_fm, xi, x, p_num, t_num, out, 0, C_NULL)
+        check(ctx, rc, "pnjl_thermo_derivatives_batch")
+    end
+    return out
+end
+
+"""
     polarization_aniso(channel, k0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi, A1, A2, num_s)
 
 Same signature and meaning as the reference (channel :P or :S); computed on the GPU.

@@ -93,6 +93,8 @@ def lib():
                                            C.POINTER(C.c_int)]
         L.oracle_newton_capped.restype = C.c_int
         L.oracle_thermo.argtypes = [_DP, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _DP]
+        L.oracle_thermo_derivs.argtypes = [_DP, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _DP]
+        L.oracle_thermo_derivs.restype = C.c_int
         L.oracle_set_boundary.argtypes = [C.c_int, _DP, _DP, _DP, C.c_int, _DP, _DP, _DP]
         L.oracle_interp_mu_c.argtypes = [C.c_double, C.c_double]
         L.oracle_interp_mu_c.restype = C.c_double
@@ -188,6 +190,31 @@ def thermo(x, T, mu, xi=0.0, p_num=64, t_num=8):
     lib().oracle_thermo(_p(x), T, mu, xi, p_num, t_num, _p(out))
     return dict(omega=out[0], pressure=out[1], rho_norm=out[2], entropy=out[3], energy=out[4],
                 n_q=out[5:8].copy(), n_qbar=out[8:11].copy(), phys=bool(out[11]))
+
+
+THERMO_DERIV_FIELDS = ["pressure", "energy", "rho", "rho_norm", "entropy", "dP_dT", "dP_dmu", "dEpsilon_dT",
+                       "dEpsilon_dmu", "dn_dT", "dn_dmu", "dP_depsilon_n", "dP_dn_epsilon", "masses", "dM_dT",
+                       "dM_dmu", "v_n_sq", "dmuB_dT_sigma", "dx_dT", "dx_dmu", "ds_dT", "ds_dmu"]
+_VEC = {"masses": 3, "dM_dT": 3, "dM_dmu": 3, "dx_dT": 5, "dx_dmu": 5}
+
+
+def unpack_thermo_derivs(out):
+    """36 doubles (one point) or (n, 36) -> dict of the named fields."""
+    out = np.asarray(out)
+    d, k = {}, 0
+    for f in THERMO_DERIV_FIELDS:
+        w = _VEC.get(f, 1)
+        d[f] = out[..., k] if w == 1 else out[..., k:k + w]
+        k += w
+    return d
+
+
+def thermo_derivs(x, T, mu, xi=0.0, p_num=64, t_num=8):
+    """Implicit thermodynamic derivatives at the root x (oracle_thermo_derivs)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(36)
+    lib().oracle_thermo_derivs(_p(x), T, mu, xi, p_num, t_num, _p(out))
+    return unpack_thermo_derivs(out)
 
 
 def alloc_out(n):

@@ -1602,3 +1602,192 @@ void oracle_thermo(const double* x, double T, double mu, double xi, int p_num, i
     }
     out12[11] = c.phys;
 }
+
+/* ------------------------------------------------------------------------- */
+/* implicit thermodynamic derivatives                                        */
+/* ThermoDerivatives.jl:132-245 (mass_derivatives order 1, thermo_derivatives)*/
+/* and :342-466 (bulk_viscosity_coefficients): at a root x of F(x; T, mu) =  */
+/* grad_x P = 0, dx/dtheta = -(dF/dx)^-1 dF/dtheta (the implicit function    */
+/* theorem that ImplicitDifferentiation.jl applies), then chain rules.  The  */
+/* reference obtains the partial derivatives by forward-mode AD; here they   */
+/* are closed forms of the same node terms.                                  */
+/* ------------------------------------------------------------------------- */
+/* Per-flavour derivatives of S = sum C Q(E; mu, T, Phi, Phib), Q = lnF+(a) +
+ * lnF-(b), a = (mu - E)/T, b = -(E + mu)/T, E = E(M): the mixed ones with
+ * theta = (mu, T) for one common mu. */
+typedef struct {
+    double Smu, ST, Smumu, SmuT, STT, SMmu, SMT, SPmu, SPbmu, SPT, SPbT;
+} flav_theta_t;
+
+static void flavour_theta_sums(const nodes_t* nd, double M, double mu, double T, double Phi, double Phib,
+                               double xi, flav_theta_t* o)
+{
+    memset(o, 0, sizeof(*o));
+    double invT = 1.0 / T, invT2 = invT * invT;
+    for (int idx = 0; idx < nd->n; idx++) {
+        double p = nd->p[idx], t = nd->c[idx], C = nd->C[idx];
+        double pt = p * t;
+        double E = sqrt(p * p + M * M + xi * (pt * pt));
+        double a = -(E - mu) * invT;
+        double b = -(E + mu) * invT;
+        lnF_t qp, qm;
+        lnF_eval(a, Phi, Phib, &qp);
+        lnF_eval(b, Phib, Phi, &qm);
+        double EM = M / E;
+        /* da/dmu = 1/T, db/dmu = -1/T, da/dT = -a/T, db/dT = -b/T, da/dE = db/dE = -1/T */
+        o->Smu += C * (qp.a - qm.a) * invT;
+        o->ST += C * (-(qp.a * a + qm.a * b) * invT);
+        o->Smumu += C * (qp.aa + qm.aa) * invT2;
+        o->SmuT += C * (-(qp.a - qm.a) - (qp.aa * a - qm.aa * b)) * invT2;
+        o->STT += C * (2.0 * (qp.a * a + qm.a * b) + qp.aa * a * a + qm.aa * b * b) * invT2;
+        o->SMmu += C * (-(qp.aa - qm.aa) * invT2) * EM;
+        o->SMT += C * ((qp.a + qm.a + qp.aa * a + qm.aa * b) * invT2) * EM;
+        o->SPmu += C * (qp.ac1 - qm.ac2) * invT;
+        o->SPbmu += C * (qp.ac2 - qm.ac1) * invT;
+        o->SPT += C * (-(qp.ac1 * a + qm.ac2 * b) * invT);
+        o->SPbT += C * (-(qp.ac2 * a + qm.ac1 * b) * invT);
+    }
+}
+
+/* d/dT and d^2/dT^2 of U at fixed (Phi, Phib), and d^2 U / dPhi dT, dPhib dT
+ * (Thermodynamics.jl:124-131: U = T^4 [-a(T)/2 Phi Phib + b(T) log(...)],
+ * a = a0 + a1 T0/T + a2 (T0/T)^2, b = b3 (T0/T)^3, log argument floored). */
+static void polyakov_U_T(double T, double Phi, double Phib, double* UT, double* UTT, double* UPT, double* UPbT)
+{
+    double val = 1.0 - 6.0 * Phib * Phi + 4.0 * (Phib * Phib * Phib + Phi * Phi * Phi) - 3.0 * (Phib * Phi) * (Phib * Phi);
+    int floored = (val <= 0.0) || (val < POLYAKOV_EPS);
+    double lg = floored ? log(POLYAKOV_EPS) : log(val);
+    double lP = 0.0, lPb = 0.0;
+    if (!floored) {
+        lP = (-6.0 * Phib + 12.0 * Phi * Phi - 6.0 * Phi * Phib * Phib) / val;
+        lPb = (-6.0 * Phi + 12.0 * Phib * Phib - 6.0 * Phi * Phi * Phib) / val;
+    }
+    /* T^4 a(T) = a0 T^4 + a1 T0 T^3 + a2 T0^2 T^2;  T^4 b(T) = b3 T0^3 T */
+    double T0 = P.T0;
+    double A1 = 4.0 * P.a0 * T * T * T + 3.0 * P.a1 * T0 * T * T + 2.0 * P.a2 * T0 * T0 * T;
+    double A2 = 12.0 * P.a0 * T * T + 6.0 * P.a1 * T0 * T + 2.0 * P.a2 * T0 * T0;
+    double B1 = P.b3 * T0 * T0 * T0;
+    *UT = -0.5 * Phi * Phib * A1 + B1 * lg;
+    *UTT = -0.5 * Phi * Phib * A2;
+    *UPT = -0.5 * Phib * A1 + B1 * lP;
+    *UPbT = -0.5 * Phi * A1 + B1 * lPb;
+}
+
+/* The reference's mass function of ThermoDerivatives.jl:111-121 (bare masses
+ * 5.5 and 140.0 MeV written out there). */
+static void deriv_masses(const double* x, double M[3])
+{
+    double mu0 = 0.0055 / 0.197327, ms0 = 0.140 / 0.197327;
+    M[0] = mu0 - 4.0 * P.G * x[0] + 2.0 * P.K * x[1] * x[2];
+    M[1] = mu0 - 4.0 * P.G * x[1] + 2.0 * P.K * x[0] * x[2];
+    M[2] = ms0 - 4.0 * P.G * x[2] + 2.0 * P.K * x[0] * x[1];
+}
+
+/* out[36]: pressure, energy, rho (= sum rho_i / 3), rho_norm, entropy,
+ * dP_dT, dP_dmu, dEpsilon_dT, dEpsilon_dmu, dn_dT, dn_dmu, dP_depsilon_n,
+ * dP_dn_epsilon, masses[3], dM_dT[3], dM_dmu[3], v_n_sq, dmuB_dT_sigma,
+ * dx_dT[5], dx_dmu[5], ds_dT, ds_dmu.  Returns 1 if dF/dx is singular. */
+int oracle_thermo_derivs(const double* x, double T, double mu, double xi, int p_num, int t_num, double* out)
+{
+    ensure_params();
+    const nodes_t* nd = cache_nodes(p_num, t_num);
+    double muv[3] = {mu, mu, mu};
+    eval_t e;
+    eval_point(nd, x, muv, T, xi, &e);
+    double M[3], dM[3][3], d2M[3][3][3];
+    mass_vec(x, M, dM, d2M);
+    flav_theta_t th[3];
+    for (int i = 0; i < 3; i++) flavour_theta_sums(nd, M[i], mu, T, x[3], x[4], xi, &th[i]);
+    double UT, UTT, UPT, UPbT;
+    polyakov_U_T(T, x[3], x[4], &UT, &UTT, &UPT, &UPbT);
+
+    /* derivatives of Omega = chi + U + Evac - 2 T sum_i S_i in (mu, T) */
+    double Om_mu = 0, Om_T = UT, Om_mumu = 0, Om_muT = 0, Om_TT = UTT;
+    double Om_xmu[5] = {0}, Om_xT[5] = {0};
+    for (int i = 0; i < 3; i++) {
+        const flav_t* f = &e.fl[i];
+        const flav_theta_t* g = &th[i];
+        Om_mu += -2.0 * T * g->Smu;
+        Om_T += -2.0 * f->S - 2.0 * T * g->ST;
+        Om_mumu += -2.0 * T * g->Smumu;
+        Om_muT += -2.0 * g->Smu - 2.0 * T * g->SmuT;
+        Om_TT += -4.0 * g->ST - 2.0 * T * g->STT;
+        double LMmu = -2.0 * T * g->SMmu;
+        double LMT = -2.0 * f->SM - 2.0 * T * g->SMT;
+        for (int a = 0; a < 3; a++) {
+            Om_xmu[a] += dM[i][a] * LMmu;
+            Om_xT[a] += dM[i][a] * LMT;
+        }
+        Om_xmu[3] += -2.0 * T * g->SPmu;
+        Om_xmu[4] += -2.0 * T * g->SPbmu;
+        Om_xT[3] += -2.0 * f->SP - 2.0 * T * g->SPT;
+        Om_xT[4] += -2.0 * f->SPb - 2.0 * T * g->SPbT;
+    }
+    Om_xT[3] += UPT;
+    Om_xT[4] += UPbT;
+
+    /* dx/dtheta = -(d2 Omega/dx2)^-1 d2 Omega/dx dtheta */
+    double H[25], rhs[5], dxdT[5], dxdmu[5];
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) H[i * 5 + j] = e.hess[i][j];
+    for (int i = 0; i < 5; i++) rhs[i] = -Om_xT[i];
+    int sing = lu_solve(5, H, rhs, dxdT);
+    for (int i = 0; i < 5; i++) rhs[i] = -Om_xmu[i];
+    sing |= lu_solve(5, H, rhs, dxdmu);
+    if (sing) {
+        for (int k = 0; k < 36; k++) out[k] = NAN;
+        return 1;
+    }
+    double Pr = -e.omega;
+    double s = -Om_T;              /* dP/dT at fixed x          */
+    double ntot = -Om_mu;          /* dP/dmu = sum_i rho_i      */
+    double ds_dT = -Om_TT, ds_dmu = -Om_muT, dnt_dT = -Om_muT, dnt_dmu = -Om_mumu;
+    for (int a = 0; a < 5; a++) {
+        ds_dT += -Om_xT[a] * dxdT[a];
+        ds_dmu += -Om_xT[a] * dxdmu[a];
+        dnt_dT += -Om_xmu[a] * dxdT[a];
+        dnt_dmu += -Om_xmu[a] * dxdmu[a];
+    }
+    double n = ntot / 3.0, dn_dT = dnt_dT / 3.0, dn_dmu = dnt_dmu / 3.0;
+    double eps = -Pr + T * s + mu * ntot;
+    double E_T = T * ds_dT + mu * dnt_dT, E_mu = T * ds_dmu + mu * dnt_dmu;
+    double P_T = s, P_mu = ntot;
+    double den_e = E_T * dn_dmu - E_mu * dn_dT;
+    double den_n = dn_dT * E_mu - dn_dmu * E_T;
+    out[0] = Pr;
+    out[1] = eps;
+    out[2] = n;
+    out[3] = ntot / (3.0 * P.rho0);
+    out[4] = s;
+    out[5] = P_T;
+    out[6] = P_mu;
+    out[7] = E_T;
+    out[8] = E_mu;
+    out[9] = dn_dT;
+    out[10] = dn_dmu;
+    out[11] = den_e == 0.0 ? NAN : (P_T * dn_dmu - P_mu * dn_dT) / den_e;
+    out[12] = den_n == 0.0 ? NAN : (P_T * E_mu - P_mu * E_T) / den_n;
+    double Mr[3];
+    deriv_masses(x, Mr);
+    for (int i = 0; i < 3; i++) {
+        double gT = 0.0, gmu = 0.0;
+        for (int a = 0; a < 3; a++) {
+            gT += dM[i][a] * dxdT[a];
+            gmu += dM[i][a] * dxdmu[a];
+        }
+        out[13 + i] = Mr[i];
+        out[16 + i] = gT;
+        out[19 + i] = gmu;
+    }
+    /* bulk_viscosity_coefficients: mu_B = 3 mu_q, n_B = n */
+    double ds_dmuB = ds_dmu / 3.0, dn_dmuB = dn_dmu / 3.0;
+    out[22] = (s * dn_dmuB - n * dn_dT) / (T * (ds_dT * dn_dmuB - ds_dmuB * dn_dT));
+    out[23] = -(n * ds_dT - s * dn_dT) / (n * ds_dmuB - s * dn_dmuB);
+    for (int a = 0; a < 5; a++) {
+        out[24 + a] = dxdT[a];
+        out[29 + a] = dxdmu[a];
+    }
+    out[34] = ds_dT;
+    out[35] = ds_dmu;
+    return 0;
+}

@@ -14,6 +14,8 @@ Sources (paths relative to the reference root):
       roots at mu = mu_avg on all three branches of the first-order S-curve
   data/reference/pnjl/spinodals.csv, boundary.csv      (calculate_phase_structure.jl: S-curve
       spinodals and the Maxwell-construction transition line, xi = 0, 0.2, 0.4)
+  data/outputs/results/pnjl/bulk_viscosity_xi0.0.csv   (calculate_derivatives.jl, ThermoDerivatives
+      bulk_viscosity_coefficients, p24/t8, 7 digits)
 """
 import csv
 import os
@@ -66,6 +68,8 @@ def main(ref):
           rows_of(os.path.join(ref, "data/reference/pnjl/spinodals.csv")))
     write("phase_boundary.csv", ["xi", "T_MeV", "mu_transition_MeV", "rho_hadron", "rho_quark"],
           rows_of(os.path.join(ref, "data/reference/pnjl/boundary.csv")))
+    bulk = os.path.join(ref, "data/outputs/results/pnjl/bulk_viscosity_xi0.0.csv")
+    write("bulk_viscosity_xi0.0.csv", list(rows_of(bulk)[0].keys()), rows_of(bulk))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,84 @@
+"""Pins the oracle's implicit thermodynamic derivatives (ThermoDerivatives.jl:
+mass_derivatives order 1, thermo_derivatives, bulk_viscosity_coefficients)
+before it serves as the checker of the GPU derivative pass:
+  * finite differences of re-solved roots and of the (pinned) thermodynamics,
+  * exact identities (Maxwell relation d s/d mu = d n/d T, dP/dT = s, dP/dmu = n),
+  * the reference's own output bulk_viscosity_xi0.0.csv (p24/t8, 7 digits)."""
+import numpy as np
+import pytest
+
+from conftest import golden_rows
+
+HB = 197.327
+
+STATES = [(150, 0, 0.0), (150, 50, 0.0), (160, 50, 0.0), (120, 250, 0.3), (60, 360, 0.0), (200, 300, -0.2),
+          (90, 150, 0.4)]
+
+
+def _solve(oracle, T, mu, xi, seed=None, pn=24, tn=8):
+    T, mu = np.atleast_1d(float(T)), np.atleast_1d(float(mu))
+    opts = oracle.default_opts(ftol=1e-13)
+    if seed is None:
+        return oracle.solve_batch(T, mu, xi, policy=oracle.SEED_DEFAULT, p_num=pn, t_num=tn, opts=opts)
+    return oracle.solve_batch(T, mu, xi, policy=oracle.SEED_EXPLICIT, seed=np.atleast_2d(seed), p_num=pn,
+                              t_num=tn, opts=opts)
+
+
+@pytest.mark.parametrize("state", STATES)
+def test_derivatives_match_finite_differences(oracle, state):
+    Tm, mm, xi = state
+    T, mu = Tm / HB, mm / HB
+    r = _solve(oracle, T, mu, xi)
+    assert r["converged"][0]
+    x = r["x"][0]
+    d = oracle.thermo_derivs(x, T, mu, xi, 24, 8)
+    h = 1e-4
+    s = {k: _solve(oracle, T + dT, mu + dm, xi, x) for k, (dT, dm) in
+         {"T+": (h, 0), "T-": (-h, 0), "m+": (0, h), "m-": (0, -h)}.items()}
+    fd = lambda key, a, b: (s[a][key][0] - s[b][key][0]) / (2 * h)
+    close = lambda a, b, tol: np.all(np.abs(np.asarray(a) - b) <= tol * np.maximum(1e-3, np.max(np.abs(b))))
+    assert close(d["dx_dT"], fd("x", "T+", "T-"), 2e-5)
+    assert close(d["dx_dmu"], fd("x", "m+", "m-"), 2e-5)
+    assert close(d["dM_dT"], fd("masses", "T+", "T-"), 2e-5)
+    assert close(d["dM_dmu"], fd("masses", "m+", "m-"), 2e-5)
+    assert close(d["entropy"], fd("pressure", "T+", "T-"), 1e-6)
+    n_of = lambda k: s[k]["rho_norm"][0] * 0.16         # n = sum rho_i / 3 = rho_norm rho0
+    assert close(d["rho"] * 3, fd("pressure", "m+", "m-"), 1e-6)
+    assert close(d["ds_dT"], fd("entropy", "T+", "T-"), 2e-5)
+    assert close(d["ds_dmu"], fd("entropy", "m+", "m-"), 2e-5)
+    assert close(d["dn_dmu"], (n_of("m+") - n_of("m-")) / (2 * h), 2e-5)
+    assert close(d["dn_dT"], (n_of("T+") - n_of("T-")) / (2 * h), 2e-5)
+    assert close(d["dEpsilon_dT"], fd("energy", "T+", "T-"), 2e-5)
+    assert close(d["dEpsilon_dmu"], fd("energy", "m+", "m-"), 2e-5)
+    # exact identities of the closed forms
+    assert abs(d["ds_dmu"] - 3 * d["dn_dT"]) <= 1e-11 * max(1.0, abs(d["ds_dmu"]))     # Maxwell relation
+    assert d["dP_dT"] == d["entropy"] and d["dP_dmu"] == 3 * d["rho"]
+    assert abs(d["pressure"] - r["pressure"][0]) <= 1e-14 * abs(d["pressure"])
+    assert abs(d["rho_norm"] - r["rho_norm"][0]) <= 1e-12 * max(1.0, abs(d["rho_norm"]))
+
+
+def test_bulk_viscosity_coefficients_match_reference_output(oracle):
+    """bulk_viscosity_xi0.0.csv rows whose root DefaultSeed reproduces (T = 150
+    MeV; the T = 160 MeV rows sit on a negative-mass root that the reference's
+    run reached and a DefaultSeed solve does not).  The stored masses use the
+    bare masses 5.5 and 140.0 MeV of ThermoDerivatives.jl:111-121."""
+    rows = [r for r in golden_rows("bulk_viscosity_xi0.0.csv") if float(r["T_MeV"]) == 150.0]
+    assert len(rows) == 2
+    for row in rows:
+        T, mu = float(row["T_MeV"]) / HB, float(row["mu_MeV"]) / HB
+        r = _solve(oracle, T, mu, 0.0)
+        d = oracle.thermo_derivs(r["x"][0], T, mu, 0.0, 24, 8)
+        ref = lambda k: float(row[k])
+        rel = lambda a, b: abs(a - b) <= 2e-6 * abs(b)
+        assert rel(d["v_n_sq"], ref("v_n_sq"))
+        assert rel(d["entropy"], ref("entropy_fm3"))
+        for i, f in enumerate("uds"):
+            assert abs(d["masses"][i] * HB - ref(f"M_{f}_MeV")) < 2e-6
+            assert rel(d["dM_dT"][i], ref(f"dM_{f}_dT"))
+        if float(row["mu_MeV"]) > 0:                      # mu = 0: n_B and d/dmu_B vanish (stored noise)
+            assert rel(d["dmuB_dT_sigma"], ref("dmuB_dT_sigma"))
+            assert rel(d["rho"], ref("n_B_fm3"))
+            for i, f in enumerate("uds"):
+                assert rel(d["dM_dmu"][i] / 3, ref(f"dM_{f}_dmuB"))
+        else:
+            assert abs(d["rho"]) < 1e-12 and np.all(np.abs(d["dM_dmu"]) < 1e-12)
