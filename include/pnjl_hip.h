@@ -388,6 +388,104 @@ int pnjl_polarization_grid(pnjl_ctx* ctx, int npts, const pnjl_pol_point* pts, i
                            int nk, const double* k, int np, int nc, double* re, double* im, int flags,
                            void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* consumers of Pi: effective couplings, meson propagators, D_S / D_P        */
+/* ------------------------------------------------------------------------ */
+/* The 12 effective couplings, in the reference's NamedTuple order
+ * (EffectiveCouplings.jl:272-277):
+ *   K0+, K0-, K123+, K123-, K4567+, K4567-, K8+, K8-, K08+, K08-, detK+, detK-   */
+#define PNJL_KCOEF_FIELDS 12
+
+/* G_f = calculate_G_from_A(A_f, m_f; Nc) for f = u, s and the couplings
+ * calculate_effective_couplings(G, K, G_u, G_s), for n rows of
+ * in[n*4] = (A_u, A_s, m_u, m_s), or with from_gf = 1 of in[n*2] =
+ * (G_u, G_s) directly.  G_f (n*2, may be NULL) and kc (n*12).  Host
+ * pointers.  G and K are arguments (the model's G_fm2 and K_fm5 are the
+ * context's params.G and params.K), as in the reference's signature.
+ * replaces: calculate_G_from_A  src/relaxtime/EffectiveCouplings.jl:56-60
+ *           calculate_effective_couplings, coupling_matrix_determinant
+ *           src/relaxtime/EffectiveCouplings.jl:240-278, :111-115           */
+int pnjl_effective_couplings_batch(pnjl_ctx* ctx, int64_t n, const double* in, int from_gf, double G, double K,
+                                   double* G_f, double* kc);
+
+/* Meson propagators from given Pi: for n rows of kc[n*12] (couplings as
+ * above) and pi[n*12] = Pi (re, im) in the pnjl_polarization_grid order
+ * (u ubar P, u ubar S, u sbar P, u sbar S, s sbar P, s sbar S), out[n*36],
+ * complex values as (re, im):
+ *    0 D_pi        2 D_K         4 D_sigma_pi    6 D_sigma_K
+ *    8 M00 (P)    10 M08 (P)    12 M88 (P)     14 M00 (S)   16 M08 (S)   18 M88 (S)
+ *   20 eta/eta' D for currents (J, J') = (a, a), 22 (a, b), 24 (b, a), 26 (b, b)
+ *   28 sigma/sigma' D for the same four current pairs
+ * where a is the current of a u ubar or d dbar pair (lambda0, lambda8 at u)
+ * and b that of s sbar (lambda8 at s); pairs of different flavour carry no
+ * current and give D = 0.  Host pointers.
+ * replaces: meson_propagator_simple, meson_propagator_mixed,
+ *           calculate_coupling_matrix / calculate_coupling_elements
+ *           src/relaxtime/MesonPropagator.jl:215-419
+ *           mixing_matrix_elements  src/relaxtime/EffectiveCouplings.jl:137-160 */
+#define PNJL_MESON_PROP_FIELDS 36
+int pnjl_meson_propagators_batch(pnjl_ctx* ctx, int64_t n, const double* kc, const double* pi, double* out);
+
+/* Mandelstam channel of the exchange (TotalPropagator.jl channel symbols) */
+#define PNJL_MANDELSTAM_T 0
+#define PNJL_MANDELSTAM_S 1
+#define PNJL_MANDELSTAM_U 2
+
+/* Scattering processes, in the order of SCATTERING_MESON_MAP
+ * (src/Constants_PNJL.jl:168-400).                                          */
+enum {
+    PNJL_PROC_UD_TO_UD = 0, PNJL_PROC_UU_TO_UU, PNJL_PROC_US_TO_US, PNJL_PROC_SS_TO_SS,
+    PNJL_PROC_UBARDBAR_TO_UBARDBAR, PNJL_PROC_UBARUBAR_TO_UBARUBAR, PNJL_PROC_UBARSBAR_TO_UBARSBAR,
+    PNJL_PROC_SBARSBAR_TO_SBARSBAR, PNJL_PROC_UDBAR_TO_UDBAR, PNJL_PROC_DUBAR_TO_DUBAR,
+    PNJL_PROC_UUBAR_TO_UUBAR, PNJL_PROC_UUBAR_TO_DDBAR, PNJL_PROC_USBAR_TO_USBAR, PNJL_PROC_SUBAR_TO_SUBAR,
+    PNJL_PROC_UUBAR_TO_SSBAR, PNJL_PROC_SSBAR_TO_UUBAR, PNJL_PROC_SSBAR_TO_SSBAR,
+    PNJL_PROC_COUNT
+};
+
+/* exchanged-meson masks */
+#define PNJL_MESON_PI       0x01
+#define PNJL_MESON_K        0x02
+#define PNJL_MESON_SIGMA_PI 0x04
+#define PNJL_MESON_SIGMA_K  0x08
+#define PNJL_MESON_MIXED_P  0x10   /* eta / eta'     */
+#define PNJL_MESON_MIXED_S  0x20   /* sigma / sigma' */
+
+/* One propagator query: the arguments of
+ *   calculate_propagator_by_channel(process, channel, k0, k_norm,
+ *                                   quark_params, thermo_params, K_coeffs)
+ *     src/relaxtime/TotalPropagator.jl:894-963
+ * with quark_params = (m, mu, A) of u (= d) and s and thermo_params =
+ * (T, Phi, Phibar, xi).  mesons = 0 takes the exchanged mesons from the
+ * process table (the reference's call); a PNJL_MESON_* mask instead gives
+ * total_propagator_simple's meson list (:423-470) and / or
+ * total_propagator_mixed (:514-560) for any process and channel.          */
+typedef struct pnjl_prop_query {
+    int    process;               /* PNJL_PROC_*                             */
+    int    channel;               /* PNJL_MANDELSTAM_*                       */
+    int    mesons;                /* 0 or a PNJL_MESON_* mask                */
+    int    reserved;
+    double k0, k;                 /* exchanged four-momentum, fm^-1          */
+    double T, Phi, Phib, xi;
+    double m_u, m_s, mu_u, mu_s, A_u, A_s;
+} pnjl_prop_query;
+
+/* (D_S, D_P) for n queries -> out[n*4] = (D_S re, im, D_P re, im).  One
+ * kernel evaluates the Pi values the exchanged mesons need (each with the
+ * pnjl_polarization_batch arithmetic), a second composes the propagators.
+ * kc = NULL derives the couplings from (A_u, A_s, m_u, m_s) with the
+ * context's G and K, as the reference's callers do; otherwise kc[n*12]
+ * gives them per query.  Host pointers unless PNJL_FLAG_DEVICE_PTRS; a query
+ * naming a process, channel or mask outside the table fails the call with
+ * PNJL_ERR_ARG (device pointers: its row is NaN).  No host-side memo: every
+ * Pi is evaluated (replaces polarization_aniso_cached's Dict,
+ * src/relaxtime/PolarizationCache.jl:203-256, for this consumer).
+ * replaces: calculate_propagator_by_channel, total_propagator_simple,
+ *           total_propagator_mixed, get_polarization_params,
+ *           get_flavor_factors_for_channel
+ *           src/relaxtime/TotalPropagator.jl:239-560, :894-963              */
+int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_query* q, const double* kc,
+                                     double* out, int flags, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

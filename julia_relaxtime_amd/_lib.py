@@ -196,6 +196,7 @@ class Context:
             raise PnjlError(f"pnjl_ctx_create failed with status {rc} (no gfx950 device visible?)")
         self.h = h
         self.device = device
+        self.params = params if params is not None else default_params()
 
     def close(self):
         if getattr(self, "h", None):

@@ -30,6 +30,8 @@ using namespace pnjl_host;
 // with m = m'), where a fused product leaves a 1e-17 residue instead of 0.
 #pragma clang fp contract(off)
 
+#include "pnjl_meson.h"
+
 namespace polar {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -832,6 +834,141 @@ __global__ void __launch_bounds__(256, 3) grid_pi_kernel(Phys P, const double* t
     }
 }
 
+// ---- consumers of Pi ---------------------------------------------------------------
+__device__ __forceinline__ void store_kc(double* o, const meson::KCoef& c)
+{
+    o[0] = c.K0p, o[1] = c.K0m, o[2] = c.K123p, o[3] = c.K123m, o[4] = c.K4567p, o[5] = c.K4567m;
+    o[6] = c.K8p, o[7] = c.K8m, o[8] = c.K08p, o[9] = c.K08m, o[10] = c.detKp, o[11] = c.detKm;
+}
+
+__device__ __forceinline__ meson::KCoef load_kc(const double* o)
+{
+    return meson::KCoef{o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9], o[10], o[11]};
+}
+
+__device__ __forceinline__ void store_c(double* o, meson::C v) { o[0] = v.r, o[1] = v.i; }
+
+// in[i*4] = (A_u, A_s, m_u, m_s), or in[i*2] = (G_u, G_s) -> G_f[i*2] (optional), kc[i*12]
+__global__ void __launch_bounds__(256) couplings_kernel(int64_t n, const double* in, int from_gf, double G, double K,
+                                                        int Nc, double* Gf, double* kc)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double Gu, Gs;
+    if (from_gf) {
+        Gu = in[i * 2], Gs = in[i * 2 + 1];
+    } else {
+        const double* a = in + i * 4;
+        Gu = meson::G_from_A(a[0], a[2], Nc), Gs = meson::G_from_A(a[1], a[3], Nc);
+    }
+    if (Gf) Gf[i * 2] = Gu, Gf[i * 2 + 1] = Gs;
+    store_kc(kc + i * 12, meson::couplings(G, K, Gu, Gs));
+}
+
+// kc[i*12], pi[i*12] (six slots) -> out[i*36] (layout of pnjl_meson_propagators_batch)
+__global__ void __launch_bounds__(256) meson_kernel(int64_t n, const double* kc, const double* pi, double* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using meson::C;
+    const meson::KCoef k = load_kc(kc + i * 12);
+    C P[6];
+    for (int s = 0; s < 6; s++) P[s] = C{pi[i * 12 + 2 * s], pi[i * 12 + 2 * s + 1]};
+    double* o = out + i * 36;
+    store_c(o + 0, meson::simple(k.K123p, P[0]));
+    store_c(o + 2, meson::simple(k.K4567p, P[2]));
+    store_c(o + 4, meson::simple(k.K123m, P[1]));
+    store_c(o + 6, meson::simple(k.K4567m, P[3]));
+    double J[2][2];   // currents a (u ubar) and b (s sbar)
+    meson::current(0, 0, J[0][0], J[0][1]);
+    meson::current(2, 2, J[1][0], J[1][1]);
+    for (int ch = 0; ch < 2; ch++) {
+        C M00, M08, M88;
+        const double detK = ch == 0 ? k.detKp : k.detKm;
+        if (ch == 0)
+            meson::mixing(P[0], P[4], k.K0p, k.K8p, k.K08p, k.detKp, M00, M08, M88);
+        else
+            meson::mixing(P[1], P[5], k.K0m, k.K8m, k.K08m, k.detKm, M00, M08, M88);
+        store_c(o + 8 + 6 * ch, M00);
+        store_c(o + 10 + 6 * ch, M08);
+        store_c(o + 12 + 6 * ch, M88);
+        for (int c = 0; c < 4; c++) {
+            const double* Ja = J[c >> 1];
+            const double* Jb = J[c & 1];
+            store_c(o + 20 + 8 * ch + 2 * c, meson::mixed(detK, M00, M08, M88, Ja[0], Ja[1], Jb[0], Jb[1]));
+        }
+    }
+}
+
+// the meson mask of a query, or MS_NONE for a process / channel / mask outside the table
+__device__ __host__ __forceinline__ unsigned query_mesons(const pnjl_prop_query& q, const meson::ProcRow* tab)
+{
+    if (q.process < 0 || q.process >= PNJL_PROC_COUNT || q.channel < 0 || q.channel > 2) return meson::MS_NONE;
+    if (q.mesons) return (q.mesons & ~0x3f) ? meson::MS_NONE : (unsigned)q.mesons;
+    return tab[q.process].ch[q.channel];
+}
+
+// Pi of the pairs a query's mesons need: thread = (pair, query), pair-major so
+// that a wavefront shares its pair; one B0 serves both channels of a pair.
+// pib[i*12 + 2*slot] as in pnjl_polarization_grid's order.
+__global__ void __launch_bounds__(256, 3) prop_pi_kernel(Phys P, const double* tab, int64_t n,
+                                                      const pnjl_prop_query* qs, double* pib)
+{
+    __shared__ double lds[kTabLen];
+    load_tabs(lds, tab);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n) return;
+    const int pair = (int)(t / n);
+    const int64_t i = t - pair * n;
+    const pnjl_prop_query a = qs[i];
+    const unsigned mes = query_mesons(a, meson::kProcs);
+    const unsigned need = mes == meson::MS_NONE ? 0u : (meson::slots_for(mes) >> (2 * pair)) & 3u;
+    double* o = pib + i * 12 + 4 * pair;
+    if (!need) {
+        o[0] = o[1] = o[2] = o[3] = 0.0;
+        return;
+    }
+    const Tabs q = make_tabs(lds);
+    const double m1 = pair == 2 ? a.m_s : a.m_u, m2 = pair == 0 ? a.m_u : a.m_s;
+    const double mu1 = pair == 2 ? a.mu_s : a.mu_u, mu2 = pair == 0 ? a.mu_u : a.mu_s;
+    const double A1 = pair == 2 ? a.A_s : a.A_u, A2 = pair == 0 ? a.A_u : a.A_s;
+    double br, bi;
+    b0_total(P, q, a.k0, a.k, m1, m2, mu1, mu2, a.T, a.Phi, a.Phib, a.xi, pair, br, bi);
+    for (int c = 0; c < 2; c++) {
+        if (need & (1u << c))
+            pi_value(P, c == 0 ? PNJL_CHANNEL_P : PNJL_CHANNEL_S, 0, a.k0, 0.0, a.k, m1, m2, a.k0 + mu1 - mu2, A1 + A2,
+                     br, bi, o[2 * c], o[2 * c + 1]);
+        else
+            o[2 * c] = o[2 * c + 1] = 0.0;
+    }
+}
+
+// (D_S, D_P) per query from its Pi slots; kc == nullptr: couplings from A
+__global__ void __launch_bounds__(256) prop_compose_kernel(int64_t n, const pnjl_prop_query* qs, const double* kc,
+                                                           double G, double K, int Nc, const double* pib,
+                                                           double* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using meson::C;
+    const pnjl_prop_query a = qs[i];
+    const unsigned mes = query_mesons(a, meson::kProcs);
+    double* o = out + i * 4;
+    if (mes == meson::MS_NONE) {
+        o[0] = o[1] = o[2] = o[3] = __builtin_nan("");
+        return;
+    }
+    const meson::KCoef k = kc ? load_kc(kc + i * 12)
+                              : meson::couplings(G, K, meson::G_from_A(a.A_u, a.m_u, Nc),
+                                                 meson::G_from_A(a.A_s, a.m_s, Nc));
+    C pi[6];
+    for (int s = 0; s < 6; s++) pi[s] = C{pib[i * 12 + 2 * s], pib[i * 12 + 2 * s + 1]};
+    C DS, DP;
+    meson::by_channel(meson::kProcs[a.process], a.channel, mes, k, pi, DS, DP);
+    store_c(o, DS);
+    store_c(o + 2, DP);
+}
+
 }  // namespace polar
 
 // =============================================================================
@@ -1108,6 +1245,102 @@ int pnjl_polarization_grid(pnjl_ctx* ctx, int npts, const pnjl_pol_point* pts, i
     HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
     HIPCHK(ctx, hipMemcpyAsync(re, g->re, (size_t)nout * sizeof(double), outk, st));
     HIPCHK(ctx, hipMemcpyAsync(im, g->im, (size_t)nout * sizeof(double), outk, st));
+    if (!(flags & PNJL_FLAG_ASYNC) || !dev) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    }
+    return PNJL_OK;
+}
+
+int pnjl_effective_couplings_batch(pnjl_ctx* ctx, int64_t n, const double* in, int from_gf, double G, double K,
+                                   double* G_f, double* kc)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || (n > 0 && (!in || !kc)))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_effective_couplings_batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int w = from_gf ? 2 : 4;
+    void *a, *g, *o;
+    if (int rc = stage_buf(ctx, 80, (size_t)n * w * sizeof(double), &a)) return rc;
+    if (int rc = stage_buf(ctx, 81, (size_t)n * 2 * sizeof(double), &g)) return rc;
+    if (int rc = stage_buf(ctx, 82, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), &o)) return rc;
+    HIPCHK(ctx, hipMemcpy(a, in, (size_t)n * w * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(polar::couplings_kernel, dim3(blocks_for(n)), dim3(256), 0, 0, n, (const double*)a,
+                       from_gf ? 1 : 0, G, K, ctx->params.N_color, (double*)g, (double*)o);
+    HIPCHK(ctx, hipGetLastError());
+    if (G_f) HIPCHK(ctx, hipMemcpy(G_f, g, (size_t)n * 2 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(kc, o, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), hipMemcpyDeviceToHost));
+    return PNJL_OK;
+}
+
+int pnjl_meson_propagators_batch(pnjl_ctx* ctx, int64_t n, const double* kc, const double* pi, double* out)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || (n > 0 && (!kc || !pi || !out)))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_meson_propagators_batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    void *k, *p, *o;
+    if (int rc = stage_buf(ctx, 80, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), &k)) return rc;
+    if (int rc = stage_buf(ctx, 81, (size_t)n * 12 * sizeof(double), &p)) return rc;
+    if (int rc = stage_buf(ctx, 82, (size_t)n * PNJL_MESON_PROP_FIELDS * sizeof(double), &o)) return rc;
+    HIPCHK(ctx, hipMemcpy(k, kc, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(p, pi, (size_t)n * 12 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(polar::meson_kernel, dim3(blocks_for(n)), dim3(256), 0, 0, n, (const double*)k,
+                       (const double*)p, (double*)o);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpy(out, o, (size_t)n * PNJL_MESON_PROP_FIELDS * sizeof(double), hipMemcpyDeviceToHost));
+    return PNJL_OK;
+}
+
+int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_query* q, const double* kc,
+                                     double* out, int flags, void* stream)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || (n > 0 && (!q || !out)))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_propagator_by_channel_batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    if (n > ((int64_t)1 << 29)) return set_err(ctx, PNJL_ERR_ARG, "pnjl_propagator_by_channel_batch: n too large");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & PNJL_FLAG_DEVICE_PTRS;
+    if (!dev)
+        for (int64_t i = 0; i < n; i++)
+            if (polar::query_mesons(q[i], meson::kProcsHost) == meson::MS_NONE)
+                return set_err(ctx, PNJL_ERR_ARG,
+                               "pnjl_propagator_by_channel_batch: process, channel or meson mask not in the table");
+    double* tab = nullptr;
+    if (int rc = polar_tab(ctx, &tab)) return rc;
+    void* pib;
+    if (int rc = stage_buf(ctx, 83, (size_t)n * 12 * sizeof(double), &pib)) return rc;
+    const pnjl_prop_query* dq = q;
+    const double* dkc = kc;
+    double* dout = out;
+    if (!dev) {
+        void *a, *b, *c = nullptr;
+        if (int rc = stage_buf(ctx, 80, (size_t)n * sizeof(pnjl_prop_query), &a)) return rc;
+        if (int rc = stage_buf(ctx, 81, (size_t)n * 4 * sizeof(double), &b)) return rc;
+        if (kc)
+            if (int rc = stage_buf(ctx, 82, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), &c)) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(a, q, (size_t)n * sizeof(pnjl_prop_query), hipMemcpyHostToDevice, st));
+        if (kc)
+            HIPCHK(ctx, hipMemcpyAsync(c, kc, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), hipMemcpyHostToDevice,
+                                       st));
+        dq = (const pnjl_prop_query*)a;
+        dkc = (const double*)c;
+        dout = (double*)b;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(polar::prop_pi_kernel, dim3(blocks_for(3 * n)), dim3(256), 0, st, phys(ctx), tab, n, dq,
+                       (double*)pib);
+    HIPCHK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(polar::prop_compose_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, dq, dkc, ctx->params.G,
+                       ctx->params.K, ctx->params.N_color, (const double*)pib, dout);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
+    if (!dev) HIPCHK(ctx, hipMemcpyAsync(out, dout, (size_t)n * 4 * sizeof(double), hipMemcpyDeviceToHost, st));
     if (!(flags & PNJL_FLAG_ASYNC) || !dev) {
         HIPCHK(ctx, hipStreamSynchronize(st));
         float ms = 0.f;

@@ -1,0 +1,290 @@
+"""CPU oracle of the consumers of the polarization Pi: effective couplings,
+meson propagators and the per-process D_S / D_P.
+
+TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path
+(pnjl_effective_couplings_batch, pnjl_meson_propagators_batch,
+pnjl_propagator_by_channel_batch) is compared against; only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import it.
+
+A Python restatement, with complex scalars, of
+  src/relaxtime/EffectiveCouplings.jl   calculate_G_from_A :56-60,
+      calculate_effective_couplings :240-278, coupling_matrix_determinant
+      :111-115, mixing_matrix_elements :137-160
+  src/relaxtime/MesonPropagator.jl      calculate_current_vector :133-141,
+      meson_propagator_simple :268-291, meson_propagator_mixed :365-419
+  src/relaxtime/TotalPropagator.jl      get_flavor_factor :239-258,
+      get_flavor_factors_for_channel :260-275, get_polarization_params
+      :308-336, total_propagator_simple :423-470, total_propagator_mixed
+      :514-560, calculate_propagator_by_channel :894-963
+  src/Constants_PNJL.jl                 lambda0 / lambda8 :106-117, the
+      process table SCATTERING_MESON_MAP :168-400
+The Pi values come from the oracle's C restatement of polarization_aniso
+(oracle.polarization_batch).  Flavour structure is built the reference's way
+(lambda matrices between flavour vectors), not from a precomputed table.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import oracle as O
+
+DENOM_EPS = 1e-12                      # PROPAGATOR_DENOM_EPS  MesonPropagator.jl:22
+
+# Constants_PNJL.jl:106-117
+LAMBDA0 = np.eye(3) * math.sqrt(2 / 3)
+LAMBDA8 = np.diag([1.0, 1.0, -2.0]) / math.sqrt(3)
+PSI = {"u": np.array([1.0, 0.0, 0.0]), "d": np.array([0.0, 1.0, 0.0]), "s": np.array([0.0, 0.0, 1.0])}
+
+KC_FIELDS = ["K0_plus", "K0_minus", "K123_plus", "K123_minus", "K4567_plus", "K4567_minus", "K8_plus", "K8_minus",
+             "K08_plus", "K08_minus", "det_K_plus", "det_K_minus"]
+
+# SCATTERING_MESON_MAP (Constants_PNJL.jl:168-400), in its order:
+# process -> {channel: (simple mesons, mixed_P, mixed_S)}
+_PI = ("pi", "sigma_pi")
+_K = ("K", "sigma_K")
+PROCESS_TABLE = {
+    "ud_to_ud": {"t": (_PI, True, True), "u": (_PI, False, False)},
+    "uu_to_uu": {"t": (_PI, True, True), "u": (_PI, True, True)},
+    "us_to_us": {"t": ((), True, True), "u": (_K, False, False)},
+    "ss_to_ss": {"t": ((), True, True), "u": ((), True, True)},
+    "ubardbar_to_ubardbar": {"t": (_PI, True, True), "u": (_PI, False, False)},
+    "ubarubar_to_ubarubar": {"t": (_PI, True, True), "u": (_PI, True, True)},
+    "ubarsbar_to_ubarsbar": {"t": ((), True, True), "u": (_K, False, False)},
+    "sbarsbar_to_sbarsbar": {"t": ((), True, True), "u": ((), True, True)},
+    "udbar_to_udbar": {"t": (_PI, True, True), "s": (_PI, False, False)},
+    "dubar_to_dubar": {"t": (_PI, True, True), "s": (_PI, False, False)},
+    "uubar_to_uubar": {"t": (_PI, True, True), "s": (_PI, True, True)},
+    "uubar_to_ddbar": {"t": (_PI, False, False), "s": (_PI, True, True)},
+    "usbar_to_usbar": {"t": ((), True, True), "s": (_K, False, False)},
+    "subar_to_subar": {"t": ((), True, True), "s": (_K, False, False)},
+    "uubar_to_ssbar": {"t": (_K, False, False), "s": ((), True, True)},
+    "ssbar_to_uubar": {"t": (_K, False, False), "s": ((), True, True)},
+    "ssbar_to_ssbar": {"t": ((), True, True), "s": ((), True, True)},
+}
+PROCESSES = list(PROCESS_TABLE)
+CHANNELS = {"t": 0, "s": 1, "u": 2}
+MESON_BITS = {"pi": 1, "K": 2, "sigma_pi": 4, "sigma_K": 8, "mixed_P": 16, "mixed_S": 32}
+
+
+def parse_process(name):
+    """'udbar_to_udbar' -> ('u', 'dbar', 'u', 'dbar')  (ParticleSymbols.parse_scattering_process)."""
+    out = []
+    for side in name.split("_to_"):
+        i = 0
+        while i < len(side):
+            f = side[i]
+            if side.startswith("bar", i + 1):
+                out.append(f + "bar")
+                i += 4
+            else:
+                out.append(f)
+                i += 1
+    assert len(out) == 4, name
+    return tuple(out)
+
+
+def flavour(q):
+    return q[0]
+
+
+# ---- EffectiveCouplings.jl --------------------------------------------------
+def G_from_A(A, m, Nc=3):
+    """calculate_G_from_A  EffectiveCouplings.jl:56-60."""
+    return -Nc / (4.0 * (math.pi * math.pi)) * (m * A)     # Julia's literal x^2 is x * x
+
+
+def coupling_matrix_determinant(K0, K8, K08):
+    """EffectiveCouplings.jl:111-115."""
+    return K0 * K8 - K08 * K08
+
+
+def effective_couplings(G, K, Gu, Gs):
+    """calculate_effective_couplings  EffectiveCouplings.jl:240-278."""
+    t0 = (1.0 / 3.0) * K * (2.0 * Gu + Gs)
+    t123 = 0.5 * K * Gs
+    t4567 = 0.5 * K * Gu
+    t8 = (1.0 / 6.0) * K * (4.0 * Gu - Gs)
+    t08 = (1.0 / 6.0) * math.sqrt(2.0) * K * (Gu - Gs)
+    c = dict(K0_plus=G - t0, K0_minus=G + t0, K123_plus=G + t123, K123_minus=G - t123, K4567_plus=G + t4567,
+             K4567_minus=G - t4567, K8_plus=G + t8, K8_minus=G - t8, K08_plus=t08, K08_minus=-t08)
+    c["det_K_plus"] = coupling_matrix_determinant(c["K0_plus"], c["K8_plus"], c["K08_plus"])
+    c["det_K_minus"] = coupling_matrix_determinant(c["K0_minus"], c["K8_minus"], c["K08_minus"])
+    return c
+
+
+def mixing_matrix_elements(Puu, Pss, kc, channel):
+    """EffectiveCouplings.jl:137-160 -> (M00, M08, M88, det_K)."""
+    sfx = {"P": "plus", "S": "minus"}[channel]
+    K0, K8, K08, dK = (kc[f"{k}_{sfx}"] for k in ("K0", "K8", "K08", "det_K"))
+    M00 = K0 - (4.0 / 3.0) * dK * (Puu + 2.0 * Pss)
+    M08 = K08 + (4.0 / 3.0) * math.sqrt(2.0) * dK * (Puu - Pss)
+    M88 = K8 - (4.0 / 3.0) * dK * (2.0 * Puu + Pss)
+    return M00, M08, M88, dK
+
+
+# ---- MesonPropagator.jl -----------------------------------------------------
+def current_vector(q1, q2):
+    """(psibar_{q2} lambda0 psi_{q1}, psibar_{q2} lambda8 psi_{q1})  MesonPropagator.jl:133-141."""
+    a, b = PSI[flavour(q1)], PSI[flavour(q2)]
+    return float(b @ LAMBDA0 @ a), float(b @ LAMBDA8 @ a)
+
+
+def _finite(z):
+    return math.isfinite(z.real) and math.isfinite(z.imag)
+
+
+def meson_propagator_simple(meson, kc, Pi):
+    """MesonPropagator.jl:268-291: D = 2K / (1 - 4 K Pi)."""
+    key = {"pi": "K123_plus", "K": "K4567_plus", "sigma_pi": "K123_minus", "sigma_K": "K4567_minus"}
+    if meson not in key:
+        raise ValueError(f"Unknown meson type: {meson}")
+    K = kc[key[meson]]
+    den = 1.0 - 4.0 * K * complex(Pi)
+    if not _finite(den):
+        return 0j
+    if abs(den) < DENOM_EPS:
+        den += complex(0.0, DENOM_EPS)
+    return (2.0 * K) / den
+
+
+def meson_propagator_mixed(det_K, M00, M08, M88, q1, q2, q3, q4, channel):
+    """MesonPropagator.jl:365-419: D = 2 det K / det M * J^T M J'."""
+    legs = {"t": ((q1, q3), (q2, q4)), "s": ((q1, q2), (q3, q4)), "u": ((q1, q4), (q2, q3))}
+    if channel not in legs:
+        raise ValueError(f"Unknown channel: {channel}")
+    (a, b), (c, d) = legs[channel]
+    J0, J8 = current_vector(a, b)
+    Jp0, Jp8 = current_vector(c, d)
+    det_M = M00 * M88 - M08 * M08
+    if not _finite(det_M):
+        return 0j
+    if abs(det_M) < DENOM_EPS:
+        det_M += complex(0.0, DENOM_EPS)
+    r = J0 * (M00 * Jp0 + M08 * Jp8) + J8 * (M08 * Jp0 + M88 * Jp8)
+    if not _finite(r):
+        return 0j
+    return 2.0 * det_K / det_M * r
+
+
+# ---- TotalPropagator.jl -----------------------------------------------------
+def flavour_factor(q1, q2):
+    """Table 5.3  TotalPropagator.jl:239-258."""
+    f1, f2 = flavour(q1), flavour(q2)
+    if f1 == f2:
+        return {"u": 1.0, "d": -1.0, "s": 2.0}[f1]
+    return math.sqrt(2.0)
+
+
+def flavour_factors_for_channel(process, channel):
+    """TotalPropagator.jl:260-275."""
+    q1, q2, q3, q4 = parse_process(process)
+    pairs = {"t": ((q1, q3), (q2, q4)), "u": ((q1, q4), (q2, q3)), "s": ((q1, q2), (q3, q4))}
+    (a, b), (c, d) = pairs[channel]
+    return flavour_factor(a, b), flavour_factor(c, d)
+
+
+def polarization_params(meson, qp):
+    """get_polarization_params  TotalPropagator.jl:308-336 (+ the s sbar pair
+    of total_propagator_mixed :530-537 as 'ss_P' / 'ss_S')."""
+    u, s = "u", "s"
+    table = {"pi": ("P", u, u, 0), "K": ("P", u, s, 1), "sigma_pi": ("S", u, u, 0), "sigma_K": ("S", u, s, 1),
+             "uu_P": ("P", u, u, 0), "uu_S": ("S", u, u, 0), "ss_P": ("P", s, s, 2), "ss_S": ("S", s, s, 2)}
+    ch, f1, f2, ns = table[meson]
+    return dict(channel=ch, m1=qp["m"][f1], m2=qp["m"][f2], mu1=qp["mu"][f1], mu2=qp["mu"][f2], A1=qp["A"][f1],
+                A2=qp["A"][f2], num_s=ns)
+
+
+def _mesons_of(process, channel, mask):
+    if mask == 0:
+        simple, mP, mS = PROCESS_TABLE[process][channel]
+        return list(simple), mP, mS
+    simple = [m for m in ("pi", "K", "sigma_pi", "sigma_K") if mask & MESON_BITS[m]]
+    return simple, bool(mask & 16), bool(mask & 32)
+
+
+def propagator_by_channel(process, channel, k0, k, qp, tp, kc, mesons=0, pi=None):
+    """calculate_propagator_by_channel  TotalPropagator.jl:894-963 -> (D_S, D_P).
+    qp = dict(m=..., mu=..., A=...) keyed by flavour, tp = dict(T, Phi, Phib, xi);
+    pi: optional {name: complex} of precomputed Pi (names of polarization_params)."""
+    simple, mP, mS = _mesons_of(process, channel, mesons)
+    T1, T2 = flavour_factors_for_channel(process, channel)
+    pi = pi if pi is not None else {}
+
+    def Pi(name):
+        if name not in pi:
+            p = polarization_params(name, qp)
+            re, im = O.polarization(O.CHANNEL_P if p["channel"] == "P" else O.CHANNEL_S, k0, k, p["m1"], p["m2"],
+                                    p["mu1"], p["mu2"], tp["T"], tp["Phi"], tp["Phib"], tp["xi"], p["A1"], p["A2"],
+                                    p["num_s"])
+            pi[name] = complex(re, im)
+        return pi[name]
+
+    DS, DP = 0j, 0j
+    for m in simple:
+        D = meson_propagator_simple(m, kc, Pi({"pi": "uu_P", "sigma_pi": "uu_S", "K": "K", "sigma_K": "sigma_K"}[m]))
+        if m in ("pi", "K"):
+            DP += T1 * D * T2
+        else:
+            DS += T1 * D * T2
+    q = parse_process(process)
+    for on, ch in ((mP, "P"), (mS, "S")):
+        if on:
+            M00, M08, M88, dK = mixing_matrix_elements(Pi(f"uu_{ch}"), Pi(f"ss_{ch}"), kc, ch)
+            D = meson_propagator_mixed(dK, M00, M08, M88, *q, channel)
+            if ch == "P":
+                DP += D
+            else:
+                DS += D
+    return DS, DP
+
+
+def needed_pi(process, channel, mesons=0):
+    """Names of the Pi values a query evaluates."""
+    simple, mP, mS = _mesons_of(process, channel, mesons)
+    names = {"pi": "uu_P", "sigma_pi": "uu_S", "K": "K", "sigma_K": "sigma_K"}
+    out = {names[m] for m in simple}
+    if mP:
+        out |= {"uu_P", "ss_P"}
+    if mS:
+        out |= {"uu_S", "ss_S"}
+    return sorted(out)
+
+
+def propagator_by_channel_batch(q, G, K, kc=None, nthreads=8, Nc=3):
+    """(D_S, D_P) for a structured array of queries (fields of
+    pnjl_prop_query); all Pi values in one multi-threaded oracle batch.
+    kc = None derives the couplings from A as the reference's callers do."""
+    n = len(q)
+    names = [needed_pi(PROCESSES[int(r["process"])], "tsu"[int(r["channel"])], int(r["mesons"])) for r in q]
+    rows = []
+    for i, r in enumerate(q):
+        qp = dict(m=dict(u=r["m_u"], s=r["m_s"]), mu=dict(u=r["mu_u"], s=r["mu_s"]), A=dict(u=r["A_u"], s=r["A_s"]))
+        for nm in names[i]:
+            p = polarization_params(nm, qp)
+            rows.append((O.CHANNEL_P if p["channel"] == "P" else O.CHANNEL_S, p["num_s"], r["k0"], r["k"], p["m1"],
+                         p["m2"], p["mu1"], p["mu2"], r["T"], r["Phi"], r["Phib"], r["xi"], p["A1"], p["A2"]))
+    pq = (O.PolQuery * len(rows))()
+    for j, t in enumerate(rows):
+        x = pq[j]
+        x.channel, x.num_s, x.width = int(t[0]), int(t[1]), 0
+        x.k0, x.gamma, x.k, x.m1, x.m2, x.mu1, x.mu2 = t[2], 0.0, t[3], t[4], t[5], t[6], t[7]
+        x.T, x.Phi, x.Phib, x.xi, x.A1, x.A2 = t[8], t[9], t[10], t[11], t[12], t[13]
+    re, im = O.polarization_batch(pq, nthreads=nthreads) if rows else (np.zeros(0), np.zeros(0))
+    DS, DP = np.zeros(n, complex), np.zeros(n, complex)
+    j = 0
+    for i, r in enumerate(q):
+        pi = {}
+        for nm in names[i]:
+            pi[nm] = complex(re[j], im[j])
+            j += 1
+        if kc is None:
+            c = effective_couplings(G, K, G_from_A(r["A_u"], r["m_u"], Nc), G_from_A(r["A_s"], r["m_s"], Nc))
+        else:
+            c = dict(zip(KC_FIELDS, kc[i]))
+        qp = dict(m=dict(u=r["m_u"], s=r["m_s"]), mu=dict(u=r["mu_u"], s=r["mu_s"]), A=dict(u=r["A_u"], s=r["A_s"]))
+        tp = dict(T=r["T"], Phi=r["Phi"], Phib=r["Phib"], xi=r["xi"])
+        DS[i], DP[i] = propagator_by_channel(PROCESSES[int(r["process"])], "tsu"[int(r["channel"])], r["k0"], r["k"],
+                                             qp, tp, c, int(r["mesons"]), pi)
+    return DS, DP
