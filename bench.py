@@ -84,6 +84,10 @@ WORKLOADS = {
                                "(T outer, mu inner, no reset), FP64, |res|<1e-10"),
     "c3_polarization": dict(polar=True, desc="256 points x {P,S} x {uu,us,ss} x 128 k0 x 64 k = 12,582,912 complex "
                                               "Pi (polarization_aniso), hipGraph replay, FP64"),
+    "propagators": dict(props=True, desc="1,048,576 calculate_propagator_by_channel queries: the 34 (process, "
+                                         "channel) entries of the process table uniformly, at the 256 equilibrium "
+                                         "points of the Pi workload, (k0, k) on its axes; every Pi evaluated "
+                                         "(no memo), couplings from A; FP64"),
     "c5_multiroot": dict(grid=lambda: grid(512, 512, [-0.2, -0.1, 0.0, 0.1, 0.2], (10, 250), (200, 450)), p_num=64,
                          t_num=4, max_iter=40, learned=True, policy="MULTIROOT3", plain_policy="MULTI",
                          desc="512x512x5 (T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2 step 0.1), p=64 t=4, "
@@ -225,6 +229,81 @@ def train_learned_seed(ctx, w, opts):
     return s, info
 
 
+PROP_METRIC = "meson propagator (D_S, D_P) queries/s (calculate_propagator_by_channel, 1,048,576-query batch)"
+# FP64 FLOPs per query of this batch, counted once like C3_FLOPS_PER_VALUE:
+# FP64 VALU counters on 65,536 of its queries run one (pair, query) per
+# wavefront (python3 -m julia_relaxtime_amd.prop_flop_probe).  The composition
+# kernel (~10^2 FLOPs per query) is not counted.
+PROP_FLOPS_PER_QUERY = None
+
+
+def prop_cpu_baseline(q, G, K, threads=None):
+    """The oracle's calculate_propagator_by_channel on the whole batch: Pi in
+    C (OpenMP over values), queries built and propagators composed as numpy
+    array operations."""
+    from oracle import mesons as OM
+    threads = threads or min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    OM.propagator_by_channel_vec(q, G, K, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=q.size / dt, unit="queries/s", cores=threads, kind="port",
+                sample=f"the whole {q.size}-query batch, oracle Pi (C, {threads} OpenMP threads) + numpy "
+                       f"composition, {dt:.2f} s wall")
+
+
+def run_propagators(args, w):
+    import torch
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import propagators as PR
+    from julia_relaxtime_amd.workloads import pi_grid_points, propagator_queries
+    dev = torch.device("cuda", 0)
+    ctx = _lib.Context(0)
+    pts = pi_grid_points(ctx, _lib.default_opts(ftol=1e-10, max_iter=40))
+    q = propagator_queries(ctx, pts)
+    n = q.size
+    qd = torch.from_numpy(q.view(np.uint8)).to(dev)
+    out = torch.empty((n, 4), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    pctx = PR._ctx(0)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def step():
+        ev[0].record(stream)
+        PR.propagator_by_channel_device(qd, out, stream=stream)
+        ev[1].record(stream)
+        return ev
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        ev[1].synchronize()
+        kms.append(ev[0].elapsed_time(ev[1]))
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) / args.steps * 1e3
+    o = out.cpu().numpy()
+    kavg = float(np.mean(kms))
+    flops = None if PROP_FLOPS_PER_QUERY is None else PROP_FLOPS_PER_QUERY * n
+    achieved = None if flops is None else flops / (kavg * 1e-3) / 1e12
+    line = {
+        "metric": PROP_METRIC, "value": n / (ms_step * 1e-3), "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(workload=args.workload, queries=n, points=len(pts), detail=w["desc"],
+                       nonfinite=int((~np.isfinite(o)).sum()), kernel_ms_per_step=kavg),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": None if achieved is None else achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "polar::prop_pi_kernel + prop_compose_kernel", "kernel_ms_per_step": kavg,
+                     "flops_per_query": PROP_FLOPS_PER_QUERY},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = prop_cpu_baseline(q, pctx.params.G, pctx.params.K)
+    print(json.dumps(line), flush=True)
+
+
 def timed_steps(step, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -263,6 +342,8 @@ def main():
     w = WORKLOADS[args.workload]
     if w.get("polar"):
         return run_polar(args, w)
+    if w.get("props"):
+        return run_propagators(args, w)
     T, mu, xi = w["grid"]()
     n = T.size
     dev = torch.device("cuda", 0)

@@ -911,12 +911,15 @@ __device__ __host__ __forceinline__ unsigned query_mesons(const pnjl_prop_query&
 // Pi of the pairs a query's mesons need: thread = (pair, query), pair-major so
 // that a wavefront shares its pair; one B0 serves both channels of a pair.
 // pib[i*12 + 2*slot] as in pnjl_polarization_grid's order.
+// stride 64 puts one (pair, query) per wavefront, for counting FLOPs.
 __global__ void __launch_bounds__(256, 3) prop_pi_kernel(Phys P, const double* tab, int64_t n,
-                                                      const pnjl_prop_query* qs, double* pib)
+                                                      const pnjl_prop_query* qs, double* pib, int stride)
 {
     __shared__ double lds[kTabLen];
     load_tabs(lds, tab);
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ts % stride) return;
+    const int64_t t = ts / stride;
     if (t >= 3 * n) return;
     const int pair = (int)(t / n);
     const int64_t i = t - pair * n;
@@ -1302,7 +1305,8 @@ int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_q
     if (n < 0 || (n > 0 && (!q || !out)))
         return set_err(ctx, PNJL_ERR_ARG, "pnjl_propagator_by_channel_batch: bad argument");
     if (n == 0) return PNJL_OK;
-    if (n > ((int64_t)1 << 29)) return set_err(ctx, PNJL_ERR_ARG, "pnjl_propagator_by_channel_batch: n too large");
+    if (n > ((int64_t)1 << 29) || ((flags & PNJL_FLAG_ONE_PER_WAVE) && n > ((int64_t)1 << 23)))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_propagator_by_channel_batch: n too large");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & PNJL_FLAG_DEVICE_PTRS;
@@ -1333,8 +1337,9 @@ int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_q
         dout = (double*)b;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
-    hipLaunchKernelGGL(polar::prop_pi_kernel, dim3(blocks_for(3 * n)), dim3(256), 0, st, phys(ctx), tab, n, dq,
-                       (double*)pib);
+    const int stride = (flags & PNJL_FLAG_ONE_PER_WAVE) ? 64 : 1;
+    hipLaunchKernelGGL(polar::prop_pi_kernel, dim3(blocks_for(3 * n * stride)), dim3(256), 0, st, phys(ctx), tab, n,
+                       dq, (double*)pib, stride);
     HIPCHK(ctx, hipGetLastError());
     hipLaunchKernelGGL(polar::prop_compose_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, dq, dkc, ctx->params.G,
                        ctx->params.K, ctx->params.N_color, (const double*)pib, dout);

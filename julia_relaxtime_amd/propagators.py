@@ -234,18 +234,47 @@ def make_queries(process, channel, k0, k, T, Phi, Phib, xi, m_u, m_s, mu_u, mu_s
     return q
 
 
-def propagator_by_channel_batch(q, K_coeffs=None, device=0):
+def process_channels(process):
+    """The Mandelstam channels SCATTERING_MESON_MAP lists for a process: t and u
+    for quark-quark (and antiquark-antiquark), t and s for quark-antiquark."""
+    left = process.split("_to_")[0]
+    return ("t", "s") if left.count("bar") == 1 else ("t", "u")
+
+
+def propagator_by_channel_batch(q, K_coeffs=None, device=0, one_per_wave=False):
     """(D_S, D_P) complex arrays for a PROP_QUERY_DTYPE array.  K_coeffs None:
-    couplings from each query's A (the reference callers' K_coeffs)."""
+    couplings from each query's A (the reference callers' K_coeffs).
+    one_per_wave: one (Pi pair, query) per wavefront, for FLOP counting."""
     ctx = _ctx(device)
     q = np.ascontiguousarray(q, dtype=PROP_QUERY_DTYPE)
     n = q.size
     kc = None if K_coeffs is None else _kc_rows(K_coeffs, n)
     out = np.zeros((n, 4))
+    flags = _lib.FLAG_ONE_PER_WAVE if one_per_wave else 0
     rc = _lib.load().pnjl_propagator_by_channel_batch(ctx.h, n, _ptr(q), None if kc is None else _ptr(kc),
-                                                       _ptr(out), 0, None)
+                                                       _ptr(out), flags, None)
     ctx._check(rc, "pnjl_propagator_by_channel_batch")
     return out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
+
+
+def propagator_by_channel_device(q, out, K_coeffs=None, stream=None, device=0):
+    """Device-resident form: q a torch uint8 tensor holding PROP_QUERY_DTYPE
+    rows in HBM, out a float64 tensor (n, 4) = (D_S re, im, D_P re, im),
+    K_coeffs None or a float64 (n, 12) tensor; launches on `stream` (a
+    torch.cuda.Stream, default the current one) and returns without a host
+    copy (PNJL_FLAG_DEVICE_PTRS | PNJL_FLAG_ASYNC)."""
+    import torch
+    ctx = _ctx(device)
+    n = q.numel() // PROP_QUERY_DTYPE.itemsize
+    if out.dtype != torch.float64 or out.numel() != 4 * n or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float64 tensor of n x 4")
+    st = stream if stream is not None else torch.cuda.current_stream(q.device)
+    flags = _lib.FLAG_DEVICE_PTRS | _lib.FLAG_ASYNC
+    rc = _lib.load().pnjl_propagator_by_channel_batch(ctx.h, n, C.c_void_p(q.data_ptr()),
+                                                       None if K_coeffs is None else C.c_void_p(K_coeffs.data_ptr()),
+                                                       C.c_void_p(out.data_ptr()), flags, C.c_void_p(st.cuda_stream))
+    ctx._check(rc, "pnjl_propagator_by_channel_batch")
+    return out
 
 
 def _query(process, channel, k0, k_norm, qp, tp, mesons):
@@ -293,5 +322,6 @@ def total_propagator_mixed(process, channel, meson_channel, k0, k_norm, quark_pa
 __all__ = ["PROCESSES", "PROP_QUERY_DTYPE", "KC_FIELDS", "PI_SLOTS", "couplings_from_A", "calculate_G_from_A",
            "calculate_effective_couplings", "meson_propagators_batch", "meson_propagator_simple",
            "mixing_matrix_elements", "calculate_coupling_matrix", "meson_propagator_mixed", "make_queries",
-           "propagator_by_channel_batch", "calculate_propagator_by_channel", "total_propagator_simple",
+           "propagator_by_channel_batch", "propagator_by_channel_device", "process_channels",
+           "calculate_propagator_by_channel", "total_propagator_simple",
            "total_propagator_mixed"]

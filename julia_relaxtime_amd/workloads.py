@@ -42,3 +42,31 @@ def pi_grid_points(ctx, opts, n=256, seed=20261202):
     allp = np.concatenate(rows)
     idx = np.random.default_rng(seed).choice(len(allp), n, replace=False)
     return allp[idx]
+
+
+def propagator_queries(ctx, pts, n=1 << 20, seed=20261203):
+    """Propagator workload: n (process, channel) queries drawn uniformly from
+    the 34 channels of the process table, each at one of the Pi workload's
+    equilibrium points (pts from pi_grid_points) with A_aniso(m_u), A_aniso(m_s)
+    of that point (GL 64 on [0, 10] fm^-1 x GL 4 in cos) and (k0, k) drawn
+    from the Pi grid's axes.  Returns a PROP_QUERY_DTYPE array."""
+    from . import _lib
+    from . import polarization as PZ
+    from . import propagators as PR
+    xp, wp = _lib.gauleg(0.0, 10.0, 64)
+    xc, wc = np.polynomial.legendre.leggauss(4)
+    args = np.concatenate([np.stack([pts[:, 3], pts[:, 1], pts[:, 0], pts[:, 5], pts[:, 6], pts[:, 2]], 1),
+                           np.stack([pts[:, 4], pts[:, 1], pts[:, 0], pts[:, 5], pts[:, 6], pts[:, 2]], 1)])
+    A = PZ.A_batch(args, xp, wp, xc, wc).reshape(2, len(pts))
+    pc = [(PR.PROCESS_ID[p], PR.CHANNEL_ID[c]) for p in PR.PROCESSES for c in PR.process_channels(p)]
+    rng = np.random.default_rng(seed)
+    sel, pt = rng.integers(0, len(pc), n), rng.integers(0, len(pts), n)
+    q = np.zeros(n, dtype=PR.PROP_QUERY_DTYPE)
+    q["process"] = np.array([c[0] for c in pc])[sel]
+    q["channel"] = np.array([c[1] for c in pc])[sel]
+    q["k0"], q["k"] = PI_K0[rng.integers(0, PI_K0.size, n)], PI_K[rng.integers(0, PI_K.size, n)]
+    P = pts[pt]
+    q["T"], q["mu_u"], q["mu_s"], q["xi"] = P[:, 0], P[:, 1], P[:, 1], P[:, 2]
+    q["m_u"], q["m_s"], q["Phi"], q["Phib"] = P[:, 3], P[:, 4], P[:, 5], P[:, 6]
+    q["A_u"], q["A_s"] = A[0, pt], A[1, pt]
+    return q

@@ -288,3 +288,95 @@ def propagator_by_channel_batch(q, G, K, kc=None, nthreads=8, Nc=3):
         DS[i], DP[i] = propagator_by_channel(PROCESSES[int(r["process"])], "tsu"[int(r["channel"])], r["k0"], r["k"],
                                              qp, tp, c, int(r["mesons"]), pi)
     return DS, DP
+
+
+# ---- vectorised form (the CPU baseline's timing path) ----------------------------
+_SLOT_NAMES = ["uu_P", "uu_S", "K", "sigma_K", "ss_P", "ss_S"]
+
+
+def _simple_vec(K, Pi):
+    den = 1.0 - (4.0 * K) * Pi
+    fin = np.isfinite(den.real) & np.isfinite(den.imag)
+    den = np.where(fin & (np.abs(den) < DENOM_EPS), den + 1j * DENOM_EPS, den)
+    with np.errstate(all="ignore"):
+        D = (2.0 * K) / np.where(fin, den, 1.0)
+    return np.where(fin, D, 0j)
+
+
+def _mixed_vec(dK, Puu, Pss, K0, K8, K08, J0, J8, Jp0, Jp8):
+    a, b = (4.0 / 3.0) * dK, (4.0 / 3.0) * math.sqrt(2.0) * dK
+    M00 = K0 - a * (Puu + 2.0 * Pss)
+    M08 = K08 + b * (Puu - Pss)
+    M88 = K8 - a * (2.0 * Puu + Pss)
+    det = M00 * M88 - M08 * M08
+    fin = np.isfinite(det.real) & np.isfinite(det.imag)
+    det = np.where(fin & (np.abs(det) < DENOM_EPS), det + 1j * DENOM_EPS, det)
+    r = J0 * (M00 * Jp0 + M08 * Jp8) + J8 * (M08 * Jp0 + M88 * Jp8)
+    ok = fin & np.isfinite(r.real) & np.isfinite(r.imag)
+    with np.errstate(all="ignore"):
+        D = (2.0 * dK) / np.where(ok, det, 1.0) * r
+    return np.where(ok, D, 0j)
+
+
+def propagator_by_channel_vec(q, G, K, nthreads=8, Nc=3):
+    """The same (D_S, D_P) as propagator_by_channel_batch (couplings from A),
+    with the Pi queries built and the propagators composed as numpy array
+    operations: the CPU baseline's timed path.  Agreement with the scalar
+    restatement is checked in the tests."""
+    n = len(q)
+    keys = q["process"].astype(np.int64) * 256 + q["channel"].astype(np.int64) * 64 + q["mesons"].astype(np.int64)
+    combos = {}
+    for key in np.unique(keys):
+        p, ch, mk = PROCESSES[int(key // 256)], "tsu"[int(key // 64 % 4)], int(key % 64)
+        simple, mP, mS = _mesons_of(p, ch, mk)
+        T1, T2 = flavour_factors_for_channel(p, ch)
+        q4 = parse_process(p)
+        legs = {"t": ((0, 2), (1, 3)), "s": ((0, 1), (2, 3)), "u": ((0, 3), (1, 2))}[ch]
+        J = current_vector(q4[legs[0][0]], q4[legs[0][1]])
+        Jp = current_vector(q4[legs[1][0]], q4[legs[1][1]])
+        combos[int(key)] = (simple, mP, mS, T1, T2, J, Jp, needed_pi(p, ch, mk))
+    need = np.zeros((n, 6), bool)
+    for key, c in combos.items():
+        sel = keys == key
+        for nm in c[7]:
+            need[sel, _SLOT_NAMES.index(nm)] = True
+    idx, slot = np.nonzero(need)
+    m = len(idx)
+    qa = np.zeros(m, dtype=np.dtype([(f, np.int32 if t is O.C.c_int else np.float64) for f, t in O.PolQuery._fields_]))
+    u_first = np.isin(slot, (0, 1, 2, 3))            # pairs starting with u: uu and us
+    second_u = np.isin(slot, (0, 1))
+    r = q[idx]
+    qa["channel"] = np.where(np.isin(slot, (0, 2, 4)), O.CHANNEL_P, O.CHANNEL_S)
+    qa["num_s"] = np.where(slot < 2, 0, np.where(slot < 4, 1, 2))
+    qa["k0"], qa["k"] = r["k0"], r["k"]
+    qa["m1"] = np.where(u_first, r["m_u"], r["m_s"])
+    qa["m2"] = np.where(second_u, r["m_u"], r["m_s"])
+    qa["mu1"] = np.where(u_first, r["mu_u"], r["mu_s"])
+    qa["mu2"] = np.where(second_u, r["mu_u"], r["mu_s"])
+    qa["A1"] = np.where(u_first, r["A_u"], r["A_s"])
+    qa["A2"] = np.where(second_u, r["A_u"], r["A_s"])
+    qa["T"], qa["Phi"], qa["Phib"], qa["xi"] = r["T"], r["Phi"], r["Phib"], r["xi"]
+    pq = (O.PolQuery * m).from_buffer(qa)
+    re, im = O.polarization_batch(pq, nthreads=nthreads) if m else (np.zeros(0), np.zeros(0))
+    pi = np.zeros((n, 6), complex)
+    pi[idx, slot] = re + 1j * im
+    Gu = -Nc / (4.0 * (math.pi * math.pi)) * (q["m_u"] * q["A_u"])
+    Gs = -Nc / (4.0 * (math.pi * math.pi)) * (q["m_s"] * q["A_s"])
+    c = effective_couplings(G, K, Gu, Gs)
+    DS, DP = np.zeros(n, complex), np.zeros(n, complex)
+    simple_k = {"pi": ("K123_plus", 0, True), "K": ("K4567_plus", 2, True), "sigma_pi": ("K123_minus", 1, False),
+                "sigma_K": ("K4567_minus", 3, False)}
+    for key, (simple, mP, mS, T1, T2, J, Jp, _) in combos.items():
+        sel = np.nonzero(keys == key)[0]
+        for mes in simple:
+            kk, s, is_p = simple_k[mes]
+            D = T1 * _simple_vec(c[kk][sel], pi[sel, s]) * T2
+            if is_p:
+                DP[sel] += D
+            else:
+                DS[sel] += D
+        for on, sfx, s0, tgt in ((mP, "plus", 0, DP), (mS, "minus", 1, DS)):
+            if on:
+                tgt[sel] += _mixed_vec(c[f"det_K_{sfx}"][sel], pi[sel, s0], pi[sel, s0 + 4], c[f"K0_{sfx}"][sel],
+                                       c[f"K8_{sfx}"][sel], c[f"K08_{sfx}"][sel], J[0], J[1], Jp[0], Jp[1])
+    return DS, DP

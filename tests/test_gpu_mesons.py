@@ -206,3 +206,21 @@ def test_explicit_couplings_and_host_api(gpu_ctx, P, M):
         P.propagator_by_channel_batch(bad)
     with pytest.raises(ValueError):
         P.meson_propagator_simple("eta", kc1, Pi)
+
+
+def test_device_pointer_and_one_per_wave_paths(gpu_ctx, P, M):
+    """The HBM-resident call (PNJL_FLAG_DEVICE_PTRS | ASYNC on a torch stream)
+    and the one-query-per-wavefront FLOP-counting launch give the host call's
+    results bit for bit."""
+    import torch
+    q = _queries(P, M, 700, 6, all_masks=True)
+    DS, DP = P.propagator_by_channel_batch(q)
+    DS1, DP1 = P.propagator_by_channel_batch(q, one_per_wave=True)
+    assert np.array_equal(DS, DS1) and np.array_equal(DP, DP1)
+    dev = torch.device("cuda", 0)
+    qd = torch.from_numpy(q.view(np.uint8).copy()).to(dev)
+    out = torch.full((q.size, 4), np.nan, dtype=torch.float64, device=dev)
+    P.propagator_by_channel_device(qd, out)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    assert np.array_equal(o[:, 0] + 1j * o[:, 1], DS) and np.array_equal(o[:, 2] + 1j * o[:, 3], DP)
