@@ -232,9 +232,11 @@ def train_learned_seed(ctx, w, opts):
 PROP_METRIC = "meson propagator (D_S, D_P) queries/s (calculate_propagator_by_channel, 1,048,576-query batch)"
 # FP64 FLOPs per query of this batch, counted once like C3_FLOPS_PER_VALUE:
 # FP64 VALU counters on 65,536 of its queries run one (pair, query) per
-# wavefront (python3 -m julia_relaxtime_amd.prop_flop_probe).  The composition
-# kernel (~10^2 FLOPs per query) is not counted.
-PROP_FLOPS_PER_QUERY = None
+# wavefront (python3 -m julia_relaxtime_amd.prop_flop_probe): 119,772 per
+# query (1.68 quark pairs of B0 + correction per query on average).  The
+# bucketing and composition kernels (~10^2 operations per query) are not
+# counted; their time is.
+PROP_FLOPS_PER_QUERY = 119772.0
 
 
 def prop_cpu_baseline(q, G, K, threads=None):
@@ -296,7 +298,8 @@ def run_propagators(args, w):
                        nonfinite=int((~np.isfinite(o)).sum()), kernel_ms_per_step=kavg),
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": None if achieved is None else achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "polar::prop_pi_kernel + prop_compose_kernel", "kernel_ms_per_step": kavg,
+                     "kernel": "polar::prop_pi_kernel (+ bucket count / scan / scatter, compose)",
+                     "kernel_ms_per_step": kavg,
                      "flops_per_query": PROP_FLOPS_PER_QUERY},
     }
     if not args.no_cpu:

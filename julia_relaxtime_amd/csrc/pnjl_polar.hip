@@ -908,29 +908,118 @@ __device__ __host__ __forceinline__ unsigned query_mesons(const pnjl_prop_query&
     return tab[q.process].ch[q.channel];
 }
 
-// Pi of the pairs a query's mesons need: thread = (pair, query), pair-major so
-// that a wavefront shares its pair; one B0 serves both channels of a pair.
-// pib[i*12 + 2*slot] as in pnjl_polarization_grid's order.
-// stride 64 puts one (pair, query) per wavefront, for counting FLOPs.
-__global__ void __launch_bounds__(256, 3) prop_pi_kernel(Phys P, const double* tab, int64_t n,
-                                                      const pnjl_prop_query* qs, double* pib, int stride)
+// ---- work order of the propagator batch ----------------------------------------
+// The (pair, query) items the queries' mesons need are bucketed by what sets
+// the branch structure of B0 -- pair, m1, |lambda| = |k0 + mu1 - mu2| and k --
+// and evaluated bucket by bucket, so that the lanes of a wavefront take the
+// same singularity branches (a counting sort: count, scan, scatter).  Items
+// of no need are not evaluated at all.  The order inside a bucket follows the
+// atomics and varies from call to call; each item's arithmetic does not.
+constexpr int kBM = 8, kBL = 16, kBK = 16;
+constexpr int kPropBuckets = 3 * kBM * kBL * kBK;
+
+__device__ __forceinline__ int bin_of(double x, double scale, int nb)
+{
+    const double v = x * scale;
+    if (!(v > 0.0)) return 0;   // negative or NaN
+    return v >= (double)(nb - 1) ? nb - 1 : (int)v;
+}
+
+__device__ __forceinline__ unsigned item_need(const pnjl_prop_query& a, int pair)
+{
+    const unsigned mes = query_mesons(a, meson::kProcs);
+    return mes == meson::MS_NONE ? 0u : (meson::slots_for(mes) >> (2 * pair)) & 3u;
+}
+
+__device__ __forceinline__ int item_bucket(const pnjl_prop_query& a, int pair)
+{
+    const double m1 = pair == 2 ? a.m_s : a.m_u;
+    const double mu1 = pair == 2 ? a.mu_s : a.mu_u, mu2 = pair == 0 ? a.mu_u : a.mu_s;
+    const int bm = bin_of(m1, 4.0, kBM), bl = bin_of(fabs(a.k0 + mu1 - mu2), 4.0, kBL), bk = bin_of(a.k, 5.0, kBK);
+    return ((pair * kBM + bm) * kBL + bl) * kBK + bk;
+}
+
+__global__ void __launch_bounds__(256) zero_u32(unsigned* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// item t = pair * n + i: bucket counts; the Pi slots of items of no need are zeroed
+__global__ void __launch_bounds__(256) prop_count_kernel(int64_t n, const pnjl_prop_query* qs, unsigned* counts,
+                                                         double* pib)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n) return;
+    const int pair = (int)(t / n);
+    const int64_t i = t - pair * n;
+    const pnjl_prop_query a = qs[i];
+    if (item_need(a, pair)) {
+        atomicAdd(&counts[item_bucket(a, pair)], 1u);
+    } else {
+        double* o = pib + i * 12 + 4 * pair;
+        o[0] = o[1] = o[2] = o[3] = 0.0;
+    }
+}
+
+// exclusive scan of the bucket counts (one block of 256); offs[kPropBuckets] = total
+__global__ void __launch_bounds__(256) prop_scan_kernel(const unsigned* counts, unsigned* offs)
+{
+    constexpr int per = kPropBuckets / 256;
+    static_assert(kPropBuckets % 256 == 0, "bucket count");
+    __shared__ unsigned part[256];
+    const int b0 = threadIdx.x * per;
+    unsigned s = 0;
+    for (int j = 0; j < per; j++) s += counts[b0 + j];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {   // Hillis-Steele inclusive scan
+        const unsigned v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned run = part[threadIdx.x] - s;
+    for (int j = 0; j < per; j++) {
+        offs[b0 + j] = run;
+        run += counts[b0 + j];
+    }
+    if (threadIdx.x == 255) offs[kPropBuckets] = part[255];
+}
+
+__global__ void __launch_bounds__(256) prop_scatter_kernel(int64_t n, const pnjl_prop_query* qs, const unsigned* offs,
+                                                           unsigned* cursor, unsigned* items)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n) return;
+    const int pair = (int)(t / n);
+    const int64_t i = t - pair * n;
+    const pnjl_prop_query a = qs[i];
+    if (!item_need(a, pair)) return;
+    const int b = item_bucket(a, pair);
+    items[offs[b] + atomicAdd(&cursor[b], 1u)] = (unsigned)(i * 3 + pair);
+}
+
+// Pi of one worklist item (query i, pair): one B0 serves both channels of the
+// pair.  pib[i*12 + 2*slot] as in pnjl_polarization_grid's order.  Threads
+// past the worklist's length (offs[kPropBuckets]) exit; stride 64 puts one
+// item per wavefront, for counting FLOPs.
+__global__ void __launch_bounds__(256, 3) prop_pi_kernel(Phys P, const double* tab, const pnjl_prop_query* qs,
+                                                      const unsigned* items, const unsigned* offs, double* pib,
+                                                      int stride)
 {
     __shared__ double lds[kTabLen];
     load_tabs(lds, tab);
     const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ts % stride) return;
     const int64_t t = ts / stride;
-    if (t >= 3 * n) return;
-    const int pair = (int)(t / n);
-    const int64_t i = t - pair * n;
+    if (t >= (int64_t)offs[kPropBuckets]) return;
+    const unsigned code = items[t];
+    const int pair = (int)(code % 3u);
+    const int64_t i = code / 3u;
     const pnjl_prop_query a = qs[i];
-    const unsigned mes = query_mesons(a, meson::kProcs);
-    const unsigned need = mes == meson::MS_NONE ? 0u : (meson::slots_for(mes) >> (2 * pair)) & 3u;
+    const unsigned need = item_need(a, pair);
     double* o = pib + i * 12 + 4 * pair;
-    if (!need) {
-        o[0] = o[1] = o[2] = o[3] = 0.0;
-        return;
-    }
     const Tabs q = make_tabs(lds);
     const double m1 = pair == 2 ? a.m_s : a.m_u, m2 = pair == 0 ? a.m_u : a.m_s;
     const double mu1 = pair == 2 ? a.mu_s : a.mu_u, mu2 = pair == 0 ? a.mu_u : a.mu_s;
@@ -1317,8 +1406,14 @@ int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_q
                                "pnjl_propagator_by_channel_batch: process, channel or meson mask not in the table");
     double* tab = nullptr;
     if (int rc = polar_tab(ctx, &tab)) return rc;
-    void* pib;
+    void *pib, *ints, *items;
     if (int rc = stage_buf(ctx, 83, (size_t)n * 12 * sizeof(double), &pib)) return rc;
+    const int nb = polar::kPropBuckets;
+    if (int rc = stage_buf(ctx, 84, (size_t)(3 * nb + 1) * sizeof(unsigned), &ints)) return rc;
+    if (int rc = stage_buf(ctx, 85, (size_t)3 * n * sizeof(unsigned), &items)) return rc;
+    unsigned* counts = (unsigned*)ints;   // counts[nb] | cursor[nb] | offs[nb + 1]
+    unsigned* cursor = counts + nb;
+    unsigned* offs = cursor + nb;
     const pnjl_prop_query* dq = q;
     const double* dkc = kc;
     double* dout = out;
@@ -1338,8 +1433,14 @@ int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_q
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
     const int stride = (flags & PNJL_FLAG_ONE_PER_WAVE) ? 64 : 1;
-    hipLaunchKernelGGL(polar::prop_pi_kernel, dim3(blocks_for(3 * n * stride)), dim3(256), 0, st, phys(ctx), tab, n,
-                       dq, (double*)pib, stride);
+    hipLaunchKernelGGL(polar::zero_u32, dim3(blocks_for(2 * nb)), dim3(256), 0, st, counts, 2 * nb);
+    hipLaunchKernelGGL(polar::prop_count_kernel, dim3(blocks_for(3 * n)), dim3(256), 0, st, n, dq, counts,
+                       (double*)pib);
+    hipLaunchKernelGGL(polar::prop_scan_kernel, dim3(1), dim3(256), 0, st, (const unsigned*)counts, offs);
+    hipLaunchKernelGGL(polar::prop_scatter_kernel, dim3(blocks_for(3 * n)), dim3(256), 0, st, n, dq,
+                       (const unsigned*)offs, cursor, (unsigned*)items);
+    hipLaunchKernelGGL(polar::prop_pi_kernel, dim3(blocks_for(3 * n * stride)), dim3(256), 0, st, phys(ctx), tab, dq,
+                       (const unsigned*)items, (const unsigned*)offs, (double*)pib, stride);
     HIPCHK(ctx, hipGetLastError());
     hipLaunchKernelGGL(polar::prop_compose_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, dq, dkc, ctx->params.G,
                        ctx->params.K, ctx->params.N_color, (const double*)pib, dout);
