@@ -11,6 +11,8 @@ is the binding a maintainer adds to the reference and is not executed here
 Replaces:
   PNJL.solve(::FixedMu, T_fm, mu_fm; ...)          src/pnjl/solver/ImplicitSolver.jl:211-328
   polarization_aniso(channel, k0, k_norm, ...)      src/relaxtime/PolarizationAniso.jl:22-24
+  ThermoDerivatives (mass / thermo / bulk-viscosity derivatives)  ThermoDerivatives.jl:132-466
+  calculate_propagator_by_channel(process, channel, ...)  src/relaxtime/TotalPropagator.jl:894-963
 """
 module PNJLHip
 
@@ -127,6 +129,58 @@ function thermo_derivatives_batch(T_fm::AbstractVector{Float64}, mu_fm::Abstract
         check(ctx, rc, "pnjl_thermo_derivatives_batch")
     end
     return out
+end
+
+# scattering processes in SCATTERING_MESON_MAP's order (PNJL_PROC_* ids 0..16)
+const PROCESSES = (:ud_to_ud, :uu_to_uu, :us_to_us, :ss_to_ss, :ubardbar_to_ubardbar, :ubarubar_to_ubarubar,
+                   :ubarsbar_to_ubarsbar, :sbarsbar_to_sbarsbar, :udbar_to_udbar, :dubar_to_dubar,
+                   :uubar_to_uubar, :uubar_to_ddbar, :usbar_to_usbar, :subar_to_subar, :uubar_to_ssbar,
+                   :ssbar_to_uubar, :ssbar_to_ssbar)
+const MANDELSTAM = Dict(:t => 0, :s => 1, :u => 2)
+
+struct PropQuery
+    process::Cint; channel::Cint; mesons::Cint; reserved::Cint
+    k0::Float64; k::Float64; T::Float64; Phi::Float64; Phib::Float64; xi::Float64
+    m_u::Float64; m_s::Float64; mu_u::Float64; mu_s::Float64; A_u::Float64; A_s::Float64
+end
+
+"""
+    propagator_by_channel_batch(procs, channels, k0s, ks, quark_params, thermo_params; K_coeffs=nothing)
+
+(D_S, D_P) vectors for every (process, channel, k0, k): one call evaluates
+every Pi the exchanged mesons need and composes the propagators on the GPU.
+quark_params / thermo_params as in the reference (m, μ, A of u and s; T, Φ,
+Φbar, ξ).  K_coeffs = nothing derives the couplings from quark_params.A.
+Replaces calculate_propagator_by_channel  TotalPropagator.jl:894-963.
+"""
+function propagator_by_channel_batch(procs::AbstractVector{Symbol}, channels::AbstractVector{Symbol},
+                                     k0s::AbstractVector{Float64}, ks::AbstractVector{Float64},
+                                     quark_params, thermo_params; K_coeffs::Union{Nothing,Matrix{Float64}}=nothing,
+                                     ctx::Context=default_context())
+    n = length(procs)
+    qp, tp = quark_params, thermo_params
+    q = [PropQuery(findfirst(==(procs[i]), PROCESSES) - 1, MANDELSTAM[channels[i]], 0, 0, k0s[i], ks[i],
+                   tp.T, tp.Φ, tp.Φbar, tp.ξ, qp.m.u, qp.m.s, qp.μ.u, qp.μ.s, qp.A.u, qp.A.s) for i in 1:n]
+    out = zeros(4, n)
+    kc = K_coeffs === nothing ? Ptr{Float64}(C_NULL) : pointer(K_coeffs)   # 12 x n, column-major = row-major n x 12
+    GC.@preserve q out K_coeffs begin
+        rc = ccall((:pnjl_propagator_by_channel_batch, LIB), Cint,
+                   (Ptr{Cvoid}, Int64, Ptr{PropQuery}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+                   ctx.h, n, q, kc, out, 0, C_NULL)
+        check(ctx, rc, "pnjl_propagator_by_channel_batch")
+    end
+    return complex.(out[1, :], out[2, :]), complex.(out[3, :], out[4, :])
+end
+
+"""
+    calculate_propagator_by_channel(process, channel, k0, k_norm, quark_params, thermo_params)
+
+Same arguments and (D_S, D_P) result as the reference (K_coeffs from quark_params.A).
+"""
+function calculate_propagator_by_channel(process::Symbol, channel::Symbol, k0::Float64, k_norm::Float64,
+                                         quark_params, thermo_params; ctx::Context=default_context())
+    DS, DP = propagator_by_channel_batch([process], [channel], [k0], [k_norm], quark_params, thermo_params; ctx=ctx)
+    return DS[1], DP[1]
 end
 
 """
