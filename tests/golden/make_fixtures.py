@@ -16,6 +16,8 @@ Sources (paths relative to the reference root):
       spinodals and the Maxwell-construction transition line, xi = 0, 0.2, 0.4)
   data/outputs/results/pnjl/bulk_viscosity_xi0.0.csv   (calculate_derivatives.jl, ThermoDerivatives
       bulk_viscosity_coefficients, p24/t8, 7 digits)
+  data/outputs/results/pnjl/derivatives_xi0.0.csv      (calculate_derivatives.jl, ThermoDerivatives
+      thermo_derivatives + mass_derivatives, p24/t8, 7 digits)
 """
 import csv
 import os
@@ -70,6 +72,8 @@ def main(ref):
           rows_of(os.path.join(ref, "data/reference/pnjl/boundary.csv")))
     bulk = os.path.join(ref, "data/outputs/results/pnjl/bulk_viscosity_xi0.0.csv")
     write("bulk_viscosity_xi0.0.csv", list(rows_of(bulk)[0].keys()), rows_of(bulk))
+    der = os.path.join(ref, "data/outputs/results/pnjl/derivatives_xi0.0.csv")
+    write("derivatives_xi0.0.csv", list(rows_of(der)[0].keys()), rows_of(der))
 
 
 if __name__ == "__main__":

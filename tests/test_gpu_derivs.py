@@ -117,3 +117,15 @@ def test_host_api_is_consistent(gpu_ctx):
         TD.mass_derivatives(T, mu, order=2)
     with pytest.raises(NotImplementedError):
         TD.dP_dT(T, mu, order=3)
+
+
+def test_derivatives_match_reference_output(gpu_ctx):
+    """derivatives_xi0.0.csv rows at T = 150 MeV through the host mirror of
+    ThermoDerivatives.jl (solve with DefaultSeed, then the GPU derivative pass)."""
+    from julia_relaxtime_amd import thermo_derivatives as TD
+    from test_oracle_derivs import check_derivative_row
+    rows = [r for r in golden_rows("derivatives_xi0.0.csv") if float(r["T_MeV"]) == 150.0]
+    for row in rows:
+        d = TD.thermo_derivatives(float(row["T_MeV"]) / HB, float(row["mu_MeV"]) / HB, xi=0.0, p_num=24, t_num=8)
+        assert d["converged"]
+        check_derivative_row(d, row)

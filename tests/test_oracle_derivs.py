@@ -82,3 +82,38 @@ def test_bulk_viscosity_coefficients_match_reference_output(oracle):
                 assert rel(d["dM_dmu"][i] / 3, ref(f"dM_{f}_dmuB"))
         else:
             assert abs(d["rho"]) < 1e-12 and np.all(np.abs(d["dM_dmu"]) < 1e-12)
+
+
+DERIV_COLS = [("dM_u_dT", "dM_dT", 0), ("dM_s_dT", "dM_dT", 2), ("dM_u_dmu", "dM_dmu", 0), ("dM_s_dmu", "dM_dmu", 2),
+              ("dP_dT", "dP_dT", None), ("dP_dmu", "dP_dmu", None), ("dEpsilon_dT", "dEpsilon_dT", None),
+              ("dEpsilon_dmu", "dEpsilon_dmu", None), ("dn_dT", "dn_dT", None), ("dn_dmu", "dn_dmu", None),
+              ("pressure_fm4", "pressure", None), ("energy_fm4", "energy", None), ("entropy_fm3", "entropy", None)]
+
+
+def check_derivative_row(d, row):
+    """One row of the reference's derivatives_xi0.0.csv (thermo_derivatives +
+    mass_derivatives, 7 significant digits; masses and rho_norm with 6
+    decimals).  Columns that vanish at mu = 0 hold rounding noise there and
+    are checked for |value| < 1e-12 on both sides."""
+    HBm = 197.327
+    for i, f in ((0, "u"), (2, "s")):
+        assert abs(d["masses"][i] * HBm - float(row[f"M_{f}_MeV"])) < 2e-6
+    assert abs(d["rho_norm"] - float(row["rho_norm"])) < 6e-7
+    for col, key, idx in DERIV_COLS:
+        ref = float(row[col])
+        got = d[key] if idx is None else d[key][idx]
+        if abs(ref) < 1e-12:
+            assert abs(got) < 1e-12, col
+        else:
+            assert abs(got - ref) <= 6e-7 * abs(ref), (col, got, ref)
+
+
+def test_derivatives_match_reference_output(oracle):
+    """derivatives_xi0.0.csv rows at T = 150 MeV (p24/t8); its T = 160 MeV rows
+    sit on the same negative-mass root as the bulk-viscosity file's."""
+    rows = [r for r in golden_rows("derivatives_xi0.0.csv") if float(r["T_MeV"]) == 150.0]
+    assert len(rows) == 2
+    for row in rows:
+        T, mu = float(row["T_MeV"]) / HB, float(row["mu_MeV"]) / HB
+        r = _solve(oracle, T, mu, 0.0)
+        check_derivative_row(oracle.thermo_derivs(r["x"][0], T, mu, 0.0, 24, 8), row)
