@@ -487,6 +487,45 @@ typedef struct pnjl_prop_query {
 int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_query* q, const double* kc,
                                      double* out, int flags, void* stream);
 
+/* Meson pole masses: the mesons of MesonMass.jl (pi, K, sigma_pi, sigma_K
+ * and the mixed eta / eta', sigma / sigma')                                 */
+enum {
+    PNJL_MESON_ID_PI = 0, PNJL_MESON_ID_K, PNJL_MESON_ID_SIGMA_PI, PNJL_MESON_ID_SIGMA_K, PNJL_MESON_ID_ETA,
+    PNJL_MESON_ID_ETA_PRIME, PNJL_MESON_ID_SIGMA, PNJL_MESON_ID_SIGMA_PRIME,
+    PNJL_MESON_ID_COUNT
+};
+
+/* One pole search: solve_meson_mass(meson, quark_params, thermo_params;
+ * K_coeffs, k_norm, initial_mass, initial_gamma, nlsolve_kwargs)
+ * (src/relaxtime/MesonMass.jl:241-276).  mass0 = NaN takes
+ * default_meson_mass_guess (:183-204).  NLsolve's defaults are ftol 1e-8,
+ * xtol 0, max_iter 1000.                                                   */
+typedef struct pnjl_mass_query {
+    int    meson;                 /* PNJL_MESON_ID_*                         */
+    int    max_iter;              /* NLsolve iterations                      */
+    int    reserved[2];
+    double k;                     /* k_norm, fm^-1                           */
+    double mass0, gamma0;         /* initial (mass, gamma), fm^-1            */
+    double ftol, xtol;
+    double T, Phi, Phib, xi;
+    double m_u, m_s, mu_u, mu_s, A_u, A_s;
+} pnjl_mass_query;
+
+/* Pole masses for n queries -> out[n*6] = (mass, gamma, residual_norm,
+ * converged, iterations, exception) with residual_norm the inf-norm of
+ * (Re f, Im f) and exception = 1 where the residual at the initial point is
+ * not finite (the reference's NLsolve throws there).  One GPU thread runs
+ * one query's trust-region iteration (dogleg, autoscaling, central
+ * finite-difference Jacobian, as NLsolve's defaults); every residual is
+ * 1 - 4 K Pi(k0 + i gamma/2) with Pi from polarization_with_width, or the
+ * light / heavy eigenvalue of 2M for the mixed mesons.  kc = NULL derives
+ * the couplings from A as _ensure_K_coeffs does.  Host pointers unless
+ * PNJL_FLAG_DEVICE_PTRS.
+ * replaces: solve_meson_mass, meson_mass_equation, default_meson_mass_guess
+ *           src/relaxtime/MesonMass.jl:183-276                             */
+int pnjl_meson_mass_batch(pnjl_ctx* ctx, int64_t n, const pnjl_mass_query* q, const double* kc, double* out,
+                          int flags, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1083,6 +1083,209 @@ __global__ void __launch_bounds__(256) prop_compose_kernel(int64_t n, const pnjl
     store_c(o + 2, DP);
 }
 
+// ---- meson pole masses (MesonMass.jl) -------------------------------------------------
+constexpr double kFdStep = 6.055454452393343e-06;   // cbrt(eps): FiniteDiff's central relstep = absstep
+
+// principal square root
+__device__ __forceinline__ meson::C csqrt_p(meson::C z)
+{
+    if (z.r == 0.0 && z.i == 0.0) return meson::C{0.0, z.i};
+    const double t = sqrt((fabs(z.r) + hypot(z.r, z.i)) * 0.5);
+    if (z.r >= 0.0) return meson::C{t, z.i / (2.0 * t)};
+    return meson::C{fabs(z.i) / (2.0 * t), copysign(t, z.i)};
+}
+
+// polarization_with_width of one quark pair (uu 0, us 1, ss 2)  MesonMass.jl:34-60, :222-231
+__device__ void mass_pi(const Phys& P, const Tabs& T, const pnjl_mass_query& a, int pair, int channel, double k0,
+                        double gamma, meson::C& out)
+{
+    const double m1 = pair == 2 ? a.m_s : a.m_u, m2 = pair == 0 ? a.m_u : a.m_s;
+    const double mu1 = pair == 2 ? a.mu_s : a.mu_u, mu2 = pair == 0 ? a.mu_u : a.mu_s;
+    const double A1 = pair == 2 ? a.A_s : a.A_u, A2 = pair == 0 ? a.A_u : a.A_s;
+    double br, bi;
+    b0_total(P, T, k0, a.k, m1, m2, mu1, mu2, a.T, a.Phi, a.Phib, a.xi, pair == 1 ? 1 : 0, br, bi);
+    pi_value(P, channel, 1, k0, gamma, a.k, m1, m2, k0 + mu1 - mu2, A1 + A2, br, bi, out.r, out.i);
+}
+
+// meson_mass_equation  MesonMass.jl:211-233: 1 - 4 K Pi, or for the mixed
+// mesons the light / heavy eigenvalue of 2M, (M00 + M88) -/+ sqrt((M00 - M88)^2 + 4 M08^2)
+__device__ __noinline__ void mass_residual(const Phys& P, const Tabs& T, const pnjl_mass_query& a,
+                                           const meson::KCoef& k, double k0, double gamma, double* f)
+{
+    using meson::C;
+    const int m = a.meson;
+    C res;
+    if (m < PNJL_MESON_ID_ETA) {
+        const int pair = (m == PNJL_MESON_ID_K || m == PNJL_MESON_ID_SIGMA_K) ? 1 : 0;
+        const int ch = (m == PNJL_MESON_ID_PI || m == PNJL_MESON_ID_K) ? PNJL_CHANNEL_P : PNJL_CHANNEL_S;
+        const double K = m == PNJL_MESON_ID_PI ? k.K123p
+                       : m == PNJL_MESON_ID_K  ? k.K4567p
+                       : m == PNJL_MESON_ID_SIGMA_PI ? k.K123m : k.K4567m;
+        C Pi;
+        mass_pi(P, T, a, pair, ch, k0, gamma, Pi);
+        const double k4 = 4.0 * K;
+        res = C{1.0 - k4 * Pi.r, -(k4 * Pi.i)};
+    } else {
+        const bool pch = m == PNJL_MESON_ID_ETA || m == PNJL_MESON_ID_ETA_PRIME;
+        const int ch = pch ? PNJL_CHANNEL_P : PNJL_CHANNEL_S;
+        C Puu, Pss, M00, M08, M88;
+        mass_pi(P, T, a, 0, ch, k0, gamma, Puu);
+        mass_pi(P, T, a, 2, ch, k0, gamma, Pss);
+        if (pch)
+            meson::mixing(Puu, Pss, k.K0p, k.K8p, k.K08p, k.detKp, M00, M08, M88);
+        else
+            meson::mixing(Puu, Pss, k.K0m, k.K8m, k.K08m, k.detKm, M00, M08, M88);
+        const C dm = meson::csub(M00, M88);
+        const C sq = csqrt_p(meson::cadd(meson::cmul(dm, dm), meson::rmul(4.0, meson::cmul(M08, M08))));
+        const C tr = meson::cadd(M00, M88);
+        res = (m == PNJL_MESON_ID_ETA || m == PNJL_MESON_ID_SIGMA) ? meson::csub(tr, sq) : meson::cadd(tr, sq);
+    }
+    f[0] = res.r;
+    f[1] = res.i;
+}
+
+// central finite differences (FiniteDiff): J[:, j] = (f(x + e_j h) - f(x - e_j h)) / 2h
+__device__ void mass_jac(const Phys& P, const Tabs& T, const pnjl_mass_query& a, const meson::KCoef& k,
+                         const double* x, double J[2][2])
+{
+    for (int j = 0; j < 2; j++) {
+        const double h = fmax(kFdStep * fabs(x[j]), kFdStep);
+        double xp[2] = {x[0], x[1]}, xm[2] = {x[0], x[1]}, fp[2], fm[2];
+        xp[j] = x[j] + h;
+        xm[j] = x[j] - h;
+        mass_residual(P, T, a, k, xp[0], xp[1], fp);
+        mass_residual(P, T, a, k, xm[0], xm[1], fm);
+        for (int i = 0; i < 2; i++) J[i][j] = (fp[i] - fm[i]) / (2.0 * h);
+    }
+}
+
+__device__ __forceinline__ double wnorm2(const double* d, const double* v)
+{
+    return sqrt((d[0] * v[0]) * (d[0] * v[0]) + (d[1] * v[1]) * (d[1] * v[1]));
+}
+
+// NLsolve dogleg! for 2 x 2 (a singular Jacobian takes the scaled steepest-descent branch)
+__device__ void dogleg2(double* p, const double* r, const double* d, const double J[2][2], double delta)
+{
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    double pi_[2];
+    if (det != 0.0 && isfinite(det)) {
+        pi_[0] = -((J[1][1] * r[0] - J[0][1] * r[1]) / det);
+        pi_[1] = -((-J[1][0] * r[0] + J[0][0] * r[1]) / det);
+    } else {
+        pi_[0] = pi_[1] = __builtin_inf();
+    }
+    if (wnorm2(d, pi_) <= delta) {
+        p[0] = pi_[0], p[1] = pi_[1];
+        return;
+    }
+    double g[2], Jg[2];
+    for (int j = 0; j < 2; j++) g[j] = (J[0][j] * r[0] + J[1][j] * r[1]) / (d[j] * d[j]);
+    for (int i = 0; i < 2; i++) Jg[i] = J[i][0] * g[0] + J[i][1] * g[1];
+    const double wg = wnorm2(d, g);
+    const double coef = -(wg * wg) / (Jg[0] * Jg[0] + Jg[1] * Jg[1]);
+    const double pc[2] = {coef * g[0], coef * g[1]};
+    if (wnorm2(d, pc) >= delta) {
+        const double sc = -delta / wg;
+        p[0] = g[0] * sc, p[1] = g[1] * sc;
+        return;
+    }
+    const double pd[2] = {pi_[0] - pc[0], pi_[1] - pc[1]};
+    const double b = 2.0 * ((d[0] * pc[0]) * (d[0] * pd[0]) + (d[1] * pc[1]) * (d[1] * pd[1]));
+    const double wa = wnorm2(d, pd);
+    const double aa = wa * wa;
+    const double wpc = wnorm2(d, pc);
+    const double tau = (-b + sqrt(b * b - 4.0 * aa * (wpc * wpc - delta * delta))) / (2.0 * aa);
+    p[0] = pc[0] + tau * pd[0], p[1] = pc[1] + tau * pd[1];
+}
+
+// default_meson_mass_guess  MesonMass.jl:183-204 (d = u)
+__device__ __forceinline__ double mass_guess(const pnjl_mass_query& a)
+{
+    switch (a.meson) {
+        case PNJL_MESON_ID_PI: return a.m_u + a.m_u;
+        case PNJL_MESON_ID_K:
+        case PNJL_MESON_ID_SIGMA_K: return a.m_u + a.m_s;
+        case PNJL_MESON_ID_ETA_PRIME:
+        case PNJL_MESON_ID_SIGMA_PRIME: return 2.0 * a.m_s;
+        default: return 2.0 * a.m_u;
+    }
+}
+
+// solve_meson_mass  MesonMass.jl:241-276 with NLsolve's trust region (dogleg,
+// autoscale, factor 1) and a central-difference Jacobian; one thread per query.
+// out[i*6] = (mass, gamma, residual_norm (inf), converged, iterations, exception)
+__global__ void __launch_bounds__(256) mass_kernel(Phys P, const double* tab, int64_t n,
+                                                   const pnjl_mass_query* qs, const double* kc, double G, double K,
+                                                   int Nc, double* out)
+{
+    __shared__ double lds[kTabLen];
+    load_tabs(lds, tab);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Tabs T = make_tabs(lds);
+    const pnjl_mass_query a = qs[i];
+    double* o = out + i * 6;
+    if (a.meson < 0 || a.meson >= PNJL_MESON_ID_COUNT || a.max_iter < 0) {
+        for (int j = 0; j < 6; j++) o[j] = __builtin_nan("");
+        return;
+    }
+    const meson::KCoef k = kc ? load_kc(kc + i * 12)
+                              : meson::couplings(G, K, meson::G_from_A(a.A_u, a.m_u, Nc),
+                                                 meson::G_from_A(a.A_s, a.m_s, Nc));
+    double x[2] = {isnan(a.mass0) ? mass_guess(a) : a.mass0, a.gamma0};
+    double r[2], J[2][2];
+    mass_residual(P, T, a, k, x[0], x[1], r);
+    if (!isfinite(r[0]) || !isfinite(r[1])) {   // NLsolve's IsFiniteException at the start
+        o[0] = x[0], o[1] = x[1], o[2] = __builtin_nan(""), o[3] = 0.0, o[4] = 0.0, o[5] = 1.0;
+        return;
+    }
+    mass_jac(P, T, a, k, x, J);
+    int it = 0;
+    bool conv = fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
+    bool stopped = isnan(x[0]) || isnan(x[1]) || isnan(r[0]) || isnan(r[1]);
+    if (!conv) {
+        double d[2];
+        for (int j = 0; j < 2; j++) {
+            d[j] = sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]);
+            if (d[j] == 0.0) d[j] = 1.0;
+        }
+        double delta = 1.0 * wnorm2(d, x);
+        if (delta == 0.0) delta = 1.0;
+        while (!stopped && !conv && it < a.max_iter) {
+            it++;
+            double p[2], fn[2];
+            dogleg2(p, r, d, J, delta);
+            const double xold[2] = {x[0], x[1]};
+            x[0] += p[0], x[1] += p[1];
+            mass_residual(P, T, a, k, x[0], x[1], fn);
+            double rp[2];
+            for (int q = 0; q < 2; q++) rp[q] = J[q][0] * p[0] + J[q][1] * p[1] + r[q];
+            const double ssr = r[0] * r[0] + r[1] * r[1];
+            const double rho = (ssr - (fn[0] * fn[0] + fn[1] * fn[1])) / (ssr - (rp[0] * rp[0] + rp[1] * rp[1]));
+            if (rho > 1e-4) {
+                r[0] = fn[0], r[1] = fn[1];
+                mass_jac(P, T, a, k, x, J);
+                for (int j = 0; j < 2; j++) d[j] = fmax(0.1 * d[j], sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]));
+                const bool xc = fmax(fabs(x[0] - xold[0]), fabs(x[1] - xold[1])) < a.xtol;
+                conv = xc || fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
+            } else {
+                x[0] -= p[0], x[1] -= p[1];
+                conv = false;
+            }
+            if (rho < 0.1)
+                delta = delta / 2.0;
+            else if (rho >= 0.9)
+                delta = 2.0 * wnorm2(d, p);
+            else if (rho >= 0.5)
+                delta = fmax(delta, 2.0 * wnorm2(d, p));
+            stopped = isnan(x[0]) || isnan(x[1]) || isnan(fn[0]) || isnan(fn[1]);
+        }
+    }
+    o[0] = x[0], o[1] = x[1], o[2] = fmax(fabs(r[0]), fabs(r[1])), o[3] = conv ? 1.0 : 0.0, o[4] = (double)it;
+    o[5] = 0.0;
+}
+
 }  // namespace polar
 
 // =============================================================================
@@ -1469,6 +1672,53 @@ int pnjl_propagator_by_channel_batch(pnjl_ctx* ctx, int64_t n, const pnjl_prop_q
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
     if (!dev) HIPCHK(ctx, hipMemcpyAsync(out, dout, (size_t)n * 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (!(flags & PNJL_FLAG_ASYNC) || !dev) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    }
+    return PNJL_OK;
+}
+
+int pnjl_meson_mass_batch(pnjl_ctx* ctx, int64_t n, const pnjl_mass_query* q, const double* kc, double* out,
+                          int flags, void* stream)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (n < 0 || (n > 0 && (!q || !out)) || n > ((int64_t)1 << 28))
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_meson_mass_batch: bad argument");
+    if (n == 0) return PNJL_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & PNJL_FLAG_DEVICE_PTRS;
+    if (!dev)
+        for (int64_t i = 0; i < n; i++)
+            if (q[i].meson < 0 || q[i].meson >= PNJL_MESON_ID_COUNT || q[i].max_iter < 0)
+                return set_err(ctx, PNJL_ERR_ARG, "pnjl_meson_mass_batch: unknown meson or negative max_iter");
+    double* tab = nullptr;
+    if (int rc = polar_tab(ctx, &tab)) return rc;
+    const pnjl_mass_query* dq = q;
+    const double* dkc = kc;
+    double* dout = out;
+    if (!dev) {
+        void *a, *b, *c = nullptr;
+        if (int rc = stage_buf(ctx, 86, (size_t)n * sizeof(pnjl_mass_query), &a)) return rc;
+        if (int rc = stage_buf(ctx, 87, (size_t)n * 6 * sizeof(double), &b)) return rc;
+        if (kc)
+            if (int rc = stage_buf(ctx, 88, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), &c)) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(a, q, (size_t)n * sizeof(pnjl_mass_query), hipMemcpyHostToDevice, st));
+        if (kc)
+            HIPCHK(ctx, hipMemcpyAsync(c, kc, (size_t)n * PNJL_KCOEF_FIELDS * sizeof(double), hipMemcpyHostToDevice,
+                                       st));
+        dq = (const pnjl_mass_query*)a;
+        dkc = (const double*)c;
+        dout = (double*)b;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(polar::mass_kernel, dim3(blocks_for(n)), dim3(256), 0, st, phys(ctx), tab, n, dq, dkc,
+                       ctx->params.G, ctx->params.K, ctx->params.N_color, dout);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
+    if (!dev) HIPCHK(ctx, hipMemcpyAsync(out, dout, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost, st));
     if (!(flags & PNJL_FLAG_ASYNC) || !dev) {
         HIPCHK(ctx, hipStreamSynchronize(st));
         float ms = 0.f;

@@ -325,3 +325,108 @@ __all__ = ["PROCESSES", "PROP_QUERY_DTYPE", "KC_FIELDS", "PI_SLOTS", "couplings_
            "propagator_by_channel_batch", "propagator_by_channel_device", "process_channels",
            "calculate_propagator_by_channel", "total_propagator_simple",
            "total_propagator_mixed"]
+
+
+# ---- meson pole masses (MesonMass.jl / MottTransition.jl) ---------------------------------
+MASS_MESONS = ["pi", "K", "sigma_pi", "sigma_K", "eta", "eta_prime", "sigma", "sigma_prime"]   # PNJL_MESON_ID_*
+MASS_QUERY_DTYPE = np.dtype([("meson", np.int32), ("max_iter", np.int32), ("reserved", np.int32, 2),
+                             ("k", np.float64), ("mass0", np.float64), ("gamma0", np.float64), ("ftol", np.float64),
+                             ("xtol", np.float64), ("T", np.float64), ("Phi", np.float64), ("Phib", np.float64),
+                             ("xi", np.float64), ("m_u", np.float64), ("m_s", np.float64), ("mu_u", np.float64),
+                             ("mu_s", np.float64), ("A_u", np.float64), ("A_s", np.float64)])
+assert MASS_QUERY_DTYPE.itemsize == 16 + 15 * 8
+
+
+@_lib.register_extra
+def _register_mass(L):
+    vp = C.c_void_p
+    L.pnjl_meson_mass_batch.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_int, vp]
+    L.pnjl_meson_mass_batch.restype = C.c_int
+
+
+def make_mass_queries(meson, T, Phi, Phib, xi, m_u, m_s, mu_u, mu_s, A_u, A_s, k=0.0, mass0=np.nan, gamma0=0.0,
+                      ftol=1e-8, xtol=0.0, max_iter=1000):
+    """MASS_QUERY_DTYPE array from broadcastable columns (meson by name or id)."""
+    cols = dict(T=T, Phi=Phi, Phib=Phib, xi=xi, m_u=m_u, m_s=m_s, mu_u=mu_u, mu_s=mu_s, A_u=A_u, A_s=A_s, k=k,
+                mass0=mass0, gamma0=gamma0, ftol=ftol, xtol=xtol)
+    n = int(max([np.size(v) for v in cols.values()] + [np.size(meson), np.size(max_iter)]))
+    q = np.zeros(n, dtype=MASS_QUERY_DTYPE)
+    ms = np.broadcast_to(np.asarray(meson, dtype=object), (n,))
+    q["meson"] = [MASS_MESONS.index(str(m).lstrip(":")) if not isinstance(m, (int, np.integer)) else int(m)
+                  for m in ms]
+    q["max_iter"] = np.broadcast_to(np.asarray(max_iter), (n,))
+    for f, v in cols.items():
+        q[f] = np.broadcast_to(np.asarray(v, dtype=np.float64), (n,))
+    return q
+
+
+def meson_mass_batch(q, K_coeffs=None, device=0):
+    """Pole masses for a MASS_QUERY_DTYPE array -> dict of arrays (mass, gamma,
+    residual_norm, converged, iterations, exception)."""
+    ctx = _ctx(device)
+    q = np.ascontiguousarray(q, dtype=MASS_QUERY_DTYPE)
+    n = q.size
+    kc = None if K_coeffs is None else _kc_rows(K_coeffs, n)
+    out = np.zeros((n, 6))
+    rc = _lib.load().pnjl_meson_mass_batch(ctx.h, n, _ptr(q), None if kc is None else _ptr(kc), _ptr(out), 0, None)
+    ctx._check(rc, "pnjl_meson_mass_batch")
+    return dict(mass=out[:, 0].copy(), gamma=out[:, 1].copy(), residual_norm=out[:, 2].copy(),
+                converged=out[:, 3] == 1.0, iterations=out[:, 4].astype(int), exception=out[:, 5] == 1.0)
+
+
+def default_meson_mass_guess(meson, quark_params):
+    """MesonMass.jl:183-204."""
+    m = quark_params["m"]
+    md = m.get("d", m["u"])
+    g = {"pi": m["u"] + md, "K": m["u"] + m["s"], "sigma_pi": 2.0 * m["u"], "sigma_K": m["u"] + m["s"],
+         "eta": 2.0 * m["u"], "eta_prime": 2.0 * m["s"], "sigma": 2.0 * m["u"], "sigma_prime": 2.0 * m["s"]}
+    name = str(meson).lstrip(":")
+    if name not in g:
+        raise ValueError(f"Unknown meson type: {meson}")
+    return g[name]
+
+
+def solve_meson_mass(meson, quark_params, thermo_params, K_coeffs=None, k_norm=0.0, initial_mass=None,
+                     initial_gamma=0.0, ftol=1e-8, xtol=0.0, iterations=1000, device=0):
+    """MesonMass.jl:241-276 -> dict(mass, gamma, converged, residual_norm,
+    iterations).  quark_params must carry A (as TotalPropagator's do); a
+    non-finite residual at the initial point raises, as NLsolve does."""
+    name = str(meson).lstrip(":")
+    if name not in MASS_MESONS:
+        raise ValueError(f"Unknown meson type: {meson}")
+    qp, tp = quark_params, thermo_params
+    q = make_mass_queries(name, tp["T"], tp["Phi"], tp.get("Phibar", tp.get("Phib")), tp.get("xi", 0.0),
+                          qp["m"]["u"], qp["m"]["s"], qp["mu"]["u"], qp["mu"]["s"], qp["A"]["u"], qp["A"]["s"],
+                          k=k_norm, mass0=np.nan if initial_mass is None else initial_mass, gamma0=initial_gamma,
+                          ftol=ftol, xtol=xtol, max_iter=iterations)
+    r = meson_mass_batch(q, K_coeffs, device)
+    if r["exception"][0]:
+        raise FloatingPointError("solve_meson_mass: non-finite residual at the initial point")
+    return dict(mass=float(r["mass"][0]), gamma=float(r["gamma"][0]), converged=bool(r["converged"][0]),
+                residual_norm=float(r["residual_norm"][0]), iterations=int(r["iterations"][0]))
+
+
+def mott_threshold_mass(meson, quark_params):
+    """MottTransition.jl: the lowest quark-pair threshold of a meson."""
+    m = quark_params["m"]
+    md = m.get("d", m["u"])
+    name = str(meson).lstrip(":")
+    thr = {"pi": m["u"] + md, "K": m["u"] + m["s"], "sigma_pi": m["u"] + md, "sigma_K": m["u"] + m["s"]}
+    if name in thr:
+        return thr[name]
+    if name in ("eta", "eta_prime", "sigma", "sigma_prime"):
+        return min(2.0 * m["u"], 2.0 * m["s"])
+    raise ValueError(f"Unknown meson type: {meson}")
+
+
+def mott_gap(meson, meson_mass, quark_params):
+    """MottTransition.jl: M_meson - threshold."""
+    return meson_mass - mott_threshold_mass(meson, quark_params)
+
+
+def is_mott_point(meson, meson_mass, quark_params, atol=1e-6):
+    return abs(mott_gap(meson, meson_mass, quark_params)) <= atol
+
+
+__all__ += ["MASS_MESONS", "MASS_QUERY_DTYPE", "make_mass_queries", "meson_mass_batch", "default_meson_mass_guess",
+            "solve_meson_mass", "mott_threshold_mass", "mott_gap", "is_mott_point"]

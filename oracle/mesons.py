@@ -24,6 +24,7 @@ The Pi values come from the oracle's C restatement of polarization_aniso
 """
 from __future__ import annotations
 
+import cmath
 import math
 
 import numpy as np
@@ -380,3 +381,149 @@ def propagator_by_channel_vec(q, G, K, nthreads=8, Nc=3):
                 tgt[sel] += _mixed_vec(c[f"det_K_{sfx}"][sel], pi[sel, s0], pi[sel, s0 + 4], c[f"K0_{sfx}"][sel],
                                        c[f"K8_{sfx}"][sel], c[f"K08_{sfx}"][sel], J[0], J[1], Jp[0], Jp[1])
     return DS, DP
+
+
+# ---- meson pole masses (MesonMass.jl) ----------------------------------------------
+MASS_MESONS = ["pi", "K", "sigma_pi", "sigma_K", "eta", "eta_prime", "sigma", "sigma_prime"]
+FD_STEP = 6.055454452393343e-06          # cbrt(eps(Float64)): FiniteDiff's central relstep = absstep
+
+
+def default_meson_mass_guess(meson, qp):
+    """MesonMass.jl:183-204 (d = u)."""
+    mu_, ms = qp["m"]["u"], qp["m"]["s"]
+    return {"pi": mu_ + mu_, "K": mu_ + ms, "sigma_pi": 2.0 * mu_, "sigma_K": mu_ + ms, "eta": 2.0 * mu_,
+            "eta_prime": 2.0 * ms, "sigma": 2.0 * mu_, "sigma_prime": 2.0 * ms}[meson]
+
+
+def _pi_width(channel, pair, k0, gamma, k, qp, tp):
+    """polarization_with_width for the quark pair uu / us / ss (num_s 0, 1, 0:
+    MesonMass.jl passes 0 for s sbar, equivalent to 2)."""
+    f1, f2, ns = {"uu": ("u", "u", 0), "us": ("u", "s", 1), "ss": ("s", "s", 0)}[pair]
+    re, im = O.polarization(O.CHANNEL_P if channel == "P" else O.CHANNEL_S, k0, k, qp["m"][f1], qp["m"][f2],
+                            qp["mu"][f1], qp["mu"][f2], tp["T"], tp["Phi"], tp["Phib"], tp["xi"], qp["A"][f1],
+                            qp["A"][f2], ns, gamma=gamma)
+    return complex(re, im)
+
+
+def meson_mass_equation(meson, k0, gamma, k, qp, tp, kc):
+    """MesonMass.jl:211-233: 1 - 4 K Pi(k0, gamma) for pi, K, sigma_pi, sigma_K;
+    (M00 + M88) -/+ sqrt((M00 - M88)^2 + 4 M08^2) (the light / heavy
+    eigenvalue of 2M) for eta / eta' (channel P) and sigma / sigma' (S),
+    MesonMass.jl:34-87."""
+    simple = {"pi": ("P", "uu", "K123_plus"), "K": ("P", "us", "K4567_plus"), "sigma_pi": ("S", "uu", "K123_minus"),
+              "sigma_K": ("S", "us", "K4567_minus")}
+    if meson in simple:
+        ch, pair, key = simple[meson]
+        return 1.0 - 4.0 * kc[key] * _pi_width(ch, pair, k0, gamma, k, qp, tp)
+    ch = "P" if meson in ("eta", "eta_prime") else "S"
+    Puu = _pi_width(ch, "uu", k0, gamma, k, qp, tp)
+    Pss = _pi_width(ch, "ss", k0, gamma, k, qp, tp)
+    M00, M08, M88, _ = mixing_matrix_elements(Puu, Pss, kc, ch)
+    sq = cmath.sqrt((M00 - M88) * (M00 - M88) + 4.0 * (M08 * M08))
+    return (M00 + M88) - sq if meson in ("eta", "sigma") else (M00 + M88) + sq
+
+
+def _wnorm(d, v):
+    return math.sqrt(sum((di * vi) * (di * vi) for di, vi in zip(d, v)))
+
+
+def _div(a, b):
+    """IEEE a / b (inf / nan instead of ZeroDivisionError), as Julia divides."""
+    with np.errstate(all="ignore"):
+        return float(np.float64(a) / np.float64(b))
+
+
+def _dogleg(r, d, J, delta):
+    """NLsolve dogleg! (2 x 2; a singular Jacobian takes the scaled
+    steepest-descent branch, as the gap path's restatement does)."""
+    det = J[0][0] * J[1][1] - J[0][1] * J[1][0]
+    if det != 0.0 and math.isfinite(det):
+        pi_ = [-_div(J[1][1] * r[0] - J[0][1] * r[1], det), -_div(-J[1][0] * r[0] + J[0][0] * r[1], det)]
+    else:
+        pi_ = [math.inf, math.inf]
+    if _wnorm(d, pi_) <= delta:
+        return pi_
+    g = [(J[0][j] * r[0] + J[1][j] * r[1]) / (d[j] * d[j]) for j in range(2)]
+    Jg = [J[i][0] * g[0] + J[i][1] * g[1] for i in range(2)]
+    wg = _wnorm(d, g)
+    coef = _div(-(wg * wg), Jg[0] * Jg[0] + Jg[1] * Jg[1])
+    pc = [coef * g[0], coef * g[1]]
+    if _wnorm(d, pc) >= delta:
+        sc = _div(-delta, wg)
+        return [g[0] * sc, g[1] * sc]
+    pdiff = [pi_[0] - pc[0], pi_[1] - pc[1]]
+    b = 2.0 * sum((d[i] * pc[i]) * (d[i] * pdiff[i]) for i in range(2))
+    wa = _wnorm(d, pdiff)
+    a = wa * wa
+    wpc = _wnorm(d, pc)
+    tau = _div(-b + math.sqrt(b * b - 4.0 * a * (wpc * wpc - delta * delta)), 2.0 * a)
+    return [pc[0] + tau * pdiff[0], pc[1] + tau * pdiff[1]]
+
+
+def solve_meson_mass(meson, qp, tp, kc=None, k=0.0, initial_mass=None, initial_gamma=0.0, ftol=1e-8, xtol=0.0,
+                     iterations=1000, G=None, K=None):
+    """MesonMass.solve_meson_mass (MesonMass.jl:241-276) with NLsolve's
+    defaults: trust region with dogleg and autoscaling (factor 1), central
+    finite-difference Jacobian (FiniteDiff, step max(cbrt(eps) |x|, cbrt(eps))),
+    ftol 1e-8, xtol 0, 1000 iterations.  kc None: couplings from qp's A.
+    Returns dict(mass, gamma, converged, residual_norm, iterations, exception)."""
+    if kc is None:
+        kc = effective_couplings(G, K, G_from_A(qp["A"]["u"], qp["m"]["u"]), G_from_A(qp["A"]["s"], qp["m"]["s"]))
+    x = [default_meson_mass_guess(meson, qp) if initial_mass is None else float(initial_mass), float(initial_gamma)]
+
+    def F(v):
+        f = meson_mass_equation(meson, v[0], v[1], k, qp, tp, kc)
+        return [f.real, f.imag]
+
+    def jac(v):
+        J = [[0.0, 0.0], [0.0, 0.0]]
+        for j in range(2):
+            eps = max(FD_STEP * abs(v[j]), FD_STEP)
+            vp, vm = list(v), list(v)
+            vp[j] = v[j] + eps
+            vm[j] = v[j] - eps
+            fp, fm = F(vp), F(vm)
+            for i in range(2):
+                J[i][j] = (fp[i] - fm[i]) / (2 * eps)
+        return J
+
+    r = F(x)
+    if not all(math.isfinite(t) for t in r):
+        return dict(mass=x[0], gamma=x[1], converged=False, residual_norm=math.nan, iterations=0, exception=True)
+    J = jac(x)
+    it = 0
+    conv = max(abs(r[0]), abs(r[1])) < ftol
+    stopped = any(math.isnan(t) for t in x + r)
+    if not conv:
+        d = [math.sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]) for j in range(2)]
+        d = [dj if dj != 0.0 else 1.0 for dj in d]
+        delta = 1.0 * _wnorm(d, x)
+        if delta == 0.0:
+            delta = 1.0
+        while not stopped and not conv and it < iterations:
+            it += 1
+            p = _dogleg(r, d, J, delta)
+            xold = list(x)
+            x = [x[0] + p[0], x[1] + p[1]]
+            fn = F(x)
+            rpred = [J[i][0] * p[0] + J[i][1] * p[1] + r[i] for i in range(2)]
+            ssr = r[0] * r[0] + r[1] * r[1]
+            rho = _div(ssr - (fn[0] * fn[0] + fn[1] * fn[1]), ssr - (rpred[0] * rpred[0] + rpred[1] * rpred[1]))
+            if rho > 1e-4:
+                r = fn
+                J = jac(x)
+                d = [max(0.1 * d[j], math.sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j])) for j in range(2)]
+                xc = max(abs(x[0] - xold[0]), abs(x[1] - xold[1])) < xtol
+                conv = xc or max(abs(r[0]), abs(r[1])) < ftol
+            else:
+                x = [x[0] - p[0], x[1] - p[1]]
+                conv = False
+            if rho < 0.1:
+                delta = delta / 2.0
+            elif rho >= 0.9:
+                delta = 2.0 * _wnorm(d, p)
+            elif rho >= 0.5:
+                delta = max(delta, 2.0 * _wnorm(d, p))
+            stopped = any(math.isnan(t) for t in x + fn)
+    return dict(mass=x[0], gamma=x[1], converged=bool(conv), residual_norm=max(abs(r[0]), abs(r[1])),
+                iterations=it, exception=False)

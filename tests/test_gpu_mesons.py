@@ -224,3 +224,60 @@ def test_device_pointer_and_one_per_wave_paths(gpu_ctx, P, M):
     torch.cuda.synchronize()
     o = out.cpu().numpy()
     assert np.array_equal(o[:, 0] + 1j * o[:, 1], DS) and np.array_equal(o[:, 2] + 1j * o[:, 3], DP)
+
+
+def _mass_states(oracle):
+    """(quark_params, thermo_params) at solved equilibrium points; A on GL(0, 20, 16)."""
+    out = []
+    x, w = oracle.gauleg(0.0, 20.0, 16)
+    for TM, mM, xi in ((120, 0, 0.0), (150, 0, 0.0), (150, 100, 0.3), (200, 100, 0.0), (250, 0, -0.2),
+                       (100, 250, 0.0), (180, 200, 0.2)):
+        T, mu = TM / HB, mM / HB
+        r = oracle.solve_batch(np.array([T]), np.array([mu]), xi, policy=oracle.SEED_DEFAULT, p_num=64, t_num=8,
+                               opts=oracle.default_opts(ftol=1e-12))
+        ms, X = r["masses"][0], r["x"][0]
+        qp = dict(m=dict(u=ms[0], s=ms[2]), mu=dict(u=mu, s=mu))
+        qp["A"] = dict(u=oracle.A(ms[0], mu, T, X[3], X[4], x, w), s=oracle.A(ms[2], mu, T, X[3], X[4], x, w))
+        out.append((qp, dict(T=T, Phi=X[3], Phib=X[4], xi=xi)))
+    return out
+
+
+def test_meson_masses_match_oracle(gpu_ctx, oracle, P, M):
+    """Every meson at seven equilibrium points, from the default guess and from
+    two other starts: the same convergence / exception outcome as the oracle's
+    restatement of NLsolve's trust region; converged poles to 1e-9 relative
+    (1e-7 absolute where a rounding difference changed the trust-region path)."""
+    G, K = _GK(gpu_ctx)
+    rows, ref = [], []
+    for qp, tp in _mass_states(oracle):
+        for mes in M.MASS_MESONS:
+            for m0, g0 in ((None, 0.0), (1.2, 0.0), (0.9, 0.1)):
+                rows.append((mes, qp, tp, m0, g0))
+                ref.append(M.solve_meson_mass(mes, qp, tp, initial_mass=m0, initial_gamma=g0, G=G, K=K))
+    q = np.concatenate([P.make_mass_queries(mes, tp["T"], tp["Phi"], tp["Phib"], tp["xi"], qp["m"]["u"], qp["m"]["s"],
+                                            qp["mu"]["u"], qp["mu"]["s"], qp["A"]["u"], qp["A"]["s"],
+                                            mass0=np.nan if m0 is None else m0, gamma0=g0)
+                        for mes, qp, tp, m0, g0 in rows])
+    g = P.meson_mass_batch(q)
+    exc = np.array([r["exception"] for r in ref])
+    conv = np.array([r["converged"] for r in ref])
+    assert np.array_equal(g["exception"], exc)
+    assert np.array_equal(g["converged"], conv)
+    om, og = np.array([r["mass"] for r in ref]), np.array([r["gamma"] for r in ref])
+    it = np.array([r["iterations"] for r in ref])
+    c = conv & ~exc
+    assert c.sum() > len(rows) // 3
+    same_path = c & (g["iterations"] == it)
+    tight = np.abs(g["mass"] - om) <= 1e-9 * np.maximum(np.abs(om), 0.1)
+    tight &= np.abs(g["gamma"] - og) <= 1e-9 * np.maximum(np.abs(om), 0.1)
+    assert np.all(tight[same_path]) and same_path.sum() >= 0.9 * c.sum()
+    loose = (np.abs(g["mass"] - om) <= 1e-7) & (np.abs(g["gamma"] - og) <= 1e-7)
+    assert np.all(loose[c])
+    assert np.all(g["residual_norm"][c] < 1e-8)
+    # the host forms
+    qp, tp = _mass_states(oracle)[3]
+    r = P.solve_meson_mass("sigma", qp, dict(tp, Phibar=tp["Phib"]))
+    o = M.solve_meson_mass("sigma", qp, tp, G=G, K=K)
+    assert r["converged"] == o["converged"] and abs(r["mass"] - o["mass"]) <= 1e-7
+    with pytest.raises(ValueError):
+        P.solve_meson_mass("rho", qp, tp)

@@ -280,3 +280,91 @@ def test_vectorised_form_matches_scalar_restatement(oracle, M, GK):
     v_S, v_P = M.propagator_by_channel_vec(q, G, K, nthreads=4)
     for a, b in ((s_S, v_S), (s_P, v_P)):
         assert np.all(np.abs(a - b) <= 1e-13 * np.maximum(np.abs(a), 1e-300)), np.max(np.abs(a - b) / np.abs(a))
+
+
+# ---- pole masses (MesonMass.jl / MottTransition.jl) ---------------------------------
+def _qp_with_A(oracle, m_u, m_d, m_s, T, Phi, Phib, mu=0.0):
+    """ensure_quark_params_has_A (MesonMass.jl:88-120) at xi = 0: A on GL(0, 20, 16)."""
+    x, w = oracle.gauleg(0.0, 20.0, 16)
+    qp = dict(m=dict(u=m_u, d=m_d, s=m_s), mu=dict(u=mu, d=mu, s=mu))
+    qp["A"] = {f: oracle.A(qp["m"][f], mu, T, Phi, Phib, x, w) for f in "uds"}
+    return qp
+
+
+def test_mass_guess_and_mott_thresholds(oracle, M):
+    """test_meson_mass_mott_transition.jl:19-49."""
+    from julia_relaxtime_amd import propagators as P
+    qp = dict(m=dict(u=0.3, d=0.31, s=0.5), mu=dict(u=0.0, d=0.0, s=0.0))
+    assert P.default_meson_mass_guess("pi", qp) == 0.3 + 0.31
+    assert P.default_meson_mass_guess("K", qp) == 0.3 + 0.5
+    assert P.default_meson_mass_guess("eta", qp) == 0.6 and P.default_meson_mass_guess("eta_prime", qp) == 1.0
+    thr = P.mott_threshold_mass("K", qp)
+    assert thr == 0.8 and P.is_mott_point("K", thr, qp, atol=1e-12)
+    assert P.mott_threshold_mass("eta", qp) == min(0.6, 1.0) and P.mott_gap("sigma", 0.7, qp) == 0.7 - 0.6
+    qp_ud = dict(m=dict(u=0.3, s=0.5))
+    for mes in M.MASS_MESONS:
+        assert M.default_meson_mass_guess(mes, qp_ud) == P.default_meson_mass_guess(mes, qp_ud)
+
+
+def test_mixed_residual_is_twice_the_eigenvalues(oracle, M, GK):
+    """The eta / eta' (sigma / sigma') residuals are the two eigenvalues of 2M
+    (numpy eigvals), their product 4 det M, and at an SU(3)-symmetric point
+    M08 = 0 (test_meson_mass_mott_transition.jl:51-69)."""
+    G, K = GK
+    qp = _qp_with_A(oracle, 0.35, 0.35, 0.35, 0.12, 0.6, 0.6)
+    tp = dict(T=0.12, Phi=0.6, Phib=0.6, xi=0.0)
+    kc = M.effective_couplings(G, K, M.G_from_A(qp["A"]["u"], 0.35), M.G_from_A(qp["A"]["s"], 0.35))
+    Puu = M._pi_width("P", "uu", 0.7, 0.0, 0.0, qp, tp)
+    Pss = M._pi_width("P", "ss", 0.7, 0.0, 0.0, qp, tp)
+    M00, M08, M88, _ = M.mixing_matrix_elements(Puu, Pss, kc, "P")
+    assert abs(M08) <= 1e-10
+    fe = M.meson_mass_equation("eta", 0.7, 0.0, 0.0, qp, tp, kc)
+    fp = M.meson_mass_equation("eta_prime", 0.7, 0.0, 0.0, qp, tp, kc)
+    assert abs(fe * fp - 4.0 * (M00 * M88 - M08 * M08)) <= 1e-10 * abs(fe * fp)
+    qp = _qp_with_A(oracle, 0.3, 0.3, 0.5, 0.75, 0.5, 0.55, mu=0.2)
+    tp = dict(T=0.75, Phi=0.5, Phib=0.55, xi=0.0)
+    kc = M.effective_couplings(G, K, M.G_from_A(qp["A"]["u"], 0.3), M.G_from_A(qp["A"]["s"], 0.5))
+    for k0, gam, ch, pair in ((0.9, 0.0, "P", ("eta", "eta_prime")), (1.7, 0.3, "S", ("sigma", "sigma_prime")),
+                              (2.6, 0.8, "P", ("eta", "eta_prime"))):
+        Puu = M._pi_width(ch, "uu", k0, gam, 0.0, qp, tp)
+        Pss = M._pi_width(ch, "ss", k0, gam, 0.0, qp, tp)
+        M00, M08, M88, _ = M.mixing_matrix_elements(Puu, Pss, kc, ch)
+        ev = 2 * np.linalg.eigvals(np.array([[M00, M08], [M08, M88]]))
+        f = [M.meson_mass_equation(m, k0, gam, 0.0, qp, tp, kc) for m in pair]
+        assert sorted(abs(f[0] - e) for e in ev)[0] <= 1e-12 * abs(f[0]) + 1e-15
+        assert sorted(abs(f[1] - e) for e in ev)[0] <= 1e-12 * abs(f[1]) + 1e-15
+        assert abs(f[0] * f[1] - 4.0 * (M00 * M88 - M08 * M08)) <= 1e-10 * abs(f[0] * f[1])
+
+
+def test_pole_solver_finds_the_dyson_pole(oracle, M, GK):
+    """Converged (mass, gamma) are zeros of the residual to ftol; the central-
+    difference Jacobian matches the analytic gamma derivative of
+    polarization_with_width (Pi is a quadratic in gamma at fixed k0); the
+    solve is independent of the starting point within the basin."""
+    G, K = GK
+    HBm = 197.327
+    T = 200 / HBm
+    qp = _qp_with_A(oracle, 80 / HBm, 80 / HBm, 400 / HBm, T, 0.5, 0.5, mu=100 / HBm)
+    tp = dict(T=T, Phi=0.5, Phib=0.5, xi=0.0)
+    kc = M.effective_couplings(G, K, M.G_from_A(qp["A"]["u"], qp["m"]["u"]), M.G_from_A(qp["A"]["s"], qp["m"]["s"]))
+    for mes in ("pi", "sigma_pi", "eta", "sigma_prime"):
+        r = M.solve_meson_mass(mes, qp, tp, kc=kc)
+        assert r["converged"] and r["residual_norm"] < 1e-8
+        f = M.meson_mass_equation(mes, r["mass"], r["gamma"], 0.0, qp, tp, kc)
+        assert abs(f.real) < 1e-8 and abs(f.imag) < 1e-8
+        r2 = M.solve_meson_mass(mes, qp, tp, kc=kc, initial_mass=r["mass"] * 1.05, initial_gamma=r["gamma"])
+        assert r2["converged"] and abs(r2["mass"] - r["mass"]) < 1e-7 * abs(r["mass"])
+    # d/dgamma of 1 - 4 K Pi: Pi = c [A12 + (k^2 - lam^2 + (m1 -/+ m2)^2 + g^2/4) B + i g lam B]
+    k0, g = 1.1, 0.2
+    Pa = M._pi_width("P", "uu", k0, g, 0.0, qp, tp)
+    Pb = M._pi_width("P", "uu", k0, 0.0, 0.0, qp, tp)
+    h = 1e-3
+    Pp, Pm = M._pi_width("P", "uu", k0, g + h, 0.0, qp, tp), M._pi_width("P", "uu", k0, g - h, 0.0, qp, tp)
+    quad = (Pp - 2 * Pa + Pm) / (h * h)             # = 2c for Pi = a + b g + c g^2
+    lin = (Pp - Pm) / (2 * h)                       # = b + 2 c g
+    assert abs((Pa - Pb) - (lin - quad * g) * g - quad * g * g / 2) <= 1e-9 * abs(Pa - Pb)
+    # the solver's own central difference at its step agrees with the wide-step derivative of the quadratic
+    step = max(M.FD_STEP * abs(g), M.FD_STEP)
+    fd = (M._pi_width("P", "uu", k0, g + step, 0.0, qp, tp) - M._pi_width("P", "uu", k0, g - step, 0.0, qp, tp)) / (
+        2 * step)
+    assert abs(fd - lin) <= 1e-6 * abs(lin)
