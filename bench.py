@@ -88,6 +88,11 @@ WORKLOADS = {
                                          "channel) entries of the process table uniformly, at the 256 equilibrium "
                                          "points of the Pi workload, (k0, k) on its axes; every Pi evaluated "
                                          "(no memo), couplings from A; FP64"),
+    "meson_masses": dict(masses=True, desc="solve_meson_mass for the 8 mesons (pi, K, sigma_pi, sigma_K, eta, eta', "
+                                           "sigma, sigma') at every converged point of the xi = 0 slice of the 256 x 256 "
+                                           "(T 20-300 MeV, mu 0-450 MeV) scan: k = 0, default guess, NLsolve defaults "
+                                           "(trust region, central-difference Jacobian, ftol 1e-8, 1000 iterations); "
+                                           "FP64"),
     "c5_multiroot": dict(grid=lambda: grid(512, 512, [-0.2, -0.1, 0.0, 0.1, 0.2], (10, 250), (200, 450)), p_num=64,
                          t_num=4, max_iter=40, learned=True, policy="MULTIROOT3", plain_policy="MULTI",
                          desc="512x512x5 (T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2 step 0.1), p=64 t=4, "
@@ -307,6 +312,72 @@ def run_propagators(args, w):
     print(json.dumps(line), flush=True)
 
 
+MASS_METRIC = "meson pole masses solved/s (solve_meson_mass, 8 mesons x 65,536 (T, mu) points)"
+# FP64 FLOPs per pole-mass query of this batch, counted once on 8,192 seeded
+# queries run one per wavefront (python3 -m julia_relaxtime_amd.mass_flop_probe):
+# 821,068 per query (38.9 trust-region iterations on average; 36 % of the
+# queries stop at the start on a non-finite residual).  The kernel's time is
+# set by its longest serial chains (queries running the full 1000
+# iterations), not by FP64 throughput.
+MASS_FLOPS_PER_QUERY = 821068.0
+
+
+def mass_cpu_baseline(q, G, K, n_sample=400, seed=11):
+    """The oracle's solve_meson_mass (Pi in C, trust region in Python, one
+    thread) on a seeded sample of the batch."""
+    from oracle import mesons as OM
+    sel = np.random.default_rng(seed).choice(q.size, n_sample, replace=False)
+    t0 = time.perf_counter()
+    for r in q[sel]:
+        qp = dict(m=dict(u=r["m_u"], s=r["m_s"]), mu=dict(u=r["mu_u"], s=r["mu_s"]), A=dict(u=r["A_u"], s=r["A_s"]))
+        tp = dict(T=r["T"], Phi=r["Phi"], Phib=r["Phib"], xi=r["xi"])
+        OM.solve_meson_mass(OM.MASS_MESONS[int(r["meson"])], qp, tp, G=G, K=K, k=r["k"])
+    dt = time.perf_counter() - t0
+    return dict(value=n_sample / dt, unit="queries/s", cores=1, kind="port",
+                sample=f"{n_sample} seeded-random queries of the batch (rng {seed}), oracle solve_meson_mass, "
+                       f"1 thread, {dt:.2f} s wall")
+
+
+def run_masses(args, w):
+    import torch
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import propagators as PR
+    from julia_relaxtime_amd.workloads import meson_mass_points, meson_mass_queries
+    ctx = _lib.Context(0)
+    pts = meson_mass_points(ctx, _lib.default_opts(ftol=1e-10, max_iter=40))
+    q = meson_mass_queries(pts)
+    n = q.size
+    pctx = PR._ctx(0)
+    res = {}
+
+    def step():
+        res["r"] = PR.meson_mass_batch(q)
+        return pctx.last_kernel_ms()
+
+    ms_step, kms = timed_steps(step, args.steps, args.warmup)
+    r = res["r"]
+    kavg = float(np.mean(kms))
+    flops = None if MASS_FLOPS_PER_QUERY is None else MASS_FLOPS_PER_QUERY * n
+    achieved = None if flops is None else flops / (kavg * 1e-3) / 1e12
+    line = {
+        "metric": MASS_METRIC, "value": n / (ms_step * 1e-3), "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(workload=args.workload, queries=n, points=int((pts[:, 8] == 1).sum()), detail=w["desc"],
+                       converged_fraction=float(r["converged"].mean()),
+                       exception_fraction=float(r["exception"].mean()),
+                       mean_iterations=float(r["iterations"].mean()), max_iterations=int(r["iterations"].max()),
+                       kernel_ms_per_step=kavg),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": None if achieved is None else achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "polar::mass_kernel", "kernel_ms_per_step": kavg,
+                     "flops_per_query": MASS_FLOPS_PER_QUERY},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = mass_cpu_baseline(q, pctx.params.G, pctx.params.K)
+    print(json.dumps(line), flush=True)
+
+
 def timed_steps(step, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -347,6 +418,8 @@ def main():
         return run_polar(args, w)
     if w.get("props"):
         return run_propagators(args, w)
+    if w.get("masses"):
+        return run_masses(args, w)
     T, mu, xi = w["grid"]()
     n = T.size
     dev = torch.device("cuda", 0)

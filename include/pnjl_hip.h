@@ -39,8 +39,9 @@ extern "C" {
 /* flags */
 #define PNJL_FLAG_DEVICE_PTRS 0x1
 #define PNJL_FLAG_ASYNC       0x2
-/* pnjl_polarization_batch and pnjl_propagator_by_channel_batch only: one
- * query (one Pi pair of a query) per wavefront (63 lanes idle), so that
+/* pnjl_polarization_batch, pnjl_propagator_by_channel_batch and
+ * pnjl_meson_mass_batch only: one query (one Pi pair of a query) per
+ * wavefront (63 lanes idle), so that
  * hardware FP64 counters see each query's own operations without the
  * inactive lanes of divergent waves -- for counting algorithmic FLOPs */
 #define PNJL_FLAG_ONE_PER_WAVE 0x4

@@ -1095,46 +1095,79 @@ __device__ __forceinline__ meson::C csqrt_p(meson::C z)
     return meson::C{fabs(z.i) / (2.0 * t), copysign(t, z.i)};
 }
 
-// polarization_with_width of one quark pair (uu 0, us 1, ss 2)  MesonMass.jl:34-60, :222-231
-__device__ void mass_pi(const Phys& P, const Tabs& T, const pnjl_mass_query& a, int pair, int channel, double k0,
-                        double gamma, meson::C& out)
+// The residual at (k0, gamma) needs B0 of one quark pair (two for the mixed
+// mesons) at k0 only: gamma enters polarization_with_width algebraically.
+// B0 is therefore computed once per k0 and reused for the gamma column of
+// the finite-difference Jacobian (the same values a recomputation gives).
+struct MassB0 {
+    double r[2], i[2];
+};
+
+__device__ __forceinline__ void mass_pair(const pnjl_mass_query& a, int pair, double& m1, double& m2, double& mu1,
+                                          double& mu2, double& A12)
 {
-    const double m1 = pair == 2 ? a.m_s : a.m_u, m2 = pair == 0 ? a.m_u : a.m_s;
-    const double mu1 = pair == 2 ? a.mu_s : a.mu_u, mu2 = pair == 0 ? a.mu_u : a.mu_s;
-    const double A1 = pair == 2 ? a.A_s : a.A_u, A2 = pair == 0 ? a.A_u : a.A_s;
-    double br, bi;
-    b0_total(P, T, k0, a.k, m1, m2, mu1, mu2, a.T, a.Phi, a.Phib, a.xi, pair == 1 ? 1 : 0, br, bi);
-    pi_value(P, channel, 1, k0, gamma, a.k, m1, m2, k0 + mu1 - mu2, A1 + A2, br, bi, out.r, out.i);
+    m1 = pair == 2 ? a.m_s : a.m_u, m2 = pair == 0 ? a.m_u : a.m_s;
+    mu1 = pair == 2 ? a.mu_s : a.mu_u, mu2 = pair == 0 ? a.mu_u : a.mu_s;
+    A12 = (pair == 2 ? a.A_s : a.A_u) + (pair == 0 ? a.A_u : a.A_s);
 }
 
-// meson_mass_equation  MesonMass.jl:211-233: 1 - 4 K Pi, or for the mixed
-// mesons the light / heavy eigenvalue of 2M, (M00 + M88) -/+ sqrt((M00 - M88)^2 + 4 M08^2)
-__device__ __noinline__ void mass_residual(const Phys& P, const Tabs& T, const pnjl_mass_query& a,
-                                           const meson::KCoef& k, double k0, double gamma, double* f)
+// the pairs of a meson: pi, sigma_pi -> uu; K, sigma_K -> us; mixed -> uu, ss
+__device__ __forceinline__ int mass_pairs(int m, int* pairs)
+{
+    if (m >= PNJL_MESON_ID_ETA) {
+        pairs[0] = 0, pairs[1] = 2;
+        return 2;
+    }
+    pairs[0] = (m == PNJL_MESON_ID_K || m == PNJL_MESON_ID_SIGMA_K) ? 1 : 0;
+    return 1;
+}
+
+// B0 (symmetrised for u sbar, + correction) of the meson's pairs at k0
+//   MesonMass.jl:34-60, :222-231 (MesonMass passes num_s 0 for s sbar, equivalent to 2)
+__device__ __noinline__ void mass_b0(const Phys& P, const Tabs& T, const pnjl_mass_query& a, double k0, MassB0& B)
+{
+    int pairs[2];
+    const int np = mass_pairs(a.meson, pairs);
+#pragma unroll 1
+    for (int j = 0; j < np; j++) {
+        double m1, m2, mu1, mu2, A12;
+        mass_pair(a, pairs[j], m1, m2, mu1, mu2, A12);
+        b0_total(P, T, k0, a.k, m1, m2, mu1, mu2, a.T, a.Phi, a.Phib, a.xi, pairs[j] == 1 ? 1 : 0, B.r[j], B.i[j]);
+    }
+}
+
+// meson_mass_equation  MesonMass.jl:211-233 from the B0 at k0: 1 - 4 K Pi, or
+// for the mixed mesons the light / heavy eigenvalue of 2M,
+// (M00 + M88) -/+ sqrt((M00 - M88)^2 + 4 M08^2)
+__device__ void mass_residual(const Phys& P, const pnjl_mass_query& a, const meson::KCoef& k, const MassB0& B,
+                              double k0, double gamma, double* f)
 {
     using meson::C;
     const int m = a.meson;
+    int pairs[2];
+    const int np = mass_pairs(m, pairs);
+    const bool pch = m == PNJL_MESON_ID_PI || m == PNJL_MESON_ID_K || m == PNJL_MESON_ID_ETA ||
+                     m == PNJL_MESON_ID_ETA_PRIME;
+    const int ch = pch ? PNJL_CHANNEL_P : PNJL_CHANNEL_S;
+    C Pi[2];
+    for (int j = 0; j < np; j++) {
+        double m1, m2, mu1, mu2, A12;
+        mass_pair(a, pairs[j], m1, m2, mu1, mu2, A12);
+        pi_value(P, ch, 1, k0, gamma, a.k, m1, m2, k0 + mu1 - mu2, A12, B.r[j], B.i[j], Pi[j].r, Pi[j].i);
+    }
     C res;
     if (m < PNJL_MESON_ID_ETA) {
-        const int pair = (m == PNJL_MESON_ID_K || m == PNJL_MESON_ID_SIGMA_K) ? 1 : 0;
-        const int ch = (m == PNJL_MESON_ID_PI || m == PNJL_MESON_ID_K) ? PNJL_CHANNEL_P : PNJL_CHANNEL_S;
         const double K = m == PNJL_MESON_ID_PI ? k.K123p
                        : m == PNJL_MESON_ID_K  ? k.K4567p
                        : m == PNJL_MESON_ID_SIGMA_PI ? k.K123m : k.K4567m;
-        C Pi;
-        mass_pi(P, T, a, pair, ch, k0, gamma, Pi);
         const double k4 = 4.0 * K;
-        res = C{1.0 - k4 * Pi.r, -(k4 * Pi.i)};
+        res = C{1.0 - k4 * Pi[0].r, -(k4 * Pi[0].i)};
     } else {
-        const bool pch = m == PNJL_MESON_ID_ETA || m == PNJL_MESON_ID_ETA_PRIME;
-        const int ch = pch ? PNJL_CHANNEL_P : PNJL_CHANNEL_S;
-        C Puu, Pss, M00, M08, M88;
-        mass_pi(P, T, a, 0, ch, k0, gamma, Puu);
-        mass_pi(P, T, a, 2, ch, k0, gamma, Pss);
+        C M00, M08, M88;
         if (pch)
-            meson::mixing(Puu, Pss, k.K0p, k.K8p, k.K08p, k.detKp, M00, M08, M88);
+            meson::mixing(Pi[0], Pi[1], k.K0p, k.K8p, k.K08p, k.detKp, M00, M08, M88);
         else
-            meson::mixing(Puu, Pss, k.K0m, k.K8m, k.K08m, k.detKm, M00, M08, M88);
+            meson::mixing(Pi[0], Pi[1], k.K0m, k.K8m, k.K08m, k.detKm, M00, M08, M88);
         const C dm = meson::csub(M00, M88);
         const C sq = csqrt_p(meson::cadd(meson::cmul(dm, dm), meson::rmul(4.0, meson::cmul(M08, M08))));
         const C tr = meson::cadd(M00, M88);
@@ -1144,17 +1177,26 @@ __device__ __noinline__ void mass_residual(const Phys& P, const Tabs& T, const p
     f[1] = res.i;
 }
 
-// central finite differences (FiniteDiff): J[:, j] = (f(x + e_j h) - f(x - e_j h)) / 2h
+// central finite differences (FiniteDiff): J[:, j] = (f(x + e_j h) - f(x - e_j h)) / 2h;
+// Bx holds B0 at x[0] (the gamma column needs no new B0)
 __device__ void mass_jac(const Phys& P, const Tabs& T, const pnjl_mass_query& a, const meson::KCoef& k,
-                         const double* x, double J[2][2])
+                         const double* x, const MassB0& Bx, double J[2][2])
 {
     for (int j = 0; j < 2; j++) {
         const double h = fmax(kFdStep * fabs(x[j]), kFdStep);
         double xp[2] = {x[0], x[1]}, xm[2] = {x[0], x[1]}, fp[2], fm[2];
         xp[j] = x[j] + h;
         xm[j] = x[j] - h;
-        mass_residual(P, T, a, k, xp[0], xp[1], fp);
-        mass_residual(P, T, a, k, xm[0], xm[1], fm);
+        if (j == 0) {
+            MassB0 Bp, Bm;
+            mass_b0(P, T, a, xp[0], Bp);
+            mass_b0(P, T, a, xm[0], Bm);
+            mass_residual(P, a, k, Bp, xp[0], xp[1], fp);
+            mass_residual(P, a, k, Bm, xm[0], xm[1], fm);
+        } else {
+            mass_residual(P, a, k, Bx, xp[0], xp[1], fp);
+            mass_residual(P, a, k, Bx, xm[0], xm[1], fm);
+        }
         for (int i = 0; i < 2; i++) J[i][j] = (fp[i] - fm[i]) / (2.0 * h);
     }
 }
@@ -1215,13 +1257,16 @@ __device__ __forceinline__ double mass_guess(const pnjl_mass_query& a)
 // solve_meson_mass  MesonMass.jl:241-276 with NLsolve's trust region (dogleg,
 // autoscale, factor 1) and a central-difference Jacobian; one thread per query.
 // out[i*6] = (mass, gamma, residual_norm (inf), converged, iterations, exception)
+// stride 64 puts one query per wavefront, for counting FLOPs.
 __global__ void __launch_bounds__(256) mass_kernel(Phys P, const double* tab, int64_t n,
                                                    const pnjl_mass_query* qs, const double* kc, double G, double K,
-                                                   int Nc, double* out)
+                                                   int Nc, double* out, int stride)
 {
     __shared__ double lds[kTabLen];
     load_tabs(lds, tab);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ts % stride) return;
+    const int64_t i = ts / stride;
     if (i >= n) return;
     const Tabs T = make_tabs(lds);
     const pnjl_mass_query a = qs[i];
@@ -1235,12 +1280,14 @@ __global__ void __launch_bounds__(256) mass_kernel(Phys P, const double* tab, in
                                                  meson::G_from_A(a.A_s, a.m_s, Nc));
     double x[2] = {isnan(a.mass0) ? mass_guess(a) : a.mass0, a.gamma0};
     double r[2], J[2][2];
-    mass_residual(P, T, a, k, x[0], x[1], r);
+    MassB0 B;
+    mass_b0(P, T, a, x[0], B);
+    mass_residual(P, a, k, B, x[0], x[1], r);
     if (!isfinite(r[0]) || !isfinite(r[1])) {   // NLsolve's IsFiniteException at the start
         o[0] = x[0], o[1] = x[1], o[2] = __builtin_nan(""), o[3] = 0.0, o[4] = 0.0, o[5] = 1.0;
         return;
     }
-    mass_jac(P, T, a, k, x, J);
+    mass_jac(P, T, a, k, x, B, J);
     int it = 0;
     bool conv = fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
     bool stopped = isnan(x[0]) || isnan(x[1]) || isnan(r[0]) || isnan(r[1]);
@@ -1258,14 +1305,16 @@ __global__ void __launch_bounds__(256) mass_kernel(Phys P, const double* tab, in
             dogleg2(p, r, d, J, delta);
             const double xold[2] = {x[0], x[1]};
             x[0] += p[0], x[1] += p[1];
-            mass_residual(P, T, a, k, x[0], x[1], fn);
+            MassB0 Bn;
+            mass_b0(P, T, a, x[0], Bn);
+            mass_residual(P, a, k, Bn, x[0], x[1], fn);
             double rp[2];
             for (int q = 0; q < 2; q++) rp[q] = J[q][0] * p[0] + J[q][1] * p[1] + r[q];
             const double ssr = r[0] * r[0] + r[1] * r[1];
             const double rho = (ssr - (fn[0] * fn[0] + fn[1] * fn[1])) / (ssr - (rp[0] * rp[0] + rp[1] * rp[1]));
             if (rho > 1e-4) {
                 r[0] = fn[0], r[1] = fn[1];
-                mass_jac(P, T, a, k, x, J);
+                mass_jac(P, T, a, k, x, Bn, J);
                 for (int j = 0; j < 2; j++) d[j] = fmax(0.1 * d[j], sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]));
                 const bool xc = fmax(fabs(x[0] - xold[0]), fabs(x[1] - xold[1])) < a.xtol;
                 conv = xc || fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
@@ -1684,7 +1733,8 @@ int pnjl_meson_mass_batch(pnjl_ctx* ctx, int64_t n, const pnjl_mass_query* q, co
                           int flags, void* stream)
 {
     if (!ctx) return PNJL_ERR_ARG;
-    if (n < 0 || (n > 0 && (!q || !out)) || n > ((int64_t)1 << 28))
+    if (n < 0 || (n > 0 && (!q || !out)) || n > ((int64_t)1 << 28) ||
+        ((flags & PNJL_FLAG_ONE_PER_WAVE) && n > ((int64_t)1 << 22)))
         return set_err(ctx, PNJL_ERR_ARG, "pnjl_meson_mass_batch: bad argument");
     if (n == 0) return PNJL_OK;
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1714,8 +1764,9 @@ int pnjl_meson_mass_batch(pnjl_ctx* ctx, int64_t n, const pnjl_mass_query* q, co
         dout = (double*)b;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
-    hipLaunchKernelGGL(polar::mass_kernel, dim3(blocks_for(n)), dim3(256), 0, st, phys(ctx), tab, n, dq, dkc,
-                       ctx->params.G, ctx->params.K, ctx->params.N_color, dout);
+    const int stride = (flags & PNJL_FLAG_ONE_PER_WAVE) ? 64 : 1;
+    hipLaunchKernelGGL(polar::mass_kernel, dim3(blocks_for(n * stride)), dim3(256), 0, st, phys(ctx), tab, n, dq, dkc,
+                       ctx->params.G, ctx->params.K, ctx->params.N_color, dout, stride);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
     if (!dev) HIPCHK(ctx, hipMemcpyAsync(out, dout, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost, st));

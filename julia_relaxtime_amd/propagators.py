@@ -360,15 +360,18 @@ def make_mass_queries(meson, T, Phi, Phib, xi, m_u, m_s, mu_u, mu_s, A_u, A_s, k
     return q
 
 
-def meson_mass_batch(q, K_coeffs=None, device=0):
+def meson_mass_batch(q, K_coeffs=None, device=0, one_per_wave=False):
     """Pole masses for a MASS_QUERY_DTYPE array -> dict of arrays (mass, gamma,
-    residual_norm, converged, iterations, exception)."""
+    residual_norm, converged, iterations, exception).  one_per_wave: one
+    query per wavefront, for FLOP counting."""
     ctx = _ctx(device)
     q = np.ascontiguousarray(q, dtype=MASS_QUERY_DTYPE)
     n = q.size
     kc = None if K_coeffs is None else _kc_rows(K_coeffs, n)
     out = np.zeros((n, 6))
-    rc = _lib.load().pnjl_meson_mass_batch(ctx.h, n, _ptr(q), None if kc is None else _ptr(kc), _ptr(out), 0, None)
+    flags = _lib.FLAG_ONE_PER_WAVE if one_per_wave else 0
+    rc = _lib.load().pnjl_meson_mass_batch(ctx.h, n, _ptr(q), None if kc is None else _ptr(kc), _ptr(out), flags,
+                                           None)
     ctx._check(rc, "pnjl_meson_mass_batch")
     return dict(mass=out[:, 0].copy(), gamma=out[:, 1].copy(), residual_norm=out[:, 2].copy(),
                 converged=out[:, 3] == 1.0, iterations=out[:, 4].astype(int), exception=out[:, 5] == 1.0)

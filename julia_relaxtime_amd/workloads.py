@@ -70,3 +70,30 @@ def propagator_queries(ctx, pts, n=1 << 20, seed=20261203):
     q["m_u"], q["m_s"], q["Phi"], q["Phib"] = P[:, 3], P[:, 4], P[:, 5], P[:, 6]
     q["A_u"], q["A_s"] = A[0, pt], A[1, pt]
     return q
+
+
+def meson_mass_points(ctx, opts, nT=256, nmu=256):
+    """Equilibrium points of the pole-mass workload: the xi = 0 slice of the
+    256 x 256 (T 20-300 MeV, mu 0-450 MeV) scan, DefaultSeed p64 t4, with
+    A(m_u), A(m_s) on GL(0, 20, 16) (ensure_quark_params_has_A at xi = 0).
+    Returns (n, 9) rows (T, mu, m_u, m_s, Phi, Phib, A_u, A_s, converged)."""
+    from . import _lib
+    from . import polarization as PZ
+    T, mu, xi = grid(nT, nmu, [0.0])
+    r = ctx.solve_batch(T, mu, xi, policy=_lib.SEED_DEFAULT, p_num=64, t_num=4, opts=opts)
+    m_u, m_s, Phi, Phib = r["masses"][:, 0], r["masses"][:, 2], r["x"][:, 3], r["x"][:, 4]
+    xp, wp = _lib.gauleg(0.0, 20.0, 16)
+    args = np.concatenate([np.stack([m_u, mu, T, Phi, Phib], 1), np.stack([m_s, mu, T, Phi, Phib], 1)])
+    A = PZ.A_batch(args, xp, wp).reshape(2, T.size)
+    return np.stack([T, mu, m_u, m_s, Phi, Phib, A[0], A[1], r["converged"].astype(float)], 1)
+
+
+def meson_mass_queries(pts):
+    """All eight mesons at every converged point (k = 0, default guess,
+    NLsolve defaults): a MASS_QUERY_DTYPE array, point-major."""
+    from . import propagators as PR
+    P = pts[pts[:, 8] == 1.0]
+    n = len(P) * len(PR.MASS_MESONS)
+    rep = lambda c: np.repeat(P[:, c], len(PR.MASS_MESONS))
+    return PR.make_mass_queries(np.tile(np.arange(len(PR.MASS_MESONS)), len(P)), rep(0), rep(4), rep(5), 0.0,
+                                rep(2), rep(3), rep(1), rep(1), rep(6), rep(7))
