@@ -13,6 +13,7 @@ Replaces:
   polarization_aniso(channel, k0, k_norm, ...)      src/relaxtime/PolarizationAniso.jl:22-24
   ThermoDerivatives (mass / thermo / bulk-viscosity derivatives)  ThermoDerivatives.jl:132-466
   calculate_propagator_by_channel(process, channel, ...)  src/relaxtime/TotalPropagator.jl:894-963
+  solve_meson_mass(meson, quark_params, thermo_params; ...)  src/relaxtime/MesonMass.jl:241-276
 """
 module PNJLHip
 
@@ -181,6 +182,40 @@ function calculate_propagator_by_channel(process::Symbol, channel::Symbol, k0::F
                                          quark_params, thermo_params; ctx::Context=default_context())
     DS, DP = propagator_by_channel_batch([process], [channel], [k0], [k_norm], quark_params, thermo_params; ctx=ctx)
     return DS[1], DP[1]
+end
+
+const MASS_MESONS = (:pi, :K, :sigma_pi, :sigma_K, :eta, :eta_prime, :sigma, :sigma_prime)
+
+struct MassQuery
+    meson::Cint; max_iter::Cint; reserved1::Cint; reserved2::Cint
+    k::Float64; mass0::Float64; gamma0::Float64; ftol::Float64; xtol::Float64
+    T::Float64; Phi::Float64; Phib::Float64; xi::Float64
+    m_u::Float64; m_s::Float64; mu_u::Float64; mu_s::Float64; A_u::Float64; A_s::Float64
+end
+
+"""
+    solve_meson_mass(meson, quark_params, thermo_params; k_norm=0.0, initial_mass=nothing,
+                     initial_gamma=0.0, ftol=1e-8, xtol=0.0, iterations=1000)
+
+Same arguments and (mass, gamma, converged, residual_norm) as the reference
+(quark_params must carry A; K_coeffs from it).  Replaces MesonMass.jl:241-276.
+"""
+function solve_meson_mass(meson::Symbol, quark_params, thermo_params; k_norm::Float64=0.0,
+                          initial_mass::Union{Nothing,Float64}=nothing, initial_gamma::Float64=0.0,
+                          ftol::Float64=1e-8, xtol::Float64=0.0, iterations::Integer=1000,
+                          ctx::Context=default_context())
+    qp, tp = quark_params, thermo_params
+    q = Ref(MassQuery(findfirst(==(meson), MASS_MESONS) - 1, iterations, 0, 0, k_norm,
+                      initial_mass === nothing ? NaN : initial_mass, initial_gamma, ftol, xtol,
+                      tp.T, tp.Φ, tp.Φbar, hasproperty(tp, :ξ) ? tp.ξ : 0.0,
+                      qp.m.u, qp.m.s, qp.μ.u, qp.μ.s, qp.A.u, qp.A.s))
+    out = zeros(6)
+    rc = ccall((:pnjl_meson_mass_batch, LIB), Cint,
+               (Ptr{Cvoid}, Int64, Ref{MassQuery}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+               ctx.h, 1, q, C_NULL, out, 0, C_NULL)
+    check(ctx, rc, "pnjl_meson_mass_batch")
+    out[6] == 1.0 && throw(ErrorException("solve_meson_mass: non-finite residual at the initial point"))
+    return (mass=out[1], gamma=out[2], converged=out[4] == 1.0, residual_norm=out[3])
 end
 
 """
