@@ -93,6 +93,9 @@ WORKLOADS = {
                                            "(T 20-300 MeV, mu 0-450 MeV) scan: k = 0, default guess, NLsolve defaults "
                                            "(trust region, central-difference Jacobian, ftol 1e-8, 1000 iterations); "
                                            "FP64"),
+    "dual_branch": dict(dual=True, desc="run_dual_branch_scan for 655 (T, xi) lines (T 20-150 MeV step 1 x xi -0.2..0.2 "
+                                        "step 0.1) over mu 0-450 MeV step 2 (226 points): hadron branch forward, "
+                                        "quark branch backward, continuation seeds, p24 t8, FP64"),
     "c5_multiroot": dict(grid=lambda: grid(512, 512, [-0.2, -0.1, 0.0, 0.1, 0.2], (10, 250), (200, 450)), p_num=64,
                          t_num=4, max_iter=40, learned=True, policy="MULTIROOT3", plain_policy="MULTI",
                          desc="512x512x5 (T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2 step 0.1), p=64 t=4, "
@@ -378,6 +381,81 @@ def run_masses(args, w):
     print(json.dumps(line), flush=True)
 
 
+DUAL_METRIC = "dual-branch scan points/s (run_dual_branch_scan, 655 lines x 226 mu x 2 branches)"
+
+
+def dual_lines():
+    T = np.repeat(np.arange(20.0, 150.0 + 1e-9, 1.0), 5)
+    xi = np.tile(np.round(np.arange(-0.2, 0.2001, 0.1), 10), T.size // 5)
+    return T, xi, np.arange(0.0, 450.0 + 1e-9, 2.0)
+
+
+def dual_cpu_baseline(n_lines=40, threads=None):
+    """The same dual-branch logic driven by the oracle's solves (C, OpenMP over
+    the lines of each mu step) on the first n_lines lines."""
+    from julia_relaxtime_amd import dual_branch as DB
+    from oracle import oracle as O
+    threads = threads or min(os.cpu_count() or 1, 16)
+    T, xi, mu = dual_lines()
+    pts = [0]
+
+    def solve(Tq, muq, xiq, policy, seed):
+        pts[0] += Tq.size
+        return O.solve_batch(Tq, muq, xiq, policy=policy, seed=seed, p_num=24, t_num=8, nthreads=threads)
+    t0 = time.perf_counter()
+    DB.run_dual_branch_scan_batch(T[:n_lines], mu, xi[:n_lines], solver=solve)
+    dt = time.perf_counter() - t0
+    return dict(value=pts[0] / dt, unit="points/s", cores=threads, kind="port",
+                sample=f"the first {n_lines} lines ({pts[0]} branch points), oracle solves with {threads} OpenMP "
+                       f"threads per mu step, {dt:.2f} s wall")
+
+
+def run_dual(args, w):
+    import torch
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import dual_branch as DB
+    from julia_relaxtime_amd.pnjl import context
+    ctx = context(0)
+    T, xi, mu = dual_lines()
+    gsolve = DB.gpu_solver(0, 24, 8)
+    stats = dict(ms=0.0, passes=0.0, points=0, launches=0)
+    t_eff = lambda x: np.where(x == 0.0, 1, 8)
+
+    def solve(Tq, muq, xiq, policy, seed):
+        r = gsolve(Tq, muq, xiq, policy, seed)
+        stats["ms"] += ctx.last_kernel_ms()
+        stats["passes"] += float(np.sum(r["passes"] * 24.0 * t_eff(xiq)))
+        stats["points"] += Tq.size
+        stats["launches"] += 1
+        return r
+    res = {}
+
+    def step():
+        for k in stats:
+            stats[k] = 0
+        res["r"] = DB.run_dual_branch_scan_batch(T, mu, xi, solver=solve)
+        return stats["ms"]
+
+    ms_step, kms = timed_steps(step, args.steps, args.warmup)
+    found = sum(DB.find_phase_transition(r)["found"] for r in res["r"])
+    kavg = float(np.mean(kms))
+    achieved = stats["passes"] * C_NODE / (kavg * 1e-3) / 1e12
+    line = {
+        "metric": DUAL_METRIC, "value": stats["points"] / (ms_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(workload=args.workload, lines=int(T.size), mu_points=int(mu.size), branch_points=stats["points"],
+                       launches=stats["launches"], first_order_lines=int(found), detail=w["desc"],
+                       kernel_ms_per_step=kavg),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "pnjl::scan_kernel",
+                     "kernel_ms_per_step": kavg, "flops_per_node": C_NODE},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = dual_cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
 def timed_steps(step, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -420,6 +498,8 @@ def main():
         return run_propagators(args, w)
     if w.get("masses"):
         return run_masses(args, w)
+    if w.get("dual"):
+        return run_dual(args, w)
     T, mu, xi = w["grid"]()
     n = T.size
     dev = torch.device("cuda", 0)
