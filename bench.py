@@ -96,6 +96,9 @@ WORKLOADS = {
     "dual_branch": dict(dual=True, desc="run_dual_branch_scan for 655 (T, xi) lines (T 20-150 MeV step 1 x xi -0.2..0.2 "
                                         "step 0.1) over mu 0-450 MeV step 2 (226 points): hadron branch forward, "
                                         "quark branch backward, continuation seeds, p24 t8, FP64"),
+    "tmu_scan": dict(tmu=True, desc="run_tmu_scan over T 20-250 MeV step 1 (231 lines) x mu 0-450 MeV step 2 "
+                                      "(226 points), xi = 0: phase-aware / continuation / hadron / quark candidates, "
+                                      "refine and promote, p24 t8, FP64"),
     "c5_multiroot": dict(grid=lambda: grid(512, 512, [-0.2, -0.1, 0.0, 0.1, 0.2], (10, 250), (200, 450)), p_num=64,
                          t_num=4, max_iter=40, learned=True, policy="MULTIROOT3", plain_policy="MULTI",
                          desc="512x512x5 (T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2 step 0.1), p=64 t=4, "
@@ -456,6 +459,78 @@ def run_dual(args, w):
     print(json.dumps(line), flush=True)
 
 
+TMU_METRIC = "T-mu scan points/s (run_tmu_scan, 231 T x 226 mu)"
+
+
+def tmu_grid():
+    return [20.0 + i for i in range(231)], [2.0 * i for i in range(226)]
+
+
+def tmu_cpu_baseline(n_lines=24, threads=None):
+    """The same TmuScan logic over the oracle's fixed-seed solves (C, OpenMP
+    over the lines of each mu step) on the first n_lines temperatures."""
+    from julia_relaxtime_amd import tmu_scan as TS
+    from oracle import oracle as O
+    threads = threads or min(os.cpu_count() or 1, 16)
+    Ts, mus = tmu_grid()
+
+    def solve(T, mu, xi, seeds):
+        return O.solve_batch(T, mu, xi, policy=O.SEED_EXPLICIT, seed=seeds, p_num=24, t_num=8, nthreads=threads)
+    t0 = time.perf_counter()
+    st = TS.run_tmu_scan(T_values=Ts[:n_lines], mu_values=mus, solver=solve)
+    dt = time.perf_counter() - t0
+    return dict(value=st["total"] / dt, unit="points/s", cores=threads, kind="port",
+                sample=f"the first {n_lines} temperatures ({st['total']} points), oracle solves with {threads} "
+                       f"OpenMP threads per mu step, {dt:.2f} s wall")
+
+
+def run_tmu(args, w):
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import tmu_scan as TS
+    from julia_relaxtime_amd.pnjl import context
+    ctx = context(0)
+    Ts, mus = tmu_grid()
+    opts = _lib.default_opts()
+    stats = dict(ms=0.0, passes=0.0, launches=0)
+
+    def solve(T, mu, xi, seeds):
+        saved = ctx.launch_cfg()
+        ctx.set_launch_cfg(lanes_per_point=32, chunk=1, waves_per_cu=8, tail_lanes_per_point=32)
+        try:
+            r = ctx.solve_batch(T, mu, xi, policy=_lib.SEED_EXPLICIT, seed=seeds, p_num=24, t_num=8, opts=opts)
+        finally:
+            ctx.set_launch_cfg(**saved)
+        stats["ms"] += ctx.last_kernel_ms()
+        stats["passes"] += float(np.sum(r["passes"])) * 24.0
+        stats["launches"] += 1
+        return r
+    res = {}
+
+    def step():
+        for k in stats:
+            stats[k] = 0
+        res["st"] = TS.run_tmu_scan(T_values=Ts, mu_values=mus, solver=solve)
+        return stats["ms"]
+
+    ms_step, kms = timed_steps(step, args.steps, args.warmup)
+    st = res["st"]
+    kavg = float(np.mean(kms))
+    achieved = stats["passes"] * C_NODE / (kavg * 1e-3) / 1e12
+    line = {
+        "metric": TMU_METRIC, "value": st["total"] / (ms_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(workload=args.workload, points=st["total"], success=st["success"], failure=st["failure"],
+                       launches=stats["launches"], detail=w["desc"], kernel_ms_per_step=kavg),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "pnjl::scan_kernel",
+                     "kernel_ms_per_step": kavg, "flops_per_node": C_NODE},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = tmu_cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
 def timed_steps(step, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -500,6 +575,8 @@ def main():
         return run_masses(args, w)
     if w.get("dual"):
         return run_dual(args, w)
+    if w.get("tmu"):
+        return run_tmu(args, w)
     T, mu, xi = w["grid"]()
     n = T.size
     dev = torch.device("cuda", 0)
