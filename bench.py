@@ -96,6 +96,9 @@ WORKLOADS = {
     "dual_branch": dict(dual=True, desc="run_dual_branch_scan for 655 (T, xi) lines (T 20-150 MeV step 1 x xi -0.2..0.2 "
                                         "step 0.1) over mu 0-450 MeV step 2 (226 points): hadron branch forward, "
                                         "quark branch backward, continuation seeds, p24 t8, FP64"),
+    "maxwell_line": dict(maxwell=True, desc="S-curves (hadron and quark branches followed in mu, the unstable "
+                         "branch from 7 starts between them) and Maxwell construction for 45 (T, xi) lines below the "
+                         "CEP (xi 0, 0.2, 0.4; T 50 MeV up in 5 MeV steps) x mu 250-420 MeV step 0.25, p24 t8, FP64"),
     "tmu_scan": dict(tmu=True, desc="run_tmu_scan over T 20-250 MeV step 1 (231 lines) x mu 0-450 MeV step 2 "
                                       "(226 points), xi = 0: phase-aware / continuation / hadron / quark candidates, "
                                       "refine and promote, p24 t8, FP64"),
@@ -531,6 +534,86 @@ def run_tmu(args, w):
     print(json.dumps(line), flush=True)
 
 
+MAXWELL_METRIC = "Maxwell-line S-curve points/s (s_curves + maxwell_construction, 45 lines x 681 mu)"
+
+
+def maxwell_lines():
+    """(T, xi) lines below each xi's CEP: T 50 MeV up in 5 MeV steps (130,
+    120, 110 MeV at xi 0, 0.2, 0.4), mu 250-420 MeV in 0.25 MeV steps."""
+    T, xi = [], []
+    for x, tmax in ((0.0, 130.0), (0.2, 120.0), (0.4, 110.0)):
+        for t in np.arange(50.0, tmax + 1e-9, 5.0):
+            T.append(t)
+            xi.append(x)
+    return np.array(T), np.array(xi), np.arange(250.0, 420.0 + 1e-9, 0.25)
+
+
+def maxwell_cpu_baseline(n_lines=45, threads=None):
+    """The same S-curve + Maxwell logic over the oracle's solves (C, OpenMP
+    over the lines of each mu step) for the first n_lines lines."""
+    from julia_relaxtime_amd import phase_transition as PT
+    from oracle import oracle as O
+    threads = threads or min(os.cpu_count() or 1, 16)
+    T, xi, mu = maxwell_lines()
+    count = dict(points=0)
+
+    def solve(Tq, muq, xiq, policy, seed):
+        count["points"] += Tq.size
+        return O.solve_batch(Tq, muq, xiq, policy=policy, seed=seed, p_num=24, t_num=8, nthreads=threads)
+    t0 = time.perf_counter()
+    for mu_c, rho in PT.s_curves(T[:n_lines], mu, xi[:n_lines], solver=solve):
+        PT.maxwell_construction(mu_c, rho, min_samples=8, detect_min_points=5)
+    dt = time.perf_counter() - t0
+    return dict(value=count["points"] / dt, unit="points/s", cores=threads, kind="port",
+                sample=f"the first {n_lines} lines ({count['points']} FixedMu solves), oracle solves with {threads} "
+                       f"OpenMP threads per batch, {dt:.2f} s wall")
+
+
+def run_maxwell(args, w):
+    from julia_relaxtime_amd import dual_branch as DB
+    from julia_relaxtime_amd import phase_transition as PT
+    from julia_relaxtime_amd.pnjl import context
+    ctx = context(0)
+    T, xi, mu = maxwell_lines()
+    gsolve = DB.gpu_solver(0, 24, 8)
+    stats = dict(ms=0.0, passes=0.0, points=0, launches=0)
+    t_eff = lambda x: np.where(x == 0.0, 1, 8)
+
+    def solve(Tq, muq, xiq, policy, seed):
+        r = gsolve(Tq, muq, xiq, policy, seed)
+        stats["ms"] += ctx.last_kernel_ms()
+        stats["passes"] += float(np.sum(r["passes"] * 24.0 * t_eff(xiq)))
+        stats["points"] += Tq.size
+        stats["launches"] += 1
+        return r
+    res = {}
+
+    def step():
+        for k in stats:
+            stats[k] = 0
+        curves = PT.s_curves(T, mu, xi, solver=solve)
+        res["m"] = [PT.maxwell_construction(m, r, min_samples=8, detect_min_points=5) for m, r in curves]
+        return stats["ms"]
+
+    ms_step, kms = timed_steps(step, args.steps, args.warmup)
+    kavg = float(np.mean(kms))
+    achieved = stats["passes"] * C_NODE / (kavg * 1e-3) / 1e12
+    line = {
+        "metric": MAXWELL_METRIC, "value": stats["points"] / (ms_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(workload=args.workload, lines=int(T.size), mu_points=int(mu.size), solves=stats["points"],
+                       launches=stats["launches"], maxwell_converged=int(sum(m.converged for m in res["m"])),
+                       detail=w["desc"], kernel_ms_per_step=kavg),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "pnjl::scan_kernel",
+                     "kernel_ms_per_step": kavg, "flops_per_node": C_NODE},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = maxwell_cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
 def timed_steps(step, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -575,6 +658,8 @@ def main():
         return run_masses(args, w)
     if w.get("dual"):
         return run_dual(args, w)
+    if w.get("maxwell"):
+        return run_maxwell(args, w)
     if w.get("tmu"):
         return run_tmu(args, w)
     T, mu, xi = w["grid"]()

@@ -10,8 +10,9 @@ Sources (paths relative to the reference root):
       PhaseAwareContinuitySeed chains, p12/t6, full Float64 precision)
   data/outputs/results/pnjl/tmu_scan.csv               (TmuScan, 6 decimals)
   data/reference/pnjl/trho_scan_xi{0.0,0.2,0.4}.csv    (TrhoScan, p24/t8, 6 decimals): the
-      rows at T = 50, 80, 110 MeV, rho > 0, converged -- FixedRho states, which are FixedMu
-      roots at mu = mu_avg on all three branches of the first-order S-curve
+      rows at T = 50, 80, 110 MeV, converged (the curves calculate_phase_structure.jl's
+      load_curves_from_csv feeds to maxwell_construction) -- FixedRho states, which are
+      FixedMu roots at mu = mu_avg on all three branches of the first-order S-curve
   data/reference/pnjl/spinodals.csv, boundary.csv      (calculate_phase_structure.jl: S-curve
       spinodals and the Maxwell-construction transition line, xi = 0, 0.2, 0.4)
   data/outputs/results/pnjl/bulk_viscosity_xi0.0.csv   (calculate_derivatives.jl, ThermoDerivatives
@@ -58,7 +59,7 @@ def main(ref):
     trho = []
     for xi in ["0.0", "0.2", "0.4"]:
         for r in rows_of(os.path.join(ref, f"data/reference/pnjl/trho_scan_xi{xi}.csv")):
-            if (r["converged"] == "true" and float(r["rho"]) > 0
+            if (r["converged"] == "true" and float(r["rho"]) >= 0
                     and round(float(r["T_MeV"])) in (50, 80, 110)):
                 trho.append(r)
     write("trho_scan_T50_80_110.csv",
