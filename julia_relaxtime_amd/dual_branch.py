@@ -11,8 +11,9 @@ residual, MultiSeed without a root) or jumps (|d phi_u| > 0.5 or
 (DualBranchScan.jl:104-205, :330-445).  `run_dual_branch_scan_batch` follows
 every line of a batch at once: each mu step is one batched solve on the GPU
 (the first point of a branch with the DefaultSeed(:hadron / :quark) policy,
-the next ones from the previous solution), so a batch of L lines costs 2 x
-len(mu) launches whatever L is.  The bookkeeping between steps (which lines
+the next ones from the previous solution); step k of the hadron branch (mu
+index k) and of the quark branch (index n-1-k) go into the same launch, so a
+batch of L lines costs len(mu) + 1 launches whatever L is.  The bookkeeping between steps (which lines
 are still followed, jump tests, the merge) is host logic on the GPU results.
 """
 from __future__ import annotations
@@ -49,36 +50,53 @@ def gpu_solver(device=0, p_num=24, t_num=8, opts=None, lanes_per_point=32):
     return solve
 
 
-def _follow(solve, T, xi, mu_fm, order, first_policy):
-    """One branch for every line: arrays (L, n, ...) and the valid mask."""
+def _follow_both(solve, T, xi, mu_fm):
+    """Both branches of every line: the hadron branch over mu ascending and the
+    quark branch over mu descending.  Step k of the two is independent (mu
+    index k and n-1-k), so after the first points (HADRON / QUARK policies)
+    each step is ONE batched solve of the active lines of both branches.
+    Returns two dicts of arrays (L, n, ...) with their `valid` masks."""
     L, n = T.size, mu_fm.size
-    br = {k: np.full((L, n, 5) if k == "x" else (L, n, 3) if k == "masses" else (L, n), np.nan) for k in _FIELDS}
-    valid = np.zeros((L, n), dtype=bool)
-    active = np.ones(L, dtype=bool)
-    prev_x, prev_m = np.zeros((L, 5)), np.zeros(L)
-    first = True
-    for i in order:
-        idx = np.flatnonzero(active)
-        if idx.size == 0:
+    shape = lambda k: (L, n, 5) if k == "x" else (L, n, 3) if k == "masses" else (L, n)
+    brs = [{k: np.full(shape(k), np.nan) for k in _FIELDS} for _ in range(2)]
+    valid = [np.zeros((L, n), dtype=bool) for _ in range(2)]
+    active = [np.ones(L, dtype=bool) for _ in range(2)]
+    prev_x = [np.zeros((L, 5)) for _ in range(2)]
+    prev_m = [np.zeros(L) for _ in range(2)]
+    for k in range(n):
+        cols = (k, n - 1 - k)
+        idx = [np.flatnonzero(a) for a in active]
+        if idx[0].size + idx[1].size == 0:
             break
-        if first:
-            r = solve(T[idx], np.full(idx.size, mu_fm[i]), xi[idx], first_policy, None)
+        if k == 0:
+            rs = [solve(T[idx[b]], np.full(idx[b].size, mu_fm[cols[b]]), xi[idx[b]], pol, None) if idx[b].size
+                  else None for b, pol in ((0, _lib.SEED_HADRON), (1, _lib.SEED_QUARK))]
         else:
-            r = solve(T[idx], np.full(idx.size, mu_fm[i]), xi[idx], _lib.SEED_EXPLICIT, prev_x[idx])
-        ok = (r["converged"] == 1) & ((r["flags"] & (_lib.PT_EXCEPTION | _lib.PT_NO_ROOT)) == 0)
-        if not first:   # _is_solution_jump  DualBranchScan.jl:410-415
-            jump = (np.abs(prev_x[idx, 0] - r["x"][:, 0]) > 0.5) | (np.abs(prev_m[idx] - r["masses"][:, 0]) * HBARC > 50.0)
-            ok &= ~jump
-        keep = idx[ok]
-        for k in _FIELDS:
-            br[k][keep, i] = r[k][ok]
-        valid[keep, i] = True
-        prev_x[keep] = r["x"][ok]
-        prev_m[keep] = r["masses"][ok, 0]
-        active[idx[~ok]] = False
-        first = False
-    br["valid"] = valid
-    return br
+            r = solve(np.concatenate([T[idx[0]], T[idx[1]]]),
+                      np.concatenate([np.full(idx[0].size, mu_fm[cols[0]]), np.full(idx[1].size, mu_fm[cols[1]])]),
+                      np.concatenate([xi[idx[0]], xi[idx[1]]]), _lib.SEED_EXPLICIT,
+                      np.concatenate([prev_x[0][idx[0]], prev_x[1][idx[1]]]))
+            cut = idx[0].size
+            rs = [{f: v[:cut] for f, v in r.items()}, {f: v[cut:] for f, v in r.items()}]
+        for b in range(2):
+            if idx[b].size == 0:
+                continue
+            r, i = rs[b], cols[b]
+            ok = (r["converged"] == 1) & ((r["flags"] & (_lib.PT_EXCEPTION | _lib.PT_NO_ROOT)) == 0)
+            if k > 0:   # _is_solution_jump  DualBranchScan.jl:410-415
+                jump = (np.abs(prev_x[b][idx[b], 0] - r["x"][:, 0]) > 0.5) | \
+                    (np.abs(prev_m[b][idx[b]] - r["masses"][:, 0]) * HBARC > 50.0)
+                ok &= ~jump
+            keep = idx[b][ok]
+            for f in _FIELDS:
+                brs[b][f][keep, i] = r[f][ok]
+            valid[b][keep, i] = True
+            prev_x[b][keep] = r["x"][ok]
+            prev_m[b][keep] = r["masses"][ok, 0]
+            active[b][idx[b][~ok]] = False
+    for b in range(2):
+        brs[b]["valid"] = valid[b]
+    return brs[0], brs[1]
 
 
 def _same_solution(h, q, tol=0.01):
@@ -101,8 +119,7 @@ def run_dual_branch_scan_batch(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts
     mu_fm = mu_values / HBARC
     solve = solver or gpu_solver(device, p_num, t_num, opts)
     n = mu_values.size
-    had = _follow(solve, T, xi, mu_fm, range(n), _lib.SEED_HADRON)
-    qrk = _follow(solve, T, xi, mu_fm, range(n - 1, -1, -1), _lib.SEED_QUARK)
+    had, qrk = _follow_both(solve, T, xi, mu_fm)
     # _select_physical_branch  DualBranchScan.jl:378-398
     both = had["valid"] & qrk["valid"]
     same = both & _same_solution(had, qrk)
