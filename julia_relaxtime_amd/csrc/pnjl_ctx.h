@@ -34,6 +34,8 @@ struct pnjl_ctx {
     double jlo[3] = {0, 0, 0}, jhi[3] = {1, 1, 1};
     std::vector<void*> stage;   // device staging buffers for host-pointer calls
     std::vector<size_t> stage_bytes;
+    void* pinned = nullptr;     // page-locked host staging of the packed host-pointer solves
+    size_t pinned_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     // begin/end events of each persistent-kernel launch of the last solve

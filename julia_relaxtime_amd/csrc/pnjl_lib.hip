@@ -3,6 +3,7 @@
 // queue counter, the per-group scratch, the phase-boundary tables and the
 // launch geometry of a context; launches the kernels of pnjl_scan_kernels.h.
 #include <hip/hip_runtime.h>
+#include <cstring>
 
 #include <math.h>
 #include <stdio.h>
@@ -161,6 +162,20 @@ int stage_buf(pnjl_ctx* ctx, size_t k, size_t bytes, void** out)
         ctx->stage_bytes[k] = bytes;
     }
     *out = ctx->stage[k];
+    return PNJL_OK;
+}
+
+// page-locked host staging, grown on demand (hipHostMalloc: allocated, never registered)
+int pinned_buf(pnjl_ctx* ctx, size_t bytes, void** out)
+{
+    if (ctx->pinned_bytes < bytes) {
+        if (ctx->pinned) HIPCHK(ctx, hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+        HIPCHK(ctx, hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
+        ctx->pinned_bytes = bytes;
+    }
+    *out = ctx->pinned;
     return PNJL_OK;
 }
 
@@ -495,6 +510,7 @@ void pnjl_ctx_destroy(pnjl_ctx* c)
     for (auto& kv : c->tabs) (void)hipFree(kv.second);
     for (void* b : c->stage)
         if (b) (void)hipFree(b);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->queue) (void)hipFree(c->queue);
     if (c->bnd) (void)hipFree(c->bnd);
@@ -669,54 +685,70 @@ static int solve_impl(pnjl_ctx* ctx, int64_t n, const double* T, const double* m
         rc = solve_dev(ctx, n, T, mu, xi, seed, policy, pn, tn, &o, out, cfg, st);
         if (rc) return rc;
     } else {
-        // stage host arrays into HBM
-        size_t nd = (size_t)n * sizeof(double);
-        void* b[19];
-        size_t sizes[] = {nd, nd, nd, 5 * nd, 5 * nd, nd, nd, nd, nd, nd, 3 * nd, 3 * nd, 3 * nd, nd,
-                          (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4};
-        for (int k = 0; k < 19; k++) {
-            rc = stage_buf(ctx, k, sizes[k], &b[k]);
-            if (rc) return rc;
-        }
-        HIPCHK(ctx, hipMemcpyAsync(b[0], T, nd, hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(b[1], mu, nd, hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(b[2], xi, nd, hipMemcpyHostToDevice, st));
-        if (seed) HIPCHK(ctx, hipMemcpyAsync(b[3], seed, 5 * nd, hipMemcpyHostToDevice, st));
+        // Host arrays travel packed: the inputs are gathered into one page-locked
+        // buffer and go to HBM in one copy; the kernels write the outputs into
+        // one device block, which comes back in one copy and is scattered into
+        // the caller's arrays (two DMA transfers per call instead of ~19
+        // pageable ones).  Layout (in units of n values): inputs T, mu, xi,
+        // seed(5); outputs x(5), omega, pressure, rho_norm, entropy, energy,
+        // masses(3), n_q(3), n_qbar(3), residual_norm as doubles, then
+        // iterations, converged, flags, newton_steps, passes as int32.
+        const size_t nd = (size_t)n * sizeof(double), ni = (size_t)n * 4;
+        const size_t in_bytes = (seed ? 8 : 3) * nd;
+        const size_t out_d = 20 * nd, out_bytes = out_d + 5 * ni;
+        void *din = nullptr, *dout = nullptr, *hp = nullptr;
+        if ((rc = stage_buf(ctx, 0, in_bytes, &din))) return rc;
+        if ((rc = stage_buf(ctx, 1, out_bytes, &dout))) return rc;
+        if ((rc = pinned_buf(ctx, std::max(in_bytes, out_bytes), &hp))) return rc;
+        // the pinned buffer may still be read by the previous call's copy only
+        // if that call was asynchronous, which host-pointer calls never are
+        char* h = (char*)hp;
+        std::memcpy(h, T, nd);
+        std::memcpy(h + nd, mu, nd);
+        std::memcpy(h + 2 * nd, xi, nd);
+        if (seed) std::memcpy(h + 3 * nd, seed, 5 * nd);
+        HIPCHK(ctx, hipMemcpyAsync(din, h, in_bytes, hipMemcpyHostToDevice, st));
+        double* di = (double*)din;
+        double* dd = (double*)dout;
+        int32_t* dj = (int32_t*)((char*)dout + out_d);
         pnjl_solve_out d;
-        d.x = (double*)b[4];
-        d.omega = (double*)b[5];
-        d.pressure = (double*)b[6];
-        d.rho_norm = (double*)b[7];
-        d.entropy = (double*)b[8];
-        d.energy = (double*)b[9];
-        d.masses = (double*)b[10];
-        d.n_q = out->n_q ? (double*)b[11] : nullptr;
-        d.n_qbar = out->n_qbar ? (double*)b[12] : nullptr;
-        d.residual_norm = (double*)b[13];
-        d.iterations = (int32_t*)b[14];
-        d.converged = (int32_t*)b[15];
-        d.flags = out->flags ? (int32_t*)b[16] : nullptr;
-        d.newton_steps = out->newton_steps ? (int32_t*)b[17] : nullptr;
-        d.passes = out->passes ? (int32_t*)b[18] : nullptr;
-        rc = solve_dev(ctx, n, (double*)b[0], (double*)b[1], (double*)b[2], seed ? (double*)b[3] : nullptr, policy,
-                       pn, tn, &o, &d, cfg, st);
+        d.x = dd;
+        d.omega = dd + 5 * n;
+        d.pressure = dd + 6 * n;
+        d.rho_norm = dd + 7 * n;
+        d.entropy = dd + 8 * n;
+        d.energy = dd + 9 * n;
+        d.masses = dd + 10 * n;
+        d.n_q = out->n_q ? dd + 13 * n : nullptr;
+        d.n_qbar = out->n_qbar ? dd + 16 * n : nullptr;
+        d.residual_norm = dd + 19 * n;
+        d.iterations = dj;
+        d.converged = dj + n;
+        d.flags = out->flags ? dj + 2 * n : nullptr;
+        d.newton_steps = out->newton_steps ? dj + 3 * n : nullptr;
+        d.passes = out->passes ? dj + 4 * n : nullptr;
+        rc = solve_dev(ctx, n, di, di + n, di + 2 * n, seed ? di + 3 * n : nullptr, policy, pn, tn, &o, &d, cfg,
+                       st);
         if (rc) return rc;
-        HIPCHK(ctx, hipMemcpyAsync(out->x, d.x, 5 * nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->omega, d.omega, nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->pressure, d.pressure, nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->rho_norm, d.rho_norm, nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->entropy, d.entropy, nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->energy, d.energy, nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->masses, d.masses, 3 * nd, hipMemcpyDeviceToHost, st));
-        if (out->n_q) HIPCHK(ctx, hipMemcpyAsync(out->n_q, d.n_q, 3 * nd, hipMemcpyDeviceToHost, st));
-        if (out->n_qbar) HIPCHK(ctx, hipMemcpyAsync(out->n_qbar, d.n_qbar, 3 * nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->residual_norm, d.residual_norm, nd, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->iterations, d.iterations, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync(out->converged, d.converged, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-        if (out->flags) HIPCHK(ctx, hipMemcpyAsync(out->flags, d.flags, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-        if (out->newton_steps)
-            HIPCHK(ctx, hipMemcpyAsync(out->newton_steps, d.newton_steps, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-        if (out->passes) HIPCHK(ctx, hipMemcpyAsync(out->passes, d.passes, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(h, dout, out_bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        const double* hd = (const double*)h;
+        const int32_t* hj = (const int32_t*)(h + out_d);
+        std::memcpy(out->x, hd, 5 * nd);
+        std::memcpy(out->omega, hd + 5 * n, nd);
+        std::memcpy(out->pressure, hd + 6 * n, nd);
+        std::memcpy(out->rho_norm, hd + 7 * n, nd);
+        std::memcpy(out->entropy, hd + 8 * n, nd);
+        std::memcpy(out->energy, hd + 9 * n, nd);
+        std::memcpy(out->masses, hd + 10 * n, 3 * nd);
+        if (out->n_q) std::memcpy(out->n_q, hd + 13 * n, 3 * nd);
+        if (out->n_qbar) std::memcpy(out->n_qbar, hd + 16 * n, 3 * nd);
+        std::memcpy(out->residual_norm, hd + 19 * n, nd);
+        std::memcpy(out->iterations, hj, ni);
+        std::memcpy(out->converged, hj + n, ni);
+        if (out->flags) std::memcpy(out->flags, hj + 2 * n, ni);
+        if (out->newton_steps) std::memcpy(out->newton_steps, hj + 3 * n, ni);
+        if (out->passes) std::memcpy(out->passes, hj + 4 * n, ni);
     }
     if (!(flags & PNJL_FLAG_ASYNC) || !(flags & PNJL_FLAG_DEVICE_PTRS)) {
         HIPCHK(ctx, hipStreamSynchronize(st));
