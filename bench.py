@@ -423,15 +423,15 @@ def run_dual(args, w):
     from julia_relaxtime_amd.pnjl import context
     ctx = context(0)
     T, xi, mu = dual_lines()
-    gsolve = DB.gpu_solver(0, 24, 8)
+    gsolve = DB.gpu_chain_solver(0, 24, 8)
     stats = dict(ms=0.0, passes=0.0, points=0, launches=0)
     t_eff = lambda x: np.where(x == 0.0, 1, 8)
 
-    def solve(Tq, muq, xiq, policy, seed):
-        r = gsolve(Tq, muq, xiq, policy, seed)
+    def solve(Tq, muq, xiq, policy, chain_len):
+        r = gsolve(Tq, muq, xiq, policy, chain_len)
         stats["ms"] += ctx.last_kernel_ms()
         stats["passes"] += float(np.sum(r["passes"] * 24.0 * t_eff(xiq)))
-        stats["points"] += Tq.size
+        stats["points"] += int(np.sum((r["flags"] & _lib.PT_SKIPPED) == 0))
         stats["launches"] += 1
         return r
     res = {}
@@ -439,7 +439,7 @@ def run_dual(args, w):
     def step():
         for k in stats:
             stats[k] = 0
-        res["r"] = DB.run_dual_branch_scan_batch(T, mu, xi, solver=solve)
+        res["r"] = DB.run_dual_branch_scan_batch(T, mu, xi, chain_solver=solve)
         return stats["ms"]
 
     ms_step, kms = timed_steps(step, args.steps, args.warmup)
@@ -570,28 +570,34 @@ def maxwell_cpu_baseline(n_lines=45, threads=None):
 
 
 def run_maxwell(args, w):
+    from julia_relaxtime_amd import _lib
     from julia_relaxtime_amd import dual_branch as DB
     from julia_relaxtime_amd import phase_transition as PT
     from julia_relaxtime_amd.pnjl import context
     ctx = context(0)
     T, xi, mu = maxwell_lines()
-    gsolve = DB.gpu_solver(0, 24, 8)
+    gsolve, gchain = DB.gpu_solver(0, 24, 8), DB.gpu_chain_solver(0, 24, 8)
     stats = dict(ms=0.0, passes=0.0, points=0, launches=0)
     t_eff = lambda x: np.where(x == 0.0, 1, 8)
 
-    def solve(Tq, muq, xiq, policy, seed):
-        r = gsolve(Tq, muq, xiq, policy, seed)
+    def count(r, xiq):
         stats["ms"] += ctx.last_kernel_ms()
         stats["passes"] += float(np.sum(r["passes"] * 24.0 * t_eff(xiq)))
-        stats["points"] += Tq.size
+        stats["points"] += int(np.sum((r["flags"] & _lib.PT_SKIPPED) == 0))
         stats["launches"] += 1
         return r
+
+    def solve(Tq, muq, xiq, policy, seed):
+        return count(gsolve(Tq, muq, xiq, policy, seed), xiq)
+
+    def chains(Tq, muq, xiq, policy, chain_len):
+        return count(gchain(Tq, muq, xiq, policy, chain_len), xiq)
     res = {}
 
     def step():
         for k in stats:
             stats[k] = 0
-        curves = PT.s_curves(T, mu, xi, solver=solve)
+        curves = PT.s_curves(T, mu, xi, chain_solver=chains, point_solver=solve)
         res["m"] = [PT.maxwell_construction(m, r, min_samples=8, detect_min_points=5) for m, r in curves]
         return stats["ms"]
 

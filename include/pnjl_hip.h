@@ -111,6 +111,18 @@ int pnjl_gauleg(double a, double b, int n, double* nodes, double* weights);
                                        (scripts/relaxtime/run_gap_transport_scan.jl:410-442) */
 #define PNJL_SEED_SURROGATE      7  /* learned seed; plain seeds as fallback  */
 #define PNJL_SEED_MULTIROOT3     8  /* surrogate + HADRON + QUARK, min Omega  */
+/* DualBranchScan branches as chains (DualBranchScan.jl:104-169, :330-445):
+ * the first point of each chain is solved from DefaultSeed(:hadron) (resp.
+ * :quark), every next one from the previous solution; a chain ENDS at its
+ * first point that does not converge, raises (PNJL_PT_EXCEPTION /
+ * PNJL_PT_NO_ROOT) or jumps (|phi_u - previous phi_u| > 0.5 or |M_u -
+ * previous M_u| > 50 MeV): that point is reported with PNJL_PT_BRANCH_END
+ * and the points after it are not solved (PNJL_PT_SKIPPED, converged 0,
+ * NaN values).  Callers lay out each chain in the branch's mu order
+ * (ascending for the hadron branch, descending for the quark branch) and set
+ * pnjl_solve_opts.chain_len.                                                */
+#define PNJL_SEED_BRANCH_HADRON  9
+#define PNJL_SEED_BRANCH_QUARK  10
 
 typedef struct pnjl_solve_opts {
     double xtol;                  /* NLsolve xtol (reference 1e-9)           */
@@ -184,6 +196,8 @@ typedef struct pnjl_solve_out {
 #define PNJL_PT_CHAIN_RESET    0x80  /* chain: phase switch re-seeded the point */
 #define PNJL_PT_SURR_MASKED    0x100 /* in the judge mask: the reference policy
                                         solved the point (PNJL_JUDGE_REFERENCE) */
+#define PNJL_PT_BRANCH_END     0x200 /* branch chain: this point ended it      */
+#define PNJL_PT_SKIPPED        0x400 /* branch chain: after the end, not solved */
 
 /* Solves n FixedMu points in one persistent-kernel launch.
  * T, mu, xi: n values each.  seed: n*5 (EXPLICIT) or NULL.

@@ -20,11 +20,13 @@ FLAG_DEVICE_PTRS, FLAG_ASYNC, FLAG_ONE_PER_WAVE = 0x1, 0x2, 0x4
 # seed policies
 SEED_DEFAULT, SEED_HADRON, SEED_QUARK, SEED_MULTI, SEED_EXPLICIT = 0, 1, 2, 3, 4
 SEED_CONTINUITY, SEED_PHASE_AWARE, SEED_SURROGATE, SEED_MULTIROOT3 = 5, 6, 7, 8
+SEED_BRANCH_HADRON, SEED_BRANCH_QUARK = 9, 10
 
 # per-point flags
 PT_TR_USED, PT_MULTISEED, PT_EXCEPTION, PT_NO_ROOT = 0x01, 0x02, 0x04, 0x08
 PT_SURR_FALLBACK, PT_SURR_OMEGA, PT_FIRST_ORDER, PT_CHAIN_RESET = 0x10, 0x20, 0x40, 0x80
 PT_SURR_MASKED = 0x100
+PT_BRANCH_END, PT_SKIPPED = 0x200, 0x400
 
 # pnjl_thermo_derivatives_batch: per-point output columns (name, width)
 THERMO_DERIV_FIELDS = 36

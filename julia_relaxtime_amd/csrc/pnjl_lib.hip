@@ -200,7 +200,7 @@ void launch_scan(pnjl_ctx* ctx, const pnjl::ScanArgs& A, int lpp, int blocks, si
 
 int check_policy(int p)
 {
-    return (p >= PNJL_SEED_DEFAULT && p <= PNJL_SEED_MULTIROOT3) ? PNJL_OK : PNJL_ERR_ARG;
+    return (p >= PNJL_SEED_DEFAULT && p <= PNJL_SEED_BRANCH_QUARK) ? PNJL_OK : PNJL_ERR_ARG;
 }
 
 // core of pnjl_solve_fixedmu_batch with device pointers
@@ -216,7 +216,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     A.P = ctx->dp;
     A.n_points = n;
     A.policy = policy;
-    bool chains = (policy == PNJL_SEED_CONTINUITY || policy == PNJL_SEED_PHASE_AWARE);
+    bool chains = pnjl::chain_policy(policy);
     A.chain_len = chains ? (o->chain_len > 0 ? o->chain_len : (int)std::min<int64_t>(n, 1 << 30)) : 1;
     A.n_items = chains ? (n + A.chain_len - 1) / A.chain_len : n;
     A.T = T;

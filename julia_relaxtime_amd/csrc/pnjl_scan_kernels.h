@@ -798,7 +798,18 @@ struct St {
     bool have_best;
     bool have_prev;
     bool plan_multi;          // this point runs MultiSeed as its plan
+    double prev_mass;         // branch chain: M_u of the previous point
 };
+
+__host__ __device__ inline bool branch_policy(int p)
+{
+    return p == PNJL_SEED_BRANCH_HADRON || p == PNJL_SEED_BRANCH_QUARK;
+}
+
+__host__ __device__ inline bool chain_policy(int p)
+{
+    return p == PNJL_SEED_CONTINUITY || p == PNJL_SEED_PHASE_AWARE || branch_policy(p);
+}
 
 __device__ void start_attempt(St& st, const Scr& s, int attempt, const double x0[5])
 {
@@ -842,17 +853,49 @@ __device__ void write_point(const ScanArgs& A, const Scr& s, const St& st, int b
 
 // point finished with record `blk`: write outputs, update the chain tracker
 // (update! only after a converged solve, run_gap_transport_scan.jl:439-442,
-// tests/perf/pnjl/scan_perf.jl:69-72).
+// tests/perf/pnjl/scan_perf.jl:69-72).  A branch chain ends at its first
+// point that fails or jumps (_is_solution_jump, DualBranchScan.jl:410-415);
+// the rest of the chain is written as skipped and not solved.
 __device__ void finish_point(const ScanArgs& A, const Scr& s, St& st, int blk, int gl)
 {
-    write_point(A, s, st, blk, gl);
     bool ok = s.at(blk + RF_CONV) != 0.0 && !((int)(s.at(blk + RF_FLAGS)) & PNJL_PT_EXCEPTION) &&
               !(st.flags & PNJL_PT_NO_ROOT);
-    if ((A.policy == PNJL_SEED_CONTINUITY || A.policy == PNJL_SEED_PHASE_AWARE) && ok) {
+    const bool branch = branch_policy(A.policy);
+    if (branch && ok && st.have_prev) {
+        const double dphi = fabs(s.at(SB_PREV) - s.at(blk + RF_X));
+        const double dm = fabs(st.prev_mass - s.at(blk + RF_M)) * 197.327;
+        if (dphi > 0.5 || dm > 50.0) ok = false;
+    }
+    if (branch && !ok) st.flags |= PNJL_PT_BRANCH_END;
+    write_point(A, s, st, blk, gl);
+    if (chain_policy(A.policy) && ok) {
 #pragma unroll
         for (int i = 0; i < 5; i++) s.at(SB_PREV + i) = s.at(blk + RF_X + i);
         st.have_prev = true;
+        st.prev_mass = s.at(blk + RF_M);
         if (A.policy == PNJL_SEED_PHASE_AWARE) st.prev_phase = phase_of(A, st.xi, st.T * 197.327, st.mu * 197.327);
+    }
+    if (branch && !ok) {
+        if (gl == 0) {
+            for (int64_t i = st.pt + 1; i < st.pt_end; i++) {
+#pragma unroll
+                for (int k = 0; k < 5; k++) A.x_out[i * 5 + k] = qnan();
+                A.omega[i] = A.pressure[i] = A.rho_norm[i] = A.entropy[i] = A.energy[i] = qnan();
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    A.masses[i * 3 + k] = qnan();
+                    if (A.n_q) A.n_q[i * 3 + k] = qnan();
+                    if (A.n_qbar) A.n_qbar[i * 3 + k] = qnan();
+                }
+                A.resnorm[i] = qnan();
+                A.iterations[i] = 0;
+                A.converged[i] = 0;
+                if (A.flags) A.flags[i] = PNJL_PT_SKIPPED;
+                if (A.steps) A.steps[i] = 0;
+                if (A.passes) A.passes[i] = 0;
+            }
+        }
+        st.pt = st.pt_end - 1;      // next_point moves on to the next chain
     }
     st.phase = PH_IDLE;
 }
@@ -901,6 +944,16 @@ __device__ __noinline__ void begin_point(const ScanArgs& A, const Scr& s, St& st
             for (int k = 0; k < 5; k++) x0[k] = s.at(SB_PREV + k);
         } else {
             default_seed(0, st.T, st.mu, x0);
+        }
+        start_attempt(st, s, ATT_PRIMARY, x0);
+        break;
+    case PNJL_SEED_BRANCH_HADRON:
+    case PNJL_SEED_BRANCH_QUARK:
+        if (st.have_prev) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) x0[k] = s.at(SB_PREV + k);
+        } else {
+            default_seed(A.policy == PNJL_SEED_BRANCH_HADRON ? 1 : 2, st.T, st.mu, x0);
         }
         start_attempt(st, s, ATT_PRIMARY, x0);
         break;
@@ -1375,6 +1428,7 @@ __device__ __forceinline__ bool next_point(const ScanArgs& A, St& st, int gl, bo
         if (st.pt_end > A.n_points) st.pt_end = A.n_points;
         st.have_prev = false;
         st.prev_phase = 0;
+        st.prev_mass = 0.0;
     } else {
         st.pt = it;
         st.pt_end = it + 1;
@@ -1397,8 +1451,7 @@ __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPP;
     if (g >= A.n_groups) return;  // beyond the scratch the host allocated
     const Scr s{A.scratch + g * SCRATCH_FIELDS};
-    const bool chains = (A.stage == STAGE_NONE) &&
-                        (A.policy == PNJL_SEED_CONTINUITY || A.policy == PNJL_SEED_PHASE_AWARE);
+    const bool chains = (A.stage == STAGE_NONE) && chain_policy(A.policy);
     const int64_t n_items = (A.stage == STAGE_NONE) ? A.n_items
                             : (A.item_list ? (int64_t)(*A.item_count) : A.n_points) * A.mult;
     St st;
@@ -1409,6 +1462,7 @@ __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
     st.pt_end = 0;
     st.have_prev = false;
     st.prev_phase = 0;
+    st.prev_mass = 0.0;
     st.k = 0;
     st.item_ix = 0;
 

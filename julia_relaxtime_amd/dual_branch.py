@@ -50,6 +50,58 @@ def gpu_solver(device=0, p_num=24, t_num=8, opts=None, lanes_per_point=32):
     return solve
 
 
+def gpu_chain_solver(device=0, p_num=24, t_num=8, opts=None, lanes_per_point=32):
+    """Whole branches on the GPU: `solve(T, mu, xi, policy, chain_len)` runs
+    every chain of a PNJL_SEED_BRANCH_HADRON / _QUARK batch in one
+    persistent-kernel launch (each lane group follows its chain point after
+    point and stops at the branch end), with the context's launch geometry
+    restored after the call."""
+    from .pnjl import context
+    ctx = context(device)
+    base = opts or _lib.default_opts()
+
+    def solve(T, mu, xi, policy, chain_len):
+        o = _lib.default_opts()
+        for f, _ in _lib.SolveOpts._fields_:
+            setattr(o, f, getattr(base, f))
+        o.chain_len = int(chain_len)
+        saved = ctx.launch_cfg()
+        ctx.set_launch_cfg(lanes_per_point=lanes_per_point, chunk=1, waves_per_cu=8,
+                           tail_lanes_per_point=lanes_per_point)
+        try:
+            return ctx.solve_batch(T, mu, xi, policy=policy, p_num=p_num, t_num=t_num, opts=o)
+        finally:
+            ctx.set_launch_cfg(**saved)
+    return solve
+
+
+def _follow_chains(solve_chains, T, xi, mu_fm):
+    """Both branches of every line as branch chains: one launch per branch
+    (the hadron chains over mu ascending, the quark chains over mu
+    descending).  Same result as _follow_both: a point is on the branch
+    when it converged, did not raise and did not jump, up to the branch end."""
+    L, n = T.size, mu_fm.size
+    out = []
+    for policy, order in ((_lib.SEED_BRANCH_HADRON, np.arange(n)), (_lib.SEED_BRANCH_QUARK, np.arange(n)[::-1])):
+        r = solve_chains(np.repeat(T, n), np.tile(mu_fm[order], L), np.repeat(xi, n), policy, n)
+        br = {}
+        for f in _FIELDS:
+            v = r[f].reshape((L, n) + r[f].shape[1:])
+            br[f] = np.empty_like(v, dtype=np.float64)
+            br[f][:, order] = v
+        fl = r["flags"].reshape(L, n)
+        ok = ((r["converged"].reshape(L, n) == 1) &
+              ((fl & (_lib.PT_EXCEPTION | _lib.PT_NO_ROOT | _lib.PT_BRANCH_END | _lib.PT_SKIPPED)) == 0))
+        valid = np.zeros((L, n), dtype=bool)
+        valid[:, order] = ok
+        for f in _FIELDS:
+            mask = valid.reshape(valid.shape + (1,) * (br[f].ndim - 2))
+            br[f] = np.where(mask, br[f], np.nan)
+        br["valid"] = valid
+        out.append(br)
+    return out[0], out[1]
+
+
 def _follow_both(solve, T, xi, mu_fm):
     """Both branches of every line: the hadron branch over mu ascending and the
     quark branch over mu descending.  Step k of the two is independent (mu
@@ -106,20 +158,25 @@ def _same_solution(h, q, tol=0.01):
             & (np.abs(h["masses"][..., 0] - q["masses"][..., 0]) < tol * HBARC))
 
 
-def run_dual_branch_scan_batch(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None):
+def run_dual_branch_scan_batch(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None,
+                               chain_solver=None):
     """run_dual_branch_scan (DualBranchScan.jl:104-169) for every (T_mev[l],
     xi[l]) line over the same mu grid (MeV).  Returns a list of result dicts
     (T_mev, xi, mu_values, hadron, quark, physical), each branch a dict of
     per-mu arrays with a `valid` mask.  `solver(T, mu, xi, policy, seed)`
-    replaces the GPU batch solve (tests drive the same logic with other solvers)."""
+    replaces the GPU batch solve (tests drive the same logic with other
+    solvers) and follows the branches one mu step per call; without it each
+    branch of all lines is one launch of branch chains (`chain_solver(T, mu,
+    xi, policy, chain_len)`, by default `gpu_chain_solver`)."""
     T_mev = np.atleast_1d(np.asarray(T_mev, dtype=np.float64))
     xi = np.broadcast_to(np.asarray(xi, dtype=np.float64), T_mev.shape).copy()
     mu_values = np.asarray(list(mu_values), dtype=np.float64)
     T = T_mev / HBARC
     mu_fm = mu_values / HBARC
-    solve = solver or gpu_solver(device, p_num, t_num, opts)
-    n = mu_values.size
-    had, qrk = _follow_both(solve, T, xi, mu_fm)
+    if solver is not None:
+        had, qrk = _follow_both(solver, T, xi, mu_fm)
+    else:
+        had, qrk = _follow_chains(chain_solver or gpu_chain_solver(device, p_num, t_num, opts), T, xi, mu_fm)
     # _select_physical_branch  DualBranchScan.jl:378-398
     both = had["valid"] & qrk["valid"]
     same = both & _same_solution(had, qrk)
@@ -212,4 +269,4 @@ def merge_branches(result, output_path=None):
 
 
 __all__ = ["run_dual_branch_scan", "run_dual_branch_scan_batch", "find_phase_transition", "merge_branches",
-           "gpu_solver", "MERGED_COLUMNS"]
+           "gpu_solver", "gpu_chain_solver", "MERGED_COLUMNS"]

@@ -334,10 +334,11 @@ SADDLE_SEED_FRACTIONS = (0.5, 0.25, 0.75, 0.1, 0.9, 0.05, 0.95)
 
 
 def s_curves(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None,
-             fractions=SADDLE_SEED_FRACTIONS):
+             fractions=SADDLE_SEED_FRACTIONS, chain_solver=None, point_solver=None):
     """The mu(rho) curves of several (T, xi) lines from FixedMu solves on the
     GPU: the hadron branch followed up in mu and the quark branch down (the
-    dual-branch scan of all lines at once, one batched solve per mu step),
+    dual-branch scan of all lines at once: one launch of branch chains per
+    branch, or one batched solve per mu step when `solver` is given),
     then the unstable branch of every line in one more batch: at every mu
     where both stable branches exist and differ, Newton is started from the
     points x_h + f (x_q - x_h) of the segment between the two stable states
@@ -345,12 +346,15 @@ def s_curves(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, s
     density strictly between the two stable ones is the unstable state (the
     saddle of Omega between the minima).  Returns one (mu_MeV, rho) pair per
     line with all three branches together -- the S-curve that TrhoScan
-    traces at fixed rho."""
+    traces at fixed rho.  `solver(T, mu, xi, policy, seed)` replaces every
+    GPU solve (the branches then go one mu step per call); `chain_solver` and
+    `point_solver` replace the branch-chain and the per-point GPU solves."""
     from .dual_branch import gpu_solver, run_dual_branch_scan_batch
-    solve = solver or gpu_solver(device, p_num, t_num, opts)
+    solve = solver or point_solver or gpu_solver(device, p_num, t_num, opts)
     T_mev = np.atleast_1d(np.asarray(T_mev, dtype=np.float64))
     xi = np.broadcast_to(np.asarray(xi, dtype=np.float64), T_mev.shape)
-    lines = run_dual_branch_scan_batch(T_mev, mu_values, xi, p_num, t_num, opts, device, solver=solve)
+    lines = run_dual_branch_scan_batch(T_mev, mu_values, xi, p_num, t_num, opts, device, solver=solver,
+                                       chain_solver=chain_solver)
     f = np.asarray(fractions, dtype=np.float64)
     nf = f.size
     seeds, Ts, mus, xis, owner = [], [], [], [], []
@@ -390,15 +394,16 @@ def s_curves(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, s
 
 
 def s_curve(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None,
-            fractions=SADDLE_SEED_FRACTIONS):
+            fractions=SADDLE_SEED_FRACTIONS, chain_solver=None):
     """`s_curves` of one temperature: (mu_MeV, rho)."""
-    return s_curves([T_mev], mu_values, xi, p_num, t_num, opts, device, solver, fractions)[0]
+    return s_curves([T_mev], mu_values, xi, p_num, t_num, opts, device, solver, fractions, chain_solver)[0]
 
 
-def maxwell_line(T_values, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None, **kw):
+def maxwell_line(T_values, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None,
+                 chain_solver=None, **kw):
     """{T_MeV: MaxwellResult} of the GPU S-curves of the temperatures (all
     lines solved together); keyword arguments go to maxwell_construction."""
-    curves = s_curves(T_values, mu_values, xi, p_num, t_num, opts, device, solver)
+    curves = s_curves(T_values, mu_values, xi, p_num, t_num, opts, device, solver, chain_solver=chain_solver)
     return {float(T): maxwell_construction(mu, rho, **kw) for T, (mu, rho) in zip(T_values, curves)}
 
 

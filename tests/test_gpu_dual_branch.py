@@ -47,3 +47,42 @@ def test_batch_matches_oracle(gpu_ctx, oracle):
         if tg["found"]:
             assert abs(tg["mu_c"] - to["mu_c"]) < 1e-6
     assert agree >= 2 * len(g) - 1     # a branch end decided by a last-bit difference is allowed once
+
+
+def test_branch_chains_match_per_step_solves(gpu_ctx):
+    """One launch of branch chains per branch (PNJL_SEED_BRANCH_HADRON /
+    _QUARK: the branch end and the jump test inside the kernel) against the
+    same branches followed one batched launch per mu step: same extents,
+    same states; the points after a branch end are reported skipped."""
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import dual_branch as DB
+    T = np.repeat(np.arange(20.0, 151.0, 10.0), 2)
+    xi = np.tile([0.0, 0.3], T.size // 2)
+    mu = np.arange(0.0, 450.0 + 1e-9, 3.0)
+    seen = {}
+    base = DB.gpu_chain_solver()
+
+    def chains(Tq, muq, xiq, policy, n):
+        r = base(Tq, muq, xiq, policy, n)
+        seen[policy] = r
+        return r
+    c = DB.run_dual_branch_scan_batch(T, mu, xi, chain_solver=chains)
+    s = DB.run_dual_branch_scan_batch(T, mu, xi, solver=DB.gpu_solver())
+    for cl, sl in zip(c, s):
+        for b in ("hadron", "quark"):
+            assert np.array_equal(cl[b]["valid"], sl[b]["valid"])
+            v = cl[b]["valid"]
+            for k in ("x", "omega", "pressure", "rho_norm", "masses"):
+                assert np.allclose(cl[b][k][v], sl[b][k][v], rtol=1e-12, atol=1e-13), k
+    for pol, r in seen.items():
+        fl = r["flags"].reshape(T.size, mu.size)
+        ends = (fl & _lib.PT_BRANCH_END) != 0
+        skipped = (fl & _lib.PT_SKIPPED) != 0
+        assert np.all(ends.sum(axis=1) <= 1)
+        for l in range(T.size):
+            e = np.flatnonzero(ends[l])
+            if e.size:
+                assert skipped[l, e[0] + 1:].all() and not skipped[l, :e[0] + 1].any()
+            else:
+                assert not skipped[l].any()
+        assert np.all(r["converged"].reshape(T.size, mu.size)[skipped] == 0)
