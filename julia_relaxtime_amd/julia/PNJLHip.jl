@@ -22,6 +22,10 @@ const LIB = get(ENV, "PNJL_HIP_LIB", joinpath(@__DIR__, "..", "libpnjl_hip.so"))
 # policies / flags (include/pnjl_hip.h)
 const SEED_DEFAULT, SEED_HADRON, SEED_QUARK, SEED_MULTI, SEED_EXPLICIT = 0, 1, 2, 3, 4
 const SEED_CONTINUITY, SEED_PHASE_AWARE, SEED_SURROGATE, SEED_MULTIROOT3 = 5, 6, 7, 8
+const SEED_BRANCH_HADRON, SEED_BRANCH_QUARK = 9, 10      # DualBranchScan branches as chains
+# per-point flags (PNJL_PT_* of include/pnjl_hip.h)
+const PT_TR_USED, PT_MULTISEED, PT_EXCEPTION, PT_NO_ROOT = 0x01, 0x02, 0x04, 0x08
+const PT_CHAIN_RESET, PT_BRANCH_END, PT_SKIPPED = 0x80, 0x200, 0x400
 const CHANNEL_P, CHANNEL_S = 1, 2
 
 struct SolveOpts
