@@ -1258,7 +1258,10 @@ __device__ __forceinline__ double mass_guess(const pnjl_mass_query& a)
 // autoscale, factor 1) and a central-difference Jacobian; one thread per query.
 // out[i*6] = (mass, gamma, residual_norm (inf), converged, iterations, exception)
 // stride 64 puts one query per wavefront, for counting FLOPs.
-__global__ void __launch_bounds__(256) mass_kernel(Phys P, const double* tab, int64_t n,
+// 3 waves per SIMD (168 VGPRs, some spilled): the launch is throughput-bound
+// on its 1000-iteration queries, and the extra waves hide the quadratures'
+// latency (1.5x over 256 VGPRs at 1 wave).
+__global__ void __launch_bounds__(256, 3) mass_kernel(Phys P, const double* tab, int64_t n,
                                                    const pnjl_mass_query* qs, const double* kc, double G, double K,
                                                    int Nc, double* out, int stride)
 {
