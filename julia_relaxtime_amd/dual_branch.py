@@ -9,12 +9,16 @@ stopping at the first point that does not converge, raises (non-finite
 residual, MultiSeed without a root) or jumps (|d phi_u| > 0.5 or
 |d M_u| > 50 MeV); the physical branch is the one of lower Omega
 (DualBranchScan.jl:104-205, :330-445).  `run_dual_branch_scan_batch` follows
-every line of a batch at once: each mu step is one batched solve on the GPU
-(the first point of a branch with the DefaultSeed(:hadron / :quark) policy,
-the next ones from the previous solution); step k of the hadron branch (mu
-index k) and of the quark branch (index n-1-k) go into the same launch, so a
-batch of L lines costs len(mu) + 1 launches whatever L is.  The bookkeeping between steps (which lines
-are still followed, jump tests, the merge) is host logic on the GPU results.
+every line of a batch at once.  On the GPU each branch of all lines is ONE
+launch of branch chains (PNJL_SEED_BRANCH_HADRON / _QUARK): every lane group
+follows one line's mu grid point after point -- the first point from
+DefaultSeed(:hadron / :quark), the next ones from the previous solution --
+and stops at the branch end (failure, exception or jump, tested in the
+kernel), so a dual-branch scan of any number of lines is two launches.  With
+a `solver` argument (the oracle in the tests) the same branches are followed
+one batched solve per mu step instead, the hadron step k and the quark step
+n-1-k in one call.  Branch selection, merge and the Omega crossing are host
+logic on the results.
 """
 from __future__ import annotations
 
