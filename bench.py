@@ -224,6 +224,14 @@ def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
                 sample=f"{what}, plain {name} solve, {threads} OpenMP threads, {dt:.2f} s wall")
 
 
+# HBM bytes per scan_kernel launch of the learned-seed scan (c4_surrogate):
+# FETCH_SIZE + WRITE_SIZE of its three launches per step from two rocprofv3
+# --pmc runs of this bench (8.1 + 7.5, 14.8 + 13.0, 13.8 + 15.2 GiB), divided
+# by 3.  WRITE_SIZE reads exactly 1 GiB for a 1 GiB copy; FETCH_SIZE is
+# unscaled.  Against ~0.3 KB of algorithmic bytes per point the excess is the
+# kernel's call-frame and per-group scratch traffic.
+C4_TRAFFIC_BYTES_PER_LAUNCH = (8.1 + 7.5 + 14.8 + 13.0 + 13.8 + 15.2) / 3 * 2 ** 30
+
 # launch geometry of the learned seed's set-up solves (training targets, judge
 # mask): fixed, so that the trained network does not depend on what the
 # autotuner picked for the timed scans (lanes per point changes the node
@@ -795,7 +803,9 @@ def main():
                        mean_iterations=float(np.mean(Q["iterations"])), **extra),
         # dominant kernel: the persistent scan kernel (all its launches in one step)
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "pnjl::scan_kernel",
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": C4_TRAFFIC_BYTES_PER_LAUNCH if args.workload == "c4_surrogate" else None,
+                     "kernel": "pnjl::scan_kernel",
                      "kernel_ms_per_step": scan_avg, "flops_per_step": flops, "flops_per_node": C_NODE},
     }
     if args.copy_probe_mib:
