@@ -193,3 +193,31 @@ def test_s_curve_maxwell_matches_reference_boundary(oracle):
         assert abs(m.mu_transition - float(b["mu_transition_MeV"])) < 0.06
         assert abs(m.rho_hadron - float(b["rho_hadron"])) < 0.01
         assert abs(m.rho_quark - float(b["rho_quark"])) < 0.01
+
+
+def test_load_curves_from_csv_feeds_the_boundary(tmp_path):
+    """load_curves_from_csv (calculate_phase_structure.jl:226-273) reads a
+    TrhoScan file -- converged rows of one xi, temperatures with >= 3
+    samples -- and its curves give the boundary.csv rows; an unconverged row
+    and a short temperature are dropped."""
+    import csv
+    import os
+    P = PT()
+    src = os.path.join(os.path.dirname(__file__), "golden", "trho_scan_T50_80_110.csv")
+    rows = list(csv.DictReader(open(src)))
+    path = tmp_path / "trho.csv"
+    with open(path, "w") as f:
+        f.write(",".join(list(rows[0].keys()) + ["converged"]) + "\n")
+        for r in rows:
+            f.write(",".join(list(r.values()) + ["true"]) + "\n")
+        bad = dict(rows[0], mu_avg_MeV="999.0")
+        f.write(",".join(list(bad.values()) + ["false"]) + "\n")      # not converged: ignored
+        for k in range(2):                                               # 2 samples only: dropped
+            f.write(",".join(["60.000000", f"{k}.0", "0.000000"] + ["1.0"] * (len(rows[0]) - 3) + ["true"]) + "\n")
+    bd = {(float(r["xi"]), float(r["T_MeV"])): r for r in golden_rows("phase_boundary.csv")}
+    for xi in (0.0, 0.4):
+        curves = P.load_curves_from_csv(str(path), xi)
+        assert sorted(curves) == [50.0, 80.0, 110.0]
+        for T, (mu, rho) in curves.items():
+            m = P.maxwell_construction(mu, rho, **MAXWELL_KW)
+            assert m.converged and m.mu_transition == float(bd[(xi, T)]["mu_transition_MeV"])
