@@ -558,7 +558,7 @@ def run_tmu(args, w):
     print(json.dumps(line), flush=True)
 
 
-MAXWELL_METRIC = "Maxwell-line S-curve points/s (s_curves + maxwell_construction, 45 lines x 681 mu)"
+MAXWELL_METRIC = "Maxwell-line S-curve points/s (maxwell_line: S-curves + Maxwell construction, 45 lines x 681 mu)"
 
 
 def maxwell_lines():
@@ -585,8 +585,7 @@ def maxwell_cpu_baseline(n_lines=45, threads=None):
         count["points"] += Tq.size
         return O.solve_batch(Tq, muq, xiq, policy=policy, seed=seed, p_num=24, t_num=8, nthreads=threads)
     t0 = time.perf_counter()
-    for mu_c, rho in PT.s_curves(T[:n_lines], mu, xi[:n_lines], solver=solve):
-        PT.maxwell_construction(mu_c, rho, min_samples=8, detect_min_points=5)
+    PT.maxwell_line(T[:n_lines], mu, xi[:n_lines], solver=solve, min_samples=8, detect_min_points=5)
     dt = time.perf_counter() - t0
     return dict(value=count["points"] / dt, unit="points/s", cores=threads, kind="port",
                 sample=f"the first {n_lines} lines ({count['points']} FixedMu solves), oracle solves with {threads} "
@@ -621,8 +620,8 @@ def run_maxwell(args, w):
     def step():
         for k in stats:
             stats[k] = 0
-        curves = PT.s_curves(T, mu, xi, chain_solver=chains, point_solver=solve)
-        res["m"] = [PT.maxwell_construction(m, r, min_samples=8, detect_min_points=5) for m, r in curves]
+        ms = PT.maxwell_line(T, mu, xi, chain_solver=chains, point_solver=solve, min_samples=8, detect_min_points=5)
+        res["m"] = list(ms.values())
         return stats["ms"]
 
     ms_step, kms = timed_steps(step, args.steps, args.warmup)

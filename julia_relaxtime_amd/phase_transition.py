@@ -334,7 +334,7 @@ SADDLE_SEED_FRACTIONS = (0.5, 0.25, 0.75, 0.1, 0.9, 0.05, 0.95)
 
 
 def s_curves(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None,
-             fractions=SADDLE_SEED_FRACTIONS, chain_solver=None, point_solver=None):
+             fractions=SADDLE_SEED_FRACTIONS, chain_solver=None, point_solver=None, return_lines=False):
     """The mu(rho) curves of several (T, xi) lines from FixedMu solves on the
     GPU: the hadron branch followed up in mu and the quark branch down (the
     dual-branch scan of all lines at once: one launch of branch chains per
@@ -390,7 +390,7 @@ def s_curves(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, s
             r_list.append(rr[np.flatnonzero(has), first[has]])
             base += both.size
         out.append((np.concatenate(m_list), np.concatenate(r_list)))
-    return out
+    return (out, lines) if return_lines else out
 
 
 def s_curve(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None,
@@ -400,11 +400,39 @@ def s_curve(T_mev, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, so
 
 
 def maxwell_line(T_values, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, device=0, solver=None,
-                 chain_solver=None, **kw):
+                 chain_solver=None, point_solver=None, refine=16, **kw):
     """{T_MeV: MaxwellResult} of the GPU S-curves of the temperatures (all
-    lines solved together); keyword arguments go to maxwell_construction."""
-    curves = s_curves(T_values, mu_values, xi, p_num, t_num, opts, device, solver, chain_solver=chain_solver)
-    return {float(T): maxwell_construction(mu, rho, **kw) for T, (mu, rho) in zip(T_values, curves)}
+    lines solved together; with one xi per line the keys are (T_MeV, xi));
+    keyword arguments go to maxwell_construction.
+    A line whose construction fails although its two stable branches coexist
+    (close to the CEP the S-region can be narrower than the mu grid) is
+    solved again on a grid `refine` times finer over its coexistence window
+    (between the first quark-branch point and the last hadron-branch point,
+    also when these miss each other by up to two coarse steps, with two
+    coarse steps of margin on each side), merged with its coarse curve, and
+    constructed again; the details record `refined`."""
+    T_values = [float(T) for T in np.atleast_1d(T_values)]
+    mu_values = np.asarray(list(mu_values), dtype=np.float64)
+    per_line = np.ndim(xi) > 0
+    xi_l = np.broadcast_to(np.asarray(xi, dtype=np.float64), (len(T_values),))
+    curves, lines = s_curves(T_values, mu_values, xi_l, p_num, t_num, opts, device, solver, chain_solver=chain_solver,
+                             point_solver=point_solver, return_lines=True)
+    out = {}
+    step = float(np.min(np.diff(mu_values))) if mu_values.size > 1 else 0.0
+    for T, x, (mu, rho), res in zip(T_values, xi_l, curves, lines):
+        m = maxwell_construction(mu, rho, **kw)
+        h, q = res["hadron"]["valid"], res["quark"]["valid"]
+        if not m.converged and refine and step > 0 and h.any() and q.any():
+            lo, hi = mu_values[np.flatnonzero(q)[0]], mu_values[np.flatnonzero(h)[-1]]
+            if lo <= hi + 2 * step:      # coexisting, or meeting between two grid points
+                lo, hi = min(lo, hi), max(lo, hi)
+                fine = np.arange(lo - 2 * step, hi + 2 * step + 1e-9, step / refine)
+                mf, rf = s_curves([T], fine, x, p_num, t_num, opts, device, solver, chain_solver=chain_solver,
+                                  point_solver=point_solver)[0]
+                m = maxwell_construction(np.concatenate([mu, mf]), np.concatenate([rho, rf]), **kw)
+                m.details["refined"] = (float(fine[0]), float(fine[-1]), step / refine)
+        out[(T, float(x)) if per_line else T] = m
+    return out
 
 
 __all__ = ["SShapeResult", "MaxwellResult", "detect_s_shape", "maxwell_construction", "group_curves_by_temperature",

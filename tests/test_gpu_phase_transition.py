@@ -34,3 +34,16 @@ def test_gpu_s_curves_maxwell(gpu_ctx, oracle):
         assert abs(m.mu_transition - float(b["mu_transition_MeV"])) < 0.06
         assert abs(m.rho_hadron - float(b["rho_hadron"])) < 0.01
         assert abs(m.rho_quark - float(b["rho_quark"])) < 0.01
+
+
+def test_gpu_maxwell_line_next_to_the_cep(gpu_ctx):
+    """The refined lines just below the CEP on the GPU (branch chains and
+    batched unstable-branch solves): within 0.01 MeV of boundary.csv."""
+    from julia_relaxtime_amd import phase_transition as PT
+    bd = {(float(r["xi"]), float(r["T_MeV"])): r for r in golden_rows("phase_boundary.csv")}
+    mus = np.arange(250.0, 420.0 + 1e-9, 0.25)
+    for xi, T in ((0.0, 130.0), (0.2, 120.0), (0.4, 110.0)):
+        print("maxwell_line on the GPU:", xi, T, flush=True)
+        m = PT.maxwell_line([T], mus, xi, **MAXWELL_KW)[T]
+        assert m.converged and "refined" in m.details
+        assert abs(m.mu_transition - float(bd[(xi, T)]["mu_transition_MeV"])) < 0.01

@@ -221,3 +221,17 @@ def test_load_curves_from_csv_feeds_the_boundary(tmp_path):
         for T, (mu, rho) in curves.items():
             m = P.maxwell_construction(mu, rho, **MAXWELL_KW)
             assert m.converged and m.mu_transition == float(bd[(xi, T)]["mu_transition_MeV"])
+
+
+def test_maxwell_line_refines_next_to_the_cep(oracle):
+    """Just below each xi's CEP (T_CEP 130.9, 123.0, 113.0 MeV at xi 0, 0.2,
+    0.4) the S-region is narrower than a 0.25 MeV mu grid; maxwell_line
+    re-solves such a line 16x finer over its coexistence window and then
+    lands within 0.01 MeV of the reference's boundary.csv."""
+    P = PT()
+    bd = {(float(r["xi"]), float(r["T_MeV"])): r for r in golden_rows("phase_boundary.csv")}
+    mus = np.arange(250.0, 420.0 + 1e-9, 0.25)
+    for xi, T in ((0.0, 130.0), (0.2, 120.0), (0.4, 110.0)):
+        m = P.maxwell_line([T], mus, xi, solver=_oracle_solver(oracle), min_samples=8, detect_min_points=5)[T]
+        assert m.converged and "refined" in m.details
+        assert abs(m.mu_transition - float(bd[(xi, T)]["mu_transition_MeV"])) < 0.01
