@@ -323,6 +323,11 @@ def _polar_lib():
         L.oracle_B0.argtypes = [C.c_double] * 9 + [_DP, _DP]
         L.oracle_tilde_B0.argtypes = [C.c_int] + [C.c_double] * 8 + [_DP, _DP]
         L.oracle_B0_correction.argtypes = [C.c_double] * 10 + [_DP, _DP]
+        L.oracle_tilde_B0_correction.argtypes = [C.c_int] + [C.c_double] * 9 + [_DP, _DP]
+        L.oracle_real_tool.argtypes = [C.c_double, C.c_double]
+        L.oracle_real_tool.restype = C.c_double
+        L.oracle_imag_tool.argtypes = [C.c_double, C.c_double]
+        L.oracle_imag_tool.restype = C.c_double
         L.oracle_A_const.argtypes = [C.c_double]
         L.oracle_A_const.restype = C.c_double
         L.oracle_A.argtypes = [C.c_double] * 5 + [C.c_int, _DP, _DP]
@@ -357,6 +362,21 @@ def tilde_B0(plus, lam, k, m, mp, mu, T, Phi=0.0, Phib=0.0):
 
 def B0_correction(lam, k, m1, m2, mu1, mu2, T, Phi, Phib, xi):
     return _pair(_polar_lib().oracle_B0_correction, lam, k, m1, m2, mu1, mu2, T, Phi, Phib, xi)
+
+
+def tilde_B0_correction(quark, lam, k, m, mp, mu, T, Phi, Phib, xi):
+    """One delta-B~ term (OneLoopIntegralsAniso.jl:380-390)."""
+    return _pair(_polar_lib().oracle_tilde_B0_correction, 0 if quark else 1, lam, k, m, mp, mu, T, Phi, Phib, xi)
+
+
+def real_tool(cx, dc):
+    """real_integral_tool (OneLoopIntegralsAniso.jl:26-33)."""
+    return _polar_lib().oracle_real_tool(cx, dc)
+
+
+def imag_tool(cx, dc):
+    """imag_integral_tool (OneLoopIntegralsAniso.jl:36-38)."""
+    return _polar_lib().oracle_imag_tool(cx, dc)
 
 
 def A_const(m):

@@ -770,6 +770,18 @@ static void tilde_Bc(int sg, double lam, double k, double m, double mp, double m
     else tilde_Bc_k0(&c, re, im);
 }
 
+/* one delta-B~ term (tilde_B0_correction, OneLoopIntegralsAniso.jl:380-390); sg 0 quark, 1 antiquark */
+void oracle_tilde_B0_correction(int sg, double lam, double k, double m, double mp, double mu, double T, double Phi,
+                                double Phib, double xi, double* re, double* im)
+{
+    ensure_tables();
+    tilde_Bc(sg == 0 ? QUARK : ANTIQUARK, lam, k, m, mp, mu, T, Phi, Phib, xi, re, im);
+}
+
+/* real_integral_tool / imag_integral_tool  OneLoopIntegralsAniso.jl:26-38 */
+double oracle_real_tool(double cx, double dc) { return real_tool(cx, dc); }
+double oracle_imag_tool(double cx, double dc) { return imag_tool(cx, dc); }
+
 /* B0_correction  OneLoopIntegralsAniso.jl:396-413 */
 void oracle_B0_correction(double lam, double k, double m1, double m2, double mu1, double mu2, double T, double Phi,
                           double Phib, double xi, double* re, double* im)

@@ -7,7 +7,7 @@ exact integrals are loose; the exact invariants (linearity in xi, mirror and
 k0 symmetries, channel difference, the xi = 0 reductions) are tight."""
 import numpy as np
 import pytest
-from scipy import integrate
+from scipy import integrate, optimize
 
 HB = 197.327
 LAM = 602.3 / HB
@@ -79,33 +79,188 @@ def test_B0_k0_against_principal_value_quad(O):
             assert abs(re - exact) <= 2e-5 * max(1.0, abs(exact)), (lam, sg, re, exact)
 
 
+def _root_breaks(fns, lo, hi, n=40001):
+    """Sign changes of each function on a fine grid, refined by brentq."""
+    Es = np.linspace(lo, hi, n)
+    brk = []
+    for g in fns:
+        v = np.array([g(E) for E in Es])
+        for i in np.flatnonzero(np.sign(v[1:]) != np.sign(v[:-1])):
+            brk.append(optimize.brentq(g, Es[i], Es[i + 1], xtol=1e-15))
+    return sorted(set([lo, hi] + brk))
+
+
+def _quad_pieces(f, pts):
+    return sum(integrate.quad(f, a, b, limit=400, epsabs=1e-14, epsrel=1e-12)[0]
+               for a, b in zip(pts[:-1], pts[1:]) if b > a)
+
+
 def test_B0_kpos_against_quad(O):
+    """B~ at k > 0 (OneLoopIntegrals.jl:228-476): the real part is
+    (1/k) int f ln|(B + A)/(B - A)| dE with A = 2 p k, B = lam^2 + 2 lam E +
+    m^2 - m'^2 - k^2 (= (lam + E)^2 - m'^2 - p^2 - k^2); the imaginary part
+    is pi sign(lam)/k times the integral of f over the cut |B| < A.  Both by
+    adaptive quadrature split at the zeros of B +- A; the reference's fixed
+    rules reach ~1e-6 here."""
     m1, m2, mu, T, Phi, Phib = 0.3, 0.5, 0.1, 0.15, 0.3, 0.35
     Emax = np.sqrt(m1 ** 2 + LAM ** 2)
-    for lam, k in ((0.9, 0.27), (0.3, 1.1), (-1.4, 0.6)):
+    p = lambda E: np.sqrt(max(E * E - m1 * m1, 0.0))
+    for lam, k in ((0.9, 0.27), (0.3, 1.1), (-1.4, 0.6), (0.05, 0.8), (2.5, 0.3)):
+        A = lambda E: 2 * p(E) * k
+        B = lambda E: lam * lam + 2 * lam * E + m1 * m1 - m2 * m2 - k * k
+        pts = _root_breaks((lambda E: B(E) + A(E), lambda E: B(E) - A(E)), m1, Emax)
         for sg in (True, False):
             re, im = O.tilde_B0(sg, lam, k, m1, m2, mu, T, Phi, Phib)
             dist = (lambda E: f_plus(E, mu, T, Phi, Phib)) if sg else (lambda E: 1 - f_minus(E, mu, T, Phi, Phib))
+            exact = _quad_pieces(lambda E: dist(E) * np.log(abs((B(E) + A(E)) / (B(E) - A(E)))), pts) / k
+            ex_im = np.pi * np.sign(lam) / k * _quad_pieces(lambda E: dist(E) if abs(B(E)) < A(E) else 0.0, pts)
+            assert abs(re - exact) <= 1e-5 * max(1.0, abs(exact)), (lam, k, sg, re, exact)
+            assert abs(im - ex_im) <= 1e-12 * max(1.0, abs(ex_im)), (lam, k, sg, im, ex_im)
 
-            def integrand(E):
-                p = np.sqrt(max(E * E - m1 * m1, 0.0))
-                c = (lam + E) ** 2 - m2 ** 2
-                return dist(E) * np.log(abs((c - (p - k) ** 2) / (c - (p + k) ** 2)))
-            # logarithmic singularities: split at the zeros of both arguments
-            Es = np.linspace(m1, Emax, 20001)
-            p = np.sqrt(np.maximum(Es ** 2 - m1 ** 2, 0))
-            c = (lam + Es) ** 2 - m2 ** 2
-            brk = []
-            for g in (c - (p - k) ** 2, c - (p + k) ** 2):
-                s = np.flatnonzero(np.sign(g[1:]) != np.sign(g[:-1]))
-                brk += list(Es[s])
-            pts = sorted(set([m1, Emax] + brk))
-            exact = sum(integrate.quad(integrand, a, b, limit=400)[0] for a, b in zip(pts[:-1], pts[1:]) if b > a) / k
-            assert abs(re - exact) <= 1e-3 * max(1.0, abs(exact)), (lam, k, sg, re, exact)
-            # imaginary part: pi sign(lam)/k times the integral of the distribution over the cut
-            cut = (np.abs(c - p * p - k * k) < 2 * p * k)
-            ex_im = np.pi * np.sign(lam) / k * np.trapezoid(np.where(cut, dist(Es), 0.0), Es)
-            assert abs(im - ex_im) <= 2e-3 * max(1.0, abs(ex_im)), (lam, k, sg, im, ex_im)
+
+# ---------------------------------------------------------------------------
+# first-order anisotropic correction (OneLoopIntegralsAniso.jl), pinned by
+# quadrature of its definition: delta f = (1/2) xi (p cos)^2 / E df/dE
+# (QuarkDistribution_Aniso.jl:71-86), the cos integral x^2 / (A x + B) over
+# [-1, 1] and the energy integral
+# ---------------------------------------------------------------------------
+def _dist(quark, E, mu, T, Phi, Phib):
+    return f_plus(E, mu, T, Phi, Phib) if quark else f_minus(E, mu, T, Phi, Phib)
+
+
+def _df_dE(quark, E, mu, T, Phi, Phib):
+    """Fourth-order central difference of the distribution (relative error ~1e-11)."""
+    h = 1e-5 * max(1.0, E)
+    f = lambda x: _dist(quark, x, mu, T, Phi, Phib)
+    return (-f(E + 2 * h) + 8 * f(E + h) - 8 * f(E - h) + f(E - 2 * h)) / (12 * h)
+
+
+def _cos_coef(quark, E, m, mu, T, Phi, Phib, xi):
+    return 0.5 * xi * (E * E - m * m) / E * _df_dE(quark, E, mu, T, Phi, Phib)
+
+
+def _pv_x2(A, B):
+    """PV int_{-1}^{1} x^2 / (A x + B) dx by adaptive quadrature (Cauchy weight at the pole)."""
+    x0 = -B / A
+    if -1.0 < x0 < 1.0:
+        return integrate.quad(lambda x: x * x, -1.0, 1.0, weight="cauchy", wvar=x0, epsabs=1e-15, epsrel=1e-13)[0] / A
+    return integrate.quad(lambda x: x * x / (A * x + B), -1.0, 1.0, epsabs=1e-15, epsrel=1e-13)[0]
+
+
+def _delta_B_quad(quark, lam, k, m, mp, mu, T, Phi, Phib, xi):
+    """One delta-B~ term from its definition.  k = 0: PV int (4/3) p delta-c /
+    (2 lam E + lam^2 + m^2 - m'^2) dE, imaginary part pi (residue); k > 0:
+    int 2 p PV[x^2/(A x + B)] delta-c dE split at the zeros of A +- B, the
+    imaginary part sign(lam) int_{|B| < A} 2 p pi B^2/A^3 delta-c dE."""
+    Emin, Emax = m, np.sqrt(m * m + LAM * LAM)
+    p = lambda E: np.sqrt(max(E * E - m * m, 0.0))
+    c = lambda E: _cos_coef(quark, E, m, mu, T, Phi, Phib, xi)
+    if k <= 1e-9:
+        cE, dc = 2 * lam, lam * lam + m * m - mp * mp
+        num = lambda E: 4.0 / 3.0 * p(E) * c(E)
+        E0 = -dc / cE
+        if Emin < E0 < Emax:
+            re = integrate.quad(num, Emin, Emax, weight="cauchy", wvar=E0, limit=400, epsabs=1e-14,
+                                epsrel=1e-12)[0] / cE
+            return re, np.pi * num(E0) / cE
+        return integrate.quad(lambda E: num(E) / (cE * E + dc), Emin, Emax, limit=400, epsabs=1e-14,
+                              epsrel=1e-12)[0], 0.0
+    A = lambda E: 2 * p(E) * k
+    B = lambda E: lam * lam + 2 * lam * E + m * m - mp * mp - k * k
+    pts = _root_breaks((lambda E: A(E) + B(E), lambda E: A(E) - B(E)), Emin, Emax)
+    # the cos integral in closed form (pinned against _pv_x2 below)
+    R = lambda a, b: -2 * b / a ** 2 + b ** 2 / a ** 3 * np.log(abs((a + b) / (a - b)))
+    re = _quad_pieces(lambda E: 2 * p(E) * R(A(E), B(E)) * c(E) if p(E) > 0 else 0.0, pts)
+    im = np.sign(lam) * _quad_pieces(
+        lambda E: 2 * p(E) * np.pi * B(E) ** 2 / A(E) ** 3 * c(E) if abs(B(E)) < A(E) else 0.0, pts)
+    return re, im
+
+
+def test_angular_tools_against_pv_quad(O):
+    """real_integral_tool / imag_integral_tool (OneLoopIntegralsAniso.jl:26-38):
+    PV int_{-1}^{1} x^2/(A x + B) dx, and pi B^2/A^3 = pi sign(A) x0^2/|A| --
+    pi times the delta-function weight of the pole x0 = -B/A."""
+    rng = np.random.default_rng(11)
+    R = lambda a, b: -2 * b / a ** 2 + b ** 2 / a ** 3 * np.log(abs((a + b) / (a - b)))
+    for _ in range(60):
+        A = rng.choice([-1, 1]) * 10 ** rng.uniform(-1.5, 1.0)
+        B = A * rng.uniform(-3.0, 3.0)          # |B| < |A| (pole inside) about a third of the time
+        ref = _pv_x2(A, B)
+        assert abs(O.real_tool(A, B) - ref) <= 1e-11 * max(1.0, abs(ref)), (A, B)
+        assert abs(R(A, B) - ref) <= 1e-11 * max(1.0, abs(ref))
+        x0 = -B / A
+        assert abs(O.imag_tool(A, B) - np.pi * np.sign(A) * x0 * x0 / abs(A)) <= 1e-14 * abs(O.imag_tool(A, B))
+
+
+# (lam, k, xi) x the four terms of B0_correction at the reference test's
+# parameters (tests/unit/relaxtime/test_b0_correction.jl:22-32) and others:
+# poles at k = 0, cuts between and outside the roots at k > 0
+BC_PARAMS = dict(m1=0.24, m2=0.38, mu1=0.12, mu2=-0.05, T=0.17, Phi=0.15, Phib=0.15)
+BC_CASES = [(0.45, 0.0, 0.2), (0.45, 0.3, 0.05), (0.45, 0.5, 0.2), (0.1, 0.0, 0.2), (0.1, 0.6, 0.2),
+            (-1.3, 0.4, 0.3), (2.0, 1.5, 0.2), (0.7, 0.7, 0.2)]
+
+
+def _bc_terms(lam, m1, m2, mu1, mu2):
+    """(quark?, lambda, m, m', mu) of the four terms (OneLoopIntegralsAniso.jl:400-403)."""
+    return [(True, -lam, m1, m2, mu1), (False, lam, m1, m2, mu1), (True, lam, m2, m1, mu2), (False, -lam, m2, m1, mu2)]
+
+
+def test_B0_correction_terms_against_quad(O):
+    P = BC_PARAMS
+    for lam, k, xi in BC_CASES:
+        tot_o, tot_q = np.zeros(2), np.zeros(2)
+        for quark, L, m, mp, mu in _bc_terms(lam, P["m1"], P["m2"], P["mu1"], P["mu2"]):
+            o = np.array(O.tilde_B0_correction(quark, L, k, m, mp, mu, P["T"], P["Phi"], P["Phib"], xi))
+            q = np.array(_delta_B_quad(quark, L, k, m, mp, mu, P["T"], P["Phi"], P["Phib"], xi))
+            if k == abs(L) and L < 0 and mp < m:
+                # |lam| = k exactly: the reference's degenerate cut (singularity_k_positive,
+                # OneLoopIntegrals.jl:285-303) keeps only lam > 0, m' > m and drops this
+                # cut; the oracle follows the reference
+                assert o[1] == 0.0 and q[1] != 0.0
+                q[1] = 0.0
+            scale = max(abs(q[0]), abs(q[1]))
+            # the reference's fixed rules (16-33 nodes) against adaptive quadrature
+            assert abs(o[0] - q[0]) <= 5e-5 * scale, (lam, k, xi, quark, o, q)
+            assert abs(o[1] - q[1]) <= 1e-6 * scale, (lam, k, xi, quark, o, q)
+            tot_o += o
+            tot_q += q
+        # B0_correction is the sum of the four terms, all with + signs
+        bc = O.B0_correction(lam, k, P["m1"], P["m2"], P["mu1"], P["mu2"], P["T"], P["Phi"], P["Phib"], xi)
+        assert np.allclose(bc, tot_o, rtol=1e-14, atol=0)
+        assert np.all(np.abs(tot_o - tot_q) <= 5e-5 * np.max(np.abs(tot_q)))
+    # at the reference test's parameters: 6.2933e-3 (k = 0, xi = 0.2) and 1.53518e-3 (k = 0.3, xi = 0.05)
+    a = O.B0_correction(0.45, 0.0, P["m1"], P["m2"], P["mu1"], P["mu2"], P["T"], P["Phi"], P["Phib"], 0.2)
+    b = O.B0_correction(0.45, 0.3, P["m1"], P["m2"], P["mu1"], P["mu2"], P["T"], P["Phi"], P["Phib"], 0.05)
+    assert abs(a[0] - 6.2933e-3) < 1e-7 and a[1] == 0.0
+    assert abs(b[0] - 1.53518e-3) < 1e-8 and b[1] == 0.0
+
+
+def test_B0_correction_reference_summary_table(O):
+    """The reference's B0_correction summary
+    (tests/analysis/unit_summaries/test_b0_correction_summary.md:62-68, :86-93)
+    was written by an earlier revision that combined the four terms as
+    -(t1 - t2 + t3 - t4); the present B0_correction (OneLoopIntegralsAniso.jl:
+    396-413) adds them, since delta(1 - f-) = -delta f-.  The stored table is
+    reproduced term by term to its 4 digits in that combination (the real part
+    of its k = 0.5 row, -1.478e-3, is not: -1.481e-3 here)."""
+    P = BC_PARAMS
+
+    def old(lam, k, xi):
+        t = [np.array(O.tilde_B0_correction(q, L, k, m, mp, mu, P["T"], P["Phi"], P["Phib"], xi))
+             for q, L, m, mp, mu in _bc_terms(lam, P["m1"], P["m2"], P["mu1"], P["mu2"])]
+        return -(t[0] - t[1] + t[2] - t[3])
+
+    def four_digits(x, ref):
+        return float(f"{x:.3e}") == ref
+
+    for xi, ref in ((0.05, -9.043e-4), (0.10, -1.809e-3), (0.20, -3.617e-3), (0.30, -5.426e-3), (0.50, -9.043e-3)):
+        r = old(0.45, 0.30, xi)
+        assert four_digits(r[0], ref) and r[1] == 0.0, (xi, r, ref)
+    for k, ref in ((0.0, -4.069e-3), (0.1, -4.018e-3), (0.2, -3.862e-3), (0.3, -3.617e-3), (0.4, -3.505e-3)):
+        r = old(0.45, k, 0.2)
+        assert four_digits(r[0], ref) and r[1] == 0.0, (k, r, ref)
+    r = old(0.45, 0.5, 0.2)
+    assert four_digits(r[1], -3.622e-4), r
 
 
 def test_B0_reference_unit_checks(O):
