@@ -52,13 +52,39 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HB = 197.327
+
+
+def host_threads() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS when the environment sets
+    it (the host's CPU share for one GPU), else every core this process may
+    run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def host_cpu() -> dict:
+    """The host the CPU baseline ran on: model name, logical CPUs, the ones
+    this process may use, and OMP_NUM_THREADS."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(cpu_model=model, nproc=len(os.sched_getaffinity(0)), logical_cpus=os.cpu_count(),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
 METRIC = "PNJL gap points solved/s at |res|<1e-10: surrogate-seeded vs plain Newton"
 PEAK_FP64_TFLOPS = 78.6
 # FP64 FLOPs of one node x flavour evaluation of the fused integrand (value,
 # gradient, Hessian and thermodynamic sums) as the device routine executes it:
 # FP64 FMA = 2, add/mul/transcendental = 1, measured with rocprofv3 FP64 VALU
 # counters on the residual kernel at p = 64 and p = 32
-# (python3 -m julia_relaxtime_amd.flop_probe): 176.0 + 2.0.
+# (python3 bench.py --flop-probe node): 176.0 + 2.0.
 C_NODE = 178.0
 
 
@@ -114,7 +140,7 @@ C3_METRIC = "anisotropic one-loop polarization values/s (12,582,912-value Pi gri
 # FP64 FLOPs per complex Pi value of this grid, data-dependent and therefore
 # counted once: rocprofv3 FP64 VALU counters (FMA = 2) on 131,072 seeded cells
 # of the same grid evaluated one per wavefront, so that inactive lanes of
-# divergent waves are not counted (python3 -m julia_relaxtime_amd.pi_flop_probe):
+# divergent waves are not counted (python3 bench.py --flop-probe pi):
 # 86,571 per cell (one symmetrised B0 + correction and one Pi), two Pi values
 # per cell.  The A kernel (512 values) is negligible.
 C3_FLOPS_PER_VALUE = 43286.0
@@ -130,7 +156,7 @@ def c3_cpu_baseline(pts, k0, k, n_sample=2_000_000, threads=None):
     """The oracle's polarization_aniso (C restatement, OpenMP over values) on
     a seeded sample of the grid's Pi values, A from the oracle as well."""
     from oracle import oracle as O
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_threads()
     xp, wp = O.gauleg(0.0, 10.0, 64)
     xc, wc = np.polynomial.legendre.leggauss(4)
     t0 = time.perf_counter()
@@ -195,7 +221,7 @@ def run_polar(args, w):
                      "flops_per_value": C3_FLOPS_PER_VALUE},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = c3_cpu_baseline(pts, k0, k)
+        line["cpu_baseline"] = dict(c3_cpu_baseline(pts, k0, k), **host_cpu())
     print(json.dumps(line), flush=True)
 
 
@@ -204,7 +230,7 @@ def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
     workload's plain policy, DefaultSeed or MultiSeed) on a seeded sample of
     the same grid, OpenMP over points."""
     from oracle import oracle as O
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_threads()
     rng = np.random.default_rng(20261202)
     idx = rng.choice(T.size, min(n_sample, T.size), replace=False)
     opts = O.default_opts(ftol=1e-10, max_iter=w["max_iter"])
@@ -270,7 +296,7 @@ def _train_learned_seed(ctx, w, opts):
 PROP_METRIC = "meson propagator (D_S, D_P) queries/s (calculate_propagator_by_channel, 1,048,576-query batch)"
 # FP64 FLOPs per query of this batch, counted once like C3_FLOPS_PER_VALUE:
 # FP64 VALU counters on 65,536 of its queries run one (pair, query) per
-# wavefront (python3 -m julia_relaxtime_amd.prop_flop_probe): 119,772 per
+# wavefront (python3 bench.py --flop-probe prop): 119,772 per
 # query (1.68 quark pairs of B0 + correction per query on average).  The
 # bucketing and composition kernels (~10^2 operations per query) are not
 # counted; their time is.
@@ -282,7 +308,7 @@ def prop_cpu_baseline(q, G, K, threads=None):
     C (OpenMP over values), queries built and propagators composed as numpy
     array operations."""
     from oracle import mesons as OM
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_threads()
     t0 = time.perf_counter()
     OM.propagator_by_channel_vec(q, G, K, nthreads=threads)
     dt = time.perf_counter() - t0
@@ -341,13 +367,13 @@ def run_propagators(args, w):
                      "flops_per_query": PROP_FLOPS_PER_QUERY},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = prop_cpu_baseline(q, pctx.params.G, pctx.params.K)
+        line["cpu_baseline"] = dict(prop_cpu_baseline(q, pctx.params.G, pctx.params.K), **host_cpu())
     print(json.dumps(line), flush=True)
 
 
 MASS_METRIC = "meson pole masses solved/s (solve_meson_mass, 8 mesons x 65,536 (T, mu) points)"
 # FP64 FLOPs per pole-mass query of this batch, counted once on 8,192 seeded
-# queries run one per wavefront (python3 -m julia_relaxtime_amd.mass_flop_probe):
+# queries run one per wavefront (python3 bench.py --flop-probe mass):
 # 821,068 per query (38.9 trust-region iterations on average; 36 % of the
 # queries stop at the start on a non-finite residual).  The kernel's time is
 # set by its longest serial chains (queries running the full 1000
@@ -355,20 +381,36 @@ MASS_METRIC = "meson pole masses solved/s (solve_meson_mass, 8 mesons x 65,536 (
 MASS_FLOPS_PER_QUERY = 821068.0
 
 
-def mass_cpu_baseline(q, G, K, n_sample=400, seed=11):
-    """The oracle's solve_meson_mass (Pi in C, trust region in Python, one
-    thread) on a seeded sample of the batch."""
+def _mass_worker(args):
+    """One process of the pole-mass CPU baseline: the oracle's
+    solve_meson_mass over its share of the sampled queries."""
+    rows, G, K = args
     from oracle import mesons as OM
-    sel = np.random.default_rng(seed).choice(q.size, n_sample, replace=False)
-    t0 = time.perf_counter()
-    for r in q[sel]:
+    for r in rows:
         qp = dict(m=dict(u=r["m_u"], s=r["m_s"]), mu=dict(u=r["mu_u"], s=r["mu_s"]), A=dict(u=r["A_u"], s=r["A_s"]))
         tp = dict(T=r["T"], Phi=r["Phi"], Phib=r["Phib"], xi=r["xi"])
         OM.solve_meson_mass(OM.MASS_MESONS[int(r["meson"])], qp, tp, G=G, K=K, k=r["k"])
-    dt = time.perf_counter() - t0
-    return dict(value=n_sample / dt, unit="queries/s", cores=1, kind="port",
-                sample=f"{n_sample} seeded-random queries of the batch (rng {seed}), oracle solve_meson_mass, "
-                       f"1 thread, {dt:.2f} s wall")
+    return len(rows)
+
+
+def mass_cpu_baseline(q, G, K, n_sample=None, seed=11, threads=None):
+    """The oracle's solve_meson_mass (Pi in C, trust region in Python) on a
+    seeded sample of the batch, spread over host_threads() processes
+    (spawned: the workers never touch the GPU)."""
+    import multiprocessing as mp
+    threads = threads or host_threads()
+    n_sample = n_sample or 50 * threads
+    sel = np.random.default_rng(seed).choice(q.size, n_sample, replace=False)
+    rows = [{k: q[i][k] for k in q.dtype.names} for i in sel]
+    parts = [(rows[j::threads], G, K) for j in range(threads)]
+    with mp.get_context("spawn").Pool(threads) as pool:
+        pool.map(_mass_worker, [(p[0][:1], G, K) for p in parts])   # start-up (imports) outside the timing
+        t0 = time.perf_counter()
+        done = sum(pool.map(_mass_worker, parts))
+        dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="queries/s", cores=threads, kind="port",
+                sample=f"{done} seeded-random queries of the batch (rng {seed}), oracle solve_meson_mass, "
+                       f"{threads} processes, {dt:.2f} s wall")
 
 
 def run_masses(args, w):
@@ -407,7 +449,7 @@ def run_masses(args, w):
                      "flops_per_query": MASS_FLOPS_PER_QUERY},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = mass_cpu_baseline(q, pctx.params.G, pctx.params.K)
+        line["cpu_baseline"] = dict(mass_cpu_baseline(q, pctx.params.G, pctx.params.K), **host_cpu())
     print(json.dumps(line), flush=True)
 
 
@@ -425,7 +467,7 @@ def dual_cpu_baseline(n_lines=40, threads=None):
     the lines of each mu step) on the first n_lines lines."""
     from julia_relaxtime_amd import dual_branch as DB
     from oracle import oracle as O
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_threads()
     T, xi, mu = dual_lines()
     pts = [0]
 
@@ -482,7 +524,7 @@ def run_dual(args, w):
                      "kernel_ms_per_step": kavg, "flops_per_node": C_NODE},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = dual_cpu_baseline()
+        line["cpu_baseline"] = dict(dual_cpu_baseline(), **host_cpu())
     print(json.dumps(line), flush=True)
 
 
@@ -498,7 +540,7 @@ def tmu_cpu_baseline(n_lines=24, threads=None):
     over the lines of each mu step) on the first n_lines temperatures."""
     from julia_relaxtime_amd import tmu_scan as TS
     from oracle import oracle as O
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_threads()
     Ts, mus = tmu_grid()
 
     def solve(T, mu, xi, seeds):
@@ -554,7 +596,7 @@ def run_tmu(args, w):
                      "kernel_ms_per_step": kavg, "flops_per_node": C_NODE},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = tmu_cpu_baseline()
+        line["cpu_baseline"] = dict(tmu_cpu_baseline(), **host_cpu())
     print(json.dumps(line), flush=True)
 
 
@@ -577,7 +619,7 @@ def maxwell_cpu_baseline(n_lines=45, threads=None):
     over the lines of each mu step) for the first n_lines lines."""
     from julia_relaxtime_amd import phase_transition as PT
     from oracle import oracle as O
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_threads()
     T, xi, mu = maxwell_lines()
     count = dict(points=0)
 
@@ -639,7 +681,7 @@ def run_maxwell(args, w):
                      "kernel_ms_per_step": kavg, "flops_per_node": C_NODE},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = maxwell_cpu_baseline()
+        line["cpu_baseline"] = dict(maxwell_cpu_baseline(), **host_cpu())
     print(json.dumps(line), flush=True)
 
 
@@ -656,6 +698,96 @@ def timed_steps(step, steps, warmup):
     return (time.perf_counter() - t0) / steps * 1e3, scan_ms
 
 
+# ---------------------------------------------------------------------------
+# FLOP probes (run under rocprofv3 FP64 VALU counters; each prints what it ran)
+# ---------------------------------------------------------------------------
+def flop_probe_node(n=65536, xi=0.3, t_num=4, seed=3):
+    """FP64 FLOPs of one node x flavour evaluation of the gap integrand on the
+    device routine itself: pnjl_gap_residual_batch (one node pass of every
+    flavour plus the dense assembly per state) twice on the same 65,536
+    symmetric states at xi = 0.3, with p = 64 and with p = 32 (t = 4).  The
+    difference of the two dispatches' FP64 counts (64 lanes each, FMA = 2,
+    transcendental = 1) over the difference in node evaluations (states x 2
+    flavours x 32 x 4) is C_NODE; the remainder is the per-state assembly."""
+    from julia_relaxtime_amd import _lib
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(50, 300, n) / HB
+    mu = rng.uniform(0, 450, n) / HB
+    x = np.stack([rng.uniform(-1.9, -0.1, n), np.zeros(n), rng.uniform(-2.3, -0.5, n), rng.uniform(0.1, 0.9, n),
+                  rng.uniform(0.1, 0.9, n)], 1)
+    x[:, 1] = x[:, 0]
+    ctx = _lib.Context(0)
+    out = {}
+    for p_num in (64, 32):
+        ctx.residual_batch(x, T, mu, np.full(n, xi), p_num, t_num)
+        out[p_num] = dict(states=n, flavours=2, node_evals=n * 2 * p_num * t_num, kernel_ms=ctx.last_kernel_ms())
+    return out
+
+
+def flop_probe_pi(n_sample=131072, seed=7):
+    """The polarization grid's cells as single channel-P queries with
+    PNJL_FLAG_ONE_PER_WAVE (one query per wavefront, so 1 lane per
+    wave-instruction is that query's own FP64 work; the grid kernel's own
+    counts include the inactive lanes of divergent waves).  Two channels share
+    one B0: per grid value the algorithmic cost is half a query's."""
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import polarization as PZ
+    from julia_relaxtime_amd.pnjl import context
+    from julia_relaxtime_amd.workloads import PI_K, PI_K0, pi_grid_points
+    ctx = context(0)
+    pts = pi_grid_points(ctx, _lib.default_opts(ftol=1e-10, max_iter=40))
+    rng = np.random.default_rng(seed)
+    pt = rng.integers(0, len(pts), n_sample)
+    pair = rng.integers(0, 3, n_sample)
+    i0 = rng.integers(0, PI_K0.size, n_sample)
+    j = rng.integers(0, PI_K.size, n_sample)
+    P = pts[pt]
+    q = np.zeros(n_sample, dtype=PZ.QUERY_DTYPE)
+    q["channel"] = PZ.CHANNEL_P
+    q["num_s"] = pair
+    q["k0"], q["k"] = PI_K0[i0], PI_K[j]
+    q["m1"] = np.where(pair == 2, P[:, 4], P[:, 3])
+    q["m2"] = np.where(pair == 0, P[:, 3], P[:, 4])
+    q["mu1"] = q["mu2"] = P[:, 1]
+    q["T"], q["xi"], q["Phi"], q["Phib"] = P[:, 0], P[:, 2], P[:, 5], P[:, 6]
+    PZ.polarization_batch(q, one_per_wave=True)
+    return {"queries": int(n_sample), "seed": seed, "cells": int(len(pts) * 3 * PI_K0.size * PI_K.size),
+            "kernel_ms": ctx.last_kernel_ms()}
+
+
+def flop_probe_prop(n_sample=65536):
+    """The first n_sample queries of the propagator workload with
+    PNJL_FLAG_ONE_PER_WAVE: one (pair, query) per wavefront."""
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import propagators as PR
+    from julia_relaxtime_amd.pnjl import context
+    from julia_relaxtime_amd.workloads import pi_grid_points, propagator_queries
+    ctx = context(0)
+    pts = pi_grid_points(ctx, _lib.default_opts(ftol=1e-10, max_iter=40))
+    q = propagator_queries(ctx, pts, n=1 << 20)[:n_sample]
+    PR.propagator_by_channel_batch(q, one_per_wave=True)
+    return {"queries": int(n_sample), "workload_queries": 1 << 20, "seed": 20261203,
+            "kernel_ms": ctx.last_kernel_ms()}
+
+
+def flop_probe_mass(n_sample=8192, seed=11):
+    """A seeded sample of the pole-mass workload's queries with
+    PNJL_FLAG_ONE_PER_WAVE (one query per wavefront)."""
+    from julia_relaxtime_amd import _lib
+    from julia_relaxtime_amd import propagators as PR
+    from julia_relaxtime_amd.pnjl import context
+    from julia_relaxtime_amd.workloads import meson_mass_points, meson_mass_queries
+    ctx = context(0)
+    q = meson_mass_queries(meson_mass_points(ctx, _lib.default_opts(ftol=1e-10, max_iter=40)))
+    sel = np.sort(np.random.default_rng(seed).choice(q.size, n_sample, replace=False))
+    r = PR.meson_mass_batch(q[sel], one_per_wave=True)
+    return {"queries": int(n_sample), "workload_queries": int(q.size), "seed": seed,
+            "iterations": int(r["iterations"].sum()), "kernel_ms": ctx.last_kernel_ms()}
+
+
+FLOP_PROBES = {"node": flop_probe_node, "pi": flop_probe_pi, "prop": flop_probe_prop, "mass": flop_probe_mass}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -668,7 +800,13 @@ def main():
                          "(profiling runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (calibrates HBM byte counters)")
+    ap.add_argument("--flop-probe", default=None, choices=sorted(FLOP_PROBES),
+                    help="run one FLOP probe (for rocprofv3 FP64 counters) instead of a workload")
     args = ap.parse_args()
+    if args.flop_probe:
+        import torch  # noqa: F401  (loads the HIP runtime before the library)
+        print(json.dumps(FLOP_PROBES[args.flop_probe]()), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "the scan would shard by (xi, T) slabs with no collectives, which is not built", file=sys.stderr)
@@ -814,7 +952,7 @@ def main():
         torch.cuda.synchronize()
         line["config"]["copy_probe_bytes"] = int(src.numel() * 4)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w, T, mu, xi)
+        line["cpu_baseline"] = dict(cpu_baseline(w, T, mu, xi), **host_cpu())
     print(json.dumps(line), flush=True)
 
 

@@ -409,8 +409,11 @@ def maxwell_line(T_values, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, devi
     solved again on a grid `refine` times finer over its coexistence window
     (between the first quark-branch point and the last hadron-branch point,
     also when these miss each other by up to two coarse steps, with two
-    coarse steps of margin on each side), merged with its coarse curve, and
-    constructed again; the details record `refined`."""
+    coarse steps of margin on each side, clipped to the mu range), merged
+    with its coarse curve, and constructed again; the details record
+    `refined`.  A line whose two branches are the same root wherever both
+    exist (a crossover line above the CEP) has no coexistence window and is
+    not refined."""
     T_values = [float(T) for T in np.atleast_1d(T_values)]
     mu_values = np.asarray(list(mu_values), dtype=np.float64)
     per_line = np.ndim(xi) > 0
@@ -422,11 +425,14 @@ def maxwell_line(T_values, mu_values, xi=0.0, p_num=24, t_num=8, opts=None, devi
     for T, x, (mu, rho), res in zip(T_values, xi_l, curves, lines):
         m = maxwell_construction(mu, rho, **kw)
         h, q = res["hadron"]["valid"], res["quark"]["valid"]
-        if not m.converged and refine and step > 0 and h.any() and q.any():
+        overlap = h & q
+        crossover = overlap.any() and bool(np.all(res["same"][overlap]))
+        if not m.converged and refine and step > 0 and h.any() and q.any() and not crossover:
             lo, hi = mu_values[np.flatnonzero(q)[0]], mu_values[np.flatnonzero(h)[-1]]
             if lo <= hi + 2 * step:      # coexisting, or meeting between two grid points
                 lo, hi = min(lo, hi), max(lo, hi)
-                fine = np.arange(lo - 2 * step, hi + 2 * step + 1e-9, step / refine)
+                lo, hi = max(lo - 2 * step, mu_values[0]), min(hi + 2 * step, mu_values[-1])
+                fine = np.arange(lo, hi + 1e-9, step / refine)
                 mf, rf = s_curves([T], fine, x, p_num, t_num, opts, device, solver, chain_solver=chain_solver,
                                   point_solver=point_solver)[0]
                 m = maxwell_construction(np.concatenate([mu, mf]), np.concatenate([rho, rf]), **kw)

@@ -22,14 +22,39 @@ def _x_close(a, b, rtol=1e-10):
     return np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b))
 
 
-def assert_parity(r, o, max_chaotic=0.03, rtol=1e-10, selection=False):
+def _xdiff(a, b):
+    return np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)), axis=1)
+
+
+def ulp_unstable(rerun, T, mu, idx, o, rtol=1e-10):
+    """Points idx at which the oracle's own root moves when T or mu moves by
+    one ulp (nextafter up or down): the oracle is re-run there by
+    rerun(T', mu', idx) and compared with its root o at the exact input."""
+    idx = np.asarray(idx, dtype=np.int64)
+    out = np.zeros(idx.size, dtype=bool)
+    if idx.size == 0:
+        return out
+    up, dn = np.inf, -np.inf
+    for tT, tm in ((up, None), (dn, None), (None, up), (None, dn)):
+        Tp = T[idx] if tT is None else np.nextafter(T[idx], tT)
+        mp = mu[idx] if tm is None else np.nextafter(mu[idx], tm)
+        q = rerun(Tp, mp, idx)
+        c0 = o["converged"][idx] == 1
+        out |= (q["converged"] != o["converged"][idx]) | (c0 & (_xdiff(q["x"], o["x"][idx]) > rtol))
+    return out
+
+
+def assert_parity(r, o, T=None, mu=None, rerun=None, rtol=1e-10, selection=False, label=""):
     """Per-point parity of GPU result r with oracle result o.
 
-    A Newton run that wanders for many iterations from a far seed amplifies
-    last-bit differences (as it would between two CPUs): such points may land
-    on a different root.  They are recognised by a long winning run or by a
-    different fallback path, must stay rare, and every other point must agree
-    to rtol in x and Omega and to 1e-9 in the thermodynamics.
+    Every converged point agrees to rtol in x and Omega and to 1e-9 in the
+    thermodynamics, and the converged flags agree, except at points where the
+    oracle itself is not stable: if T or mu moves by one ulp, the oracle's
+    own root (or its converged flag) changes there (a Newton run that
+    wanders from a far seed amplifies last-bit differences, and the GPU's
+    node sums are added in another order).  Those points are found by
+    re-running the oracle through rerun(T', mu', idx) at the mismatches only;
+    the counts are printed.
 
     selection=True (policies that keep the minimum-Omega root of several
     runs): two runs that stop within ftol of the same root tie in Omega to
@@ -37,19 +62,26 @@ def assert_parity(r, o, max_chaotic=0.03, rtol=1e-10, selection=False):
     x to 1e-8, a different winning run) counts as the same root, and its
     thermodynamics (first derivatives of P, steep in x at low T) are held to
     1e-6 instead of 1e-9."""
-    err = np.max(np.abs(r["x"] - o["x"]) / np.maximum(1.0, np.abs(o["x"])), axis=1)
+    n = r["x"].shape[0]
+    err = _xdiff(r["x"], o["x"])
     eo = np.abs(r["omega"] - o["omega"]) / np.abs(o["omega"])
     bad = (r["converged"] != o["converged"]) | ((o["converged"] == 1) & ((err > rtol) | (eo > rtol)))
     tie = np.zeros_like(bad)
     if selection:
         tie = (eo < 1e-12) & (err < 1e-8) & (r["iterations"] != o["iterations"]) & bad
         bad &= ~tie
-    chaotic = (np.maximum(r["iterations"], o["iterations"]) > 25) | (r["flags"] != o["flags"])
-    worst = np.argsort(-np.where(np.isnan(err), np.inf, err))[:5]
+    unstable = np.zeros(n, dtype=bool)
+    mism = np.flatnonzero(bad)
+    if mism.size and rerun is not None:
+        unstable[mism] = ulp_unstable(rerun, T, mu, mism, o, rtol)
+    stable_bad = bad & ~unstable
+    print(f"[parity {label}] {n} points: {int(bad.sum())} root mismatches, {int(unstable.sum())} of them at "
+          f"ulp-unstable points, {int(stable_bad.sum())} at stable points; {int(tie.sum())} Omega ties",
+          flush=True)
+    worst = np.flatnonzero(stable_bad)[:5]
     info = [(int(i), float(err[i]), int(r["iterations"][i]), int(o["iterations"][i]), int(r["flags"][i]),
              int(o["flags"][i])) for i in worst]
-    assert not np.any(bad & ~chaotic), f"non-chaotic mismatches: {info}"
-    assert chaotic[bad].size <= max_chaotic * r["x"].shape[0], f"too many mismatches: {info}"
+    assert stable_bad.sum() == 0, f"mismatches at ulp-stable points: {info}"
     ok = (o["converged"] == 1) & ~bad
     tol = np.where(tie, 1e-6, 1e-9)
     for k in ("pressure", "entropy", "energy", "rho_norm"):
@@ -59,6 +91,15 @@ def assert_parity(r, o, max_chaotic=0.03, rtol=1e-10, selection=False):
     dn = np.max(np.abs(r["n_q"] - o["n_q"]), axis=1)
     assert np.all(dn[ok] <= tol[ok])
     return ok
+
+
+def oracle_rerun(oracle, xi, policy, p_num, t_num, opts, seed=None):
+    """rerun(T', mu', idx) for assert_parity: the oracle at points idx with
+    the same xi, policy, seeds and options."""
+    def run(Tp, mp, idx):
+        return oracle.solve_batch(Tp, mp, xi[idx], policy=policy, seed=None if seed is None else seed[idx],
+                                  p_num=p_num, t_num=t_num, opts=opts, nthreads=8)
+    return run
 
 
 def test_residual_and_jacobian_match_oracle(gpu_ctx, oracle):
@@ -111,18 +152,22 @@ def test_policy_parity_small_grid(gpu_ctx, oracle, policy, name):
     # |res| < 1e-10 gate: both solvers stop within 1e-10 of the same root
     r = gpu_ctx.solve_batch(T, mu, xi, policy=policy, p_num=64, t_num=4,
                             opts=_lib.default_opts(ftol=1e-10, max_iter=40))
-    o = oracle.solve_batch(T, mu, xi, policy=policy, p_num=64, t_num=4, nthreads=8,
-                           opts=oracle.default_opts(ftol=1e-10, max_iter=40))
+    oo = oracle.default_opts(ftol=1e-10, max_iter=40)
+    o = oracle.solve_batch(T, mu, xi, policy=policy, p_num=64, t_num=4, nthreads=8, opts=oo)
     assert o["converged"].mean() > 0.9
-    ok = assert_parity(r, o, selection=(policy == 3))
+    ok = assert_parity(r, o, T, mu, oracle_rerun(oracle, xi, policy, 64, 4, oo), selection=(policy == 3),
+                       label=name)
     # integer by-products: identical except at points whose residual sits on the ftol edge.
     # MultiSeed: several candidates reach the same root and argmin Omega then
     # ties to rounding, so the winning candidate (and its count) may differ.
     # Points that went through the MultiSeed fallback are excluded for the same reason.
     plain = (r["flags"] == 0) & (o["flags"] == 0)
+    same_it = np.mean(r["iterations"][ok & plain] == o["iterations"][ok & plain])
+    same_fl = np.mean(r["flags"][ok] == o["flags"][ok])
+    print(f"[parity {name}] iterations equal on {same_it:.4f} of plain points, flags on {same_fl:.4f}", flush=True)
     if policy != 3:
-        assert np.mean(r["iterations"][plain] == o["iterations"][plain]) > 0.97
-        assert np.mean(r["flags"] == o["flags"]) > 0.9
+        assert same_it > 0.97
+        assert same_fl > 0.9
 
 
 def test_explicit_seed_and_asymmetric_states(gpu_ctx, oracle):
@@ -139,9 +184,9 @@ def test_explicit_seed_and_asymmetric_states(gpu_ctx, oracle):
     seed = base * (1 + rng.uniform(-0.05, 0.05, base.shape))
     seed[: n // 2, 1] = seed[: n // 2, 0]
     r = gpu_ctx.solve_batch(T, mu, xi, policy=4, seed=seed, p_num=32, t_num=4, opts=_lib.default_opts(ftol=1e-10))
-    o = oracle.solve_batch(T, mu, xi, policy=4, seed=seed, p_num=32, t_num=4, nthreads=8,
-                           opts=oracle.default_opts(ftol=1e-10))
-    ok = assert_parity(r, o, max_chaotic=0.02)
+    oo = oracle.default_opts(ftol=1e-10)
+    o = oracle.solve_batch(T, mu, xi, policy=4, seed=seed, p_num=32, t_num=4, nthreads=8, opts=oo)
+    ok = assert_parity(r, o, T, mu, oracle_rerun(oracle, xi, 4, 32, 4, oo, seed=seed), label="explicit seeds")
     # asymmetric seeds converge to the symmetric root: phi_u = phi_d to 1e-9
     assert np.all(np.abs(r["x"][ok, 0] - r["x"][ok, 1]) < 1e-9)
 
@@ -192,15 +237,21 @@ def test_continuity_chain_c1(gpu_ctx, oracle):
                             opts=_lib.default_opts(ftol=1e-10, chain_len=n))
     run = lambda TT: oracle.solve_batch(TT, mu, xi, policy=5, p_num=64, t_num=8,
                                         opts=oracle.default_opts(ftol=1e-10, chain_len=n))
-    o, o_lo, o_hi = run(T), run(T * (1 - 1e-15)), run(T * (1 + 1e-15))
-    d = lambda a, b: np.max(np.abs(a["x"] - b["x"]) / np.maximum(1.0, np.abs(b["x"])), axis=1)
-    unstable = (d(o_lo, o) > 1e-10) | (d(o_hi, o) > 1e-10)
-    assert unstable.mean() < 0.2
+    o = run(T)
     assert o["converged"].all() and r["converged"].all()
     assert np.all(r["residual_norm"] < 1e-10)
-    stable = ~unstable
-    assert np.all(_x_close(r["x"][stable], o["x"][stable]))
-    assert np.all(np.abs(r["omega"][stable] - o["omega"][stable]) <= 1e-10 * np.abs(o["omega"][stable]))
+    bad = ~_x_close(r["x"], o["x"]).all(axis=1) | (np.abs(r["omega"] - o["omega"]) > 1e-10 * np.abs(o["omega"]))
+    # the whole chain is re-run with every T (or mu) moved by one ulp
+    unstable = np.zeros(n, dtype=bool)
+    if bad.any():
+        for Tp, mp in ((np.nextafter(T, np.inf), mu), (np.nextafter(T, -np.inf), mu),
+                       (T, np.nextafter(mu, np.inf)), (T, np.nextafter(mu, -np.inf))):
+            q = oracle.solve_batch(Tp, mp, xi, policy=5, p_num=64, t_num=8,
+                                   opts=oracle.default_opts(ftol=1e-10, chain_len=n))
+            unstable |= _xdiff(q["x"], o["x"]) > 1e-10
+    print(f"[parity c1 chain] {n} points: {int(bad.sum())} root mismatches, {int((bad & unstable).sum())} of them "
+          f"at ulp-unstable points", flush=True)
+    assert not np.any(bad & ~unstable), np.flatnonzero(bad & ~unstable)[:10]
 
 
 def test_c2_full_grid_properties(gpu_ctx, oracle):
@@ -221,9 +272,10 @@ def test_c2_full_grid_properties(gpu_ctx, oracle):
     assert np.allclose(r["pressure"], -r["omega"], rtol=0, atol=0, equal_nan=True)
     rng = np.random.default_rng(20261202)
     idx = rng.choice(T.size, 400, replace=False)
-    o = oracle.solve_batch(T[idx], mu[idx], xi[idx], policy=0, p_num=64, t_num=4,
-                           opts=oracle.default_opts(ftol=1e-10, max_iter=40), nthreads=8)
-    assert_parity({k: v[idx] for k, v in r.items()}, o, max_chaotic=0.02)
+    oo = oracle.default_opts(ftol=1e-10, max_iter=40)
+    o = oracle.solve_batch(T[idx], mu[idx], xi[idx], policy=0, p_num=64, t_num=4, opts=oo, nthreads=8)
+    assert_parity({k: v[idx] for k, v in r.items()}, o, T[idx], mu[idx],
+                  oracle_rerun(oracle, xi[idx], 0, 64, 4, oo), label="c2 sample")
 
 
 def test_launch_geometry_does_not_change_results(gpu_ctx):

@@ -235,3 +235,27 @@ def test_maxwell_line_refines_next_to_the_cep(oracle):
         m = P.maxwell_line([T], mus, xi, solver=_oracle_solver(oracle), min_samples=8, detect_min_points=5)[T]
         assert m.converged and "refined" in m.details
         assert abs(m.mu_transition - float(bd[(xi, T)]["mu_transition_MeV"])) < 0.01
+
+
+def test_maxwell_line_does_not_refine_crossover_lines(oracle):
+    """Above each xi's CEP the hadron and quark branches are the same root
+    wherever both exist: the construction fails (no S-shape) and the line is
+    not re-solved on a finer grid; no solve leaves the requested mu range."""
+    P = PT()
+    mus = np.arange(250.0, 420.0 + 1e-9, 0.25)
+    solve = _oracle_solver(oracle)
+    for xi, T in ((0.0, 140.0), (0.2, 130.0), (0.4, 120.0)):
+        seen = {"points": 0, "mu_min": np.inf, "mu_max": -np.inf}
+
+        def counting(Tq, muq, xiq, policy, seed):
+            seen["points"] += np.size(Tq)
+            seen["mu_min"] = min(seen["mu_min"], float(np.min(muq)) * 197.327)
+            seen["mu_max"] = max(seen["mu_max"], float(np.max(muq)) * 197.327)
+            return solve(Tq, muq, xiq, policy, seed)
+        m = P.maxwell_line([T], mus, xi, solver=counting, min_samples=8, detect_min_points=5)[T]
+        in_line = seen["points"]
+        seen["points"] = 0
+        P.s_curves([T], mus, xi, solver=counting)
+        assert not m.converged and "refined" not in m.details
+        assert in_line == seen["points"], (in_line, seen["points"])    # no second, finer pass
+        assert seen["mu_min"] >= 250.0 - 1e-9 and seen["mu_max"] <= 420.0 + 1e-9
