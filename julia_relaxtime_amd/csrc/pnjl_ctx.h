@@ -11,9 +11,11 @@
 #include "../../include/pnjl_hip.h"
 #include "pnjl_device.h"
 
-// device work counters of the staged pipeline (u64 words)
+// device work-queue counters of a solve (u64 words: the Q_* words of the
+// scan kernel, then two counters of the learned-seed plan's ordering pass)
 constexpr int kQueueWords = 16;
-// persistent-kernel launches timed per solve (the staged pipeline has at most 6)
+constexpr int kOrderWords = 8;
+// persistent-kernel launches timed per solve
 constexpr int kMaxTimedScans = 8;
 
 struct pnjl_ctx {
@@ -24,7 +26,7 @@ struct pnjl_ctx {
     std::string err;
     pnjl_launch_cfg cfg{4, 4, 8, 0};
     std::map<std::pair<int, int>, double*> tabs;
-    double* scratch = nullptr;
+    double* scratch = nullptr;          // per-group candidate records of the scan kernel
     size_t scratch_bytes = 0;
     unsigned long long* queue = nullptr;
     double* bnd = nullptr;

@@ -181,19 +181,47 @@ int pinned_buf(pnjl_ctx* ctx, size_t bytes, void** out)
 
 bool valid_lpp(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64; }
 
-void launch_scan(pnjl_ctx* ctx, const pnjl::ScanArgs& A, int lpp, int blocks, size_t lds, hipStream_t st)
+// Block geometry of the scan kernel for lanes per point `lpp` and node table
+// (pn, tn): blocks of whole wavefronts holding at most 64 groups, with the
+// node table and every group's LDS state within 64 KB; a lane count whose
+// groups do not fit next to a large table is raised to the next power of
+// two (a different summation order, equal to rounding).
+struct ScanGeom {
+    int lpp, threads, groups_per_block;
+    size_t lds;
+};
+
+ScanGeom scan_geom(int lpp, int pn, int tn)
+{
+    const size_t tab = (size_t)pnjl::table_len(pn, tn) * sizeof(double);
+    const size_t group = (size_t)pnjl::STATE_STRIDE * sizeof(double);
+    const size_t limit = 64 * 1024;
+    for (int l = lpp; l <= 64; l *= 2) {
+        for (int waves = 4; waves >= 1; waves--) {
+            const int threads = 64 * waves;
+            const int gpb = threads / l;
+            if (gpb > 64) continue;
+            const size_t lds = tab + (size_t)gpb * group;
+            if (lds <= limit) return ScanGeom{l, threads, gpb, lds};
+        }
+    }
+    return ScanGeom{64, 64, 1, tab + group};
+}
+
+void launch_scan(pnjl_ctx* ctx, const pnjl::ScanArgs& A, const ScanGeom& G, int blocks, hipStream_t st)
 {
     using namespace pnjl;
     const int k = ctx->n_scans < kMaxTimedScans ? ctx->n_scans++ : -1;
     if (k >= 0) (void)hipEventRecord(ctx->ev_scan[2 * k], st);
-    switch (lpp) {
-    case 1: hipLaunchKernelGGL(scan_kernel<1>, dim3(blocks), dim3(256), lds, st, A); break;
-    case 2: hipLaunchKernelGGL(scan_kernel<2>, dim3(blocks), dim3(256), lds, st, A); break;
-    case 4: hipLaunchKernelGGL(scan_kernel<4>, dim3(blocks), dim3(256), lds, st, A); break;
-    case 8: hipLaunchKernelGGL(scan_kernel<8>, dim3(blocks), dim3(256), lds, st, A); break;
-    case 16: hipLaunchKernelGGL(scan_kernel<16>, dim3(blocks), dim3(256), lds, st, A); break;
-    case 32: hipLaunchKernelGGL(scan_kernel<32>, dim3(blocks), dim3(256), lds, st, A); break;
-    default: hipLaunchKernelGGL(scan_kernel<64>, dim3(blocks), dim3(256), lds, st, A); break;
+    const dim3 grid(blocks), block(G.threads);
+    switch (G.lpp) {
+    case 1: hipLaunchKernelGGL(scan_kernel<1>, grid, block, G.lds, st, A); break;
+    case 2: hipLaunchKernelGGL(scan_kernel<2>, grid, block, G.lds, st, A); break;
+    case 4: hipLaunchKernelGGL(scan_kernel<4>, grid, block, G.lds, st, A); break;
+    case 8: hipLaunchKernelGGL(scan_kernel<8>, grid, block, G.lds, st, A); break;
+    case 16: hipLaunchKernelGGL(scan_kernel<16>, grid, block, G.lds, st, A); break;
+    case 32: hipLaunchKernelGGL(scan_kernel<32>, grid, block, G.lds, st, A); break;
+    default: hipLaunchKernelGGL(scan_kernel<64>, grid, block, G.lds, st, A); break;
     }
     if (k >= 0) (void)hipEventRecord(ctx->ev_scan[2 * k + 1], st);
 }
@@ -216,9 +244,14 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     A.P = ctx->dp;
     A.n_points = n;
     A.policy = policy;
-    bool chains = pnjl::chain_policy(policy);
+    const bool chains = pnjl::chain_policy(policy);
+    const bool surr = (policy == PNJL_SEED_SURROGATE), multi = (policy == PNJL_SEED_MULTI);
+    const bool mr3 = (policy == PNJL_SEED_MULTIROOT3);
+    A.plan = chains ? pnjl::PLAN_CHAINS
+                    : (surr ? pnjl::PLAN_SURR : (multi ? pnjl::PLAN_MULTI : (mr3 ? pnjl::PLAN_MR3 : pnjl::PLAN_DEFAULT)));
     A.chain_len = chains ? (o->chain_len > 0 ? o->chain_len : (int)std::min<int64_t>(n, 1 << 30)) : 1;
     A.n_items = chains ? (n + A.chain_len - 1) / A.chain_len : n;
+    A.mult = multi ? 6 : (mr3 ? 3 : 1);
     A.T = T;
     A.mu = mu;
     A.xi = xi;
@@ -255,155 +288,117 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     A.bnd = ctx->bnd;
     A.nb = ctx->nb;
     A.max_trips = 9LL * (2LL * o->max_iter + 4) + 16;
-    int blocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / 4);
-    int64_t groups = (int64_t)blocks * 256 / cfg.lanes_per_point;
-    // never more groups than work items (fewer blocks for small batches)
-    int64_t need_groups = std::max<int64_t>(1, (A.n_items + A.chunk - 1) / A.chunk);
-    if (groups > need_groups) {
-        int64_t gpb = 256 / cfg.lanes_per_point;
-        blocks = (int)std::max<int64_t>(1, (need_groups + gpb - 1) / gpb);
-        groups = (int64_t)blocks * gpb;
-    }
-    // the later pipeline stages run the full persistent grid: size the
-    // per-group scratch for the larger of the two grids
-    const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / 4);
-    const int tail = cfg.tail_lanes_per_point > 0 ? cfg.tail_lanes_per_point : cfg.lanes_per_point;
-    const int64_t cap_groups = std::max<int64_t>(
-        groups, (int64_t)fullblocks * 256 / std::min(cfg.lanes_per_point, tail));
-    A.n_groups = cap_groups;
-    rc = ensure_scratch(ctx, (size_t)cap_groups * pnjl::SCRATCH_FIELDS * sizeof(double));
+    const ScanGeom G = scan_geom(cfg.lanes_per_point, pn, tn);
+    const int waves_per_block = G.threads / 64;
+    int blocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / waves_per_block);
+    // never more groups than work items: a small batch gets a small grid
+    const int64_t work = chains ? A.n_items : n * A.mult;
+    const int64_t need_groups = std::max<int64_t>(1, (work + A.chunk - 1) / A.chunk);
+    if ((int64_t)blocks * G.groups_per_block > need_groups)
+        blocks = (int)std::max<int64_t>(1, (need_groups + G.groups_per_block - 1) / G.groups_per_block);
+    // the later pipeline stages run the full persistent grid: records for the larger of the two
+    const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / waves_per_block);
+    const int64_t groups = std::max<int64_t>((int64_t)blocks, (int64_t)fullblocks) * G.groups_per_block;
+    A.n_groups = groups;
+    rc = ensure_scratch(ctx, (size_t)groups * pnjl::REC_AREA * sizeof(double));
     if (rc) return rc;
-    A.scratch = ctx->scratch;
-    size_t lds = sizeof(double) * (pnjl::kMathTab + (size_t)pn + tn + (size_t)pn * tn + pn);
+    A.records = ctx->scratch;
+    unsigned long long* q = ctx->queue;
     if (chains) {
-        A.stage = pnjl::STAGE_NONE;
-        hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, ctx->queue, kQueueWords);
+        hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, kQueueWords);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
-        launch_scan(ctx, A, cfg.lanes_per_point, blocks, lds, stream);
+        launch_scan(ctx, A, G, blocks, stream);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ctx->ev1, stream));
         return PNJL_OK;
     }
-    // ---- staged pipeline: [learned seed] -> primary attempt -> MultiSeed candidates -> selection
+    // ---- per-point pipeline: stage launches over device lists, then the per-point selection
     const size_t rec = pnjl::REC_SIZE * sizeof(double);
-    const bool surr = (policy == PNJL_SEED_SURROGATE), multi = (policy == PNJL_SEED_MULTI);
-    const bool mr3 = (policy == PNJL_SEED_MULTIROOT3);
-    void *recS = nullptr, *recP = nullptr, *recM = nullptr, *listP = nullptr, *listM = nullptr, *mslot = nullptr;
-    if ((rc = stage_buf(ctx, 40, (size_t)n * rec * (multi ? 6 : 6), &recM))) return rc;
+    const bool omega_judge = surr && ctx->jmask && o->surrogate_judge == PNJL_JUDGE_OMEGA;
+    void *recS = nullptr, *recP = nullptr, *recM = nullptr, *mlist = nullptr, *mslot = nullptr, *order = nullptr;
+    void *recJ = nullptr, *jlist = nullptr, *jslot = nullptr, *jb = nullptr, *j2list = nullptr;
+    if ((rc = stage_buf(ctx, 40, (size_t)n * rec * 6, &recM))) return rc;
     if (!multi && !mr3) {
         if ((rc = stage_buf(ctx, 41, (size_t)n * rec, &recP))) return rc;
-        if ((rc = stage_buf(ctx, 42, (size_t)n * 4, &listM))) return rc;
+        if ((rc = stage_buf(ctx, 42, (size_t)n * 4, &mlist))) return rc;
         if ((rc = stage_buf(ctx, 43, (size_t)n * 4, &mslot))) return rc;
     }
-    void *recJ = nullptr, *listJ = nullptr, *jslot = nullptr, *jb = nullptr, *listJ2 = nullptr;
-    const bool omega_judge = surr && ctx->jmask && o->surrogate_judge == PNJL_JUDGE_OMEGA;
     if (surr) {
         if ((rc = stage_buf(ctx, 44, (size_t)n * rec, &recS))) return rc;
-        if ((rc = stage_buf(ctx, 45, (size_t)n * 4, &listP))) return rc;
+        if ((rc = stage_buf(ctx, 45, (size_t)n * 4, &order))) return rc;
         if (omega_judge) {
             if ((rc = stage_buf(ctx, 46, (size_t)n * pnjl::JUDGE_SEEDS * rec, &recJ))) return rc;
-            if ((rc = stage_buf(ctx, 47, (size_t)n * 4, &listJ))) return rc;
+            if ((rc = stage_buf(ctx, 47, (size_t)n * 4, &jlist))) return rc;
             if ((rc = stage_buf(ctx, 48, (size_t)n * 4, &jslot))) return rc;
             if ((rc = stage_buf(ctx, 49, (size_t)n * 4, &jb))) return rc;
-            if ((rc = stage_buf(ctx, 50, (size_t)n * 4, &listJ2))) return rc;
+            if ((rc = stage_buf(ctx, 50, (size_t)n * 4, &j2list))) return rc;
         }
     }
-    // q[0..3]: stage work counters, q[4]: |listP|, q[5]: |listM|, q[6]: |listJ|, q[7]: judge work
-    // counter, q[8]: |listJ2|, q[9]: second judge work counter
-    unsigned long long* q = ctx->queue;
+    A.recS = (double*)recS;
+    A.recP = (double*)recP;
+    A.recM = (double*)recM;
+    A.mlist = (int32_t*)mlist;
+    A.mslot = (int32_t*)mslot;
+    if (surr && ctx->jmask) {
+        A.judge_mask = ctx->jmask;
+        for (int k = 0; k < 3; k++) {
+            A.jdim[k] = ctx->jdim[k];
+            A.jlo[k] = ctx->jlo[k];
+            A.jhi[k] = ctx->jhi[k];
+        }
+    }
+    if (omega_judge) {
+        A.recJ = (double*)recJ;
+        A.jlist = (int32_t*)jlist;
+        A.jslot = (int32_t*)jslot;
+        A.jb = (int32_t*)jb;
+        A.j2list = (int32_t*)j2list;
+    }
+    // queue words: q[0] the running stage's item counter (cleared before each
+    // stage), q[Q_MCOUNT..Q_J2COUNT] the list lengths, q[kOrderWords..] the
+    // ordering pass's counters
     HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
     hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, kQueueWords);
     if (A.passes)
         hipLaunchKernelGGL(pnjl::zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A.passes, n);
     if (A.steps)
         hipLaunchKernelGGL(pnjl::zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A.steps, n);
-    HIPCHK(ctx, hipGetLastError());
-    pnjl::ScanArgs S = A;
-    S.mslot = (int32_t*)mslot;
-    if (multi || mr3) {
-        S.stage = multi ? pnjl::STAGE_MULTI : pnjl::STAGE_MR3;
-        S.mult = multi ? 6 : 3;
-        S.item_list = nullptr;
-        S.queue = q + 2;
-        S.rec_out = (double*)recM;
-        launch_scan(ctx, S, cfg.lanes_per_point, fullblocks, lds, stream);
-    } else {
-        if (surr) {
-            S.stage = pnjl::STAGE_SURR;
-            S.mult = 1;
-            S.item_list = nullptr;
-            S.queue = q + 0;
-            S.rec_out = (double*)recS;
-            S.next_list = (int32_t*)listP;
-            S.next_count = q + 4;
-            if (ctx->jmask) {
-                S.judge_mask = ctx->jmask;
-                for (int k = 0; k < 3; k++) {
-                    S.jdim[k] = ctx->jdim[k];
-                    S.jlo[k] = ctx->jlo[k];
-                    S.jhi[k] = ctx->jhi[k];
-                }
-                if (omega_judge) {
-                    S.judge_list = (int32_t*)listJ;
-                    S.judge_count = q + 6;
-                    S.jslot = (int32_t*)jslot;
-                    S.jb = (int32_t*)jb;
-                }
-            }
-            launch_scan(ctx, S, cfg.lanes_per_point, blocks, lds, stream);
-            HIPCHK(ctx, hipGetLastError());
-        }
-        S.stage = pnjl::STAGE_PRIMARY;
-        S.mult = 1;
-        S.item_list = surr ? (const int32_t*)listP : nullptr;
-        S.item_count = q + 4;
-        S.queue = q + 1;
-        S.rec_out = (double*)recP;
-        S.next_list = (int32_t*)listM;
-        S.next_count = q + 5;
-        launch_scan(ctx, S, surr ? tail : cfg.lanes_per_point, surr ? fullblocks : blocks, lds, stream);
-        HIPCHK(ctx, hipGetLastError());
-        S.stage = pnjl::STAGE_MULTI;
-        S.mult = 6;
-        S.item_list = (const int32_t*)listM;
-        S.item_count = q + 5;
-        S.queue = q + 2;
-        S.rec_out = (double*)recM;
-        S.next_list = nullptr;
-        S.next_count = nullptr;
-        S.judge_mask = nullptr;
-        S.jslot = nullptr;
-        launch_scan(ctx, S, tail, fullblocks, lds, stream);
-        if (omega_judge) {
-            HIPCHK(ctx, hipGetLastError());
-            S.stage = pnjl::STAGE_JUDGE;
-            S.mult = 2;
-            S.item_list = (const int32_t*)listJ;
-            S.item_count = q + 6;
-            S.queue = q + 7;
-            S.rec_out = (double*)recJ;
-            S.jb = (int32_t*)jb;
-            S.judge2_list = (int32_t*)listJ2;
-            S.judge2_count = q + 8;
-            launch_scan(ctx, S, tail, fullblocks, lds, stream);
-            HIPCHK(ctx, hipGetLastError());
-            S.stage = pnjl::STAGE_JUDGE2;
-            S.mult = pnjl::JUDGE_SEEDS - 2;
-            S.item_list = (const int32_t*)listJ2;
-            S.item_count = q + 8;
-            S.queue = q + 9;
-            S.jslot = (int32_t*)jslot;
-            S.jb = nullptr;
-            S.judge2_list = nullptr;
-            S.judge2_count = nullptr;
-            launch_scan(ctx, S, tail, fullblocks, lds, stream);
-        }
+    if (surr && ctx->jmask && o->surrogate_judge == PNJL_JUDGE_REFERENCE) {
+        // the judge's plain points first: they run longest
+        hipLaunchKernelGGL(pnjl::surr_order_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A,
+                           (int32_t*)order, q + kOrderWords);
+        A.order = (const int32_t*)order;
     }
     HIPCHK(ctx, hipGetLastError());
+    // stage 1: every point (or every (point, seed) of the multi-seed plans)
+    pnjl::ScanArgs S = A;
+    S.kind = surr ? pnjl::K_SURR : (multi ? pnjl::K_MPOL : (mr3 ? pnjl::K_MR3 : pnjl::K_PRIMARY));
+    launch_scan(ctx, S, G, blocks, stream);
+    HIPCHK(ctx, hipGetLastError());
+    // the later stages run on the full persistent grid
+    auto next_stage = [&](int kind, int mult, int k_base, const void* list, int count_word) -> int {
+        hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q + pnjl::Q_HEAD, 1);
+        pnjl::ScanArgs L = A;
+        L.kind = kind;
+        L.mult = mult;
+        L.k_base = k_base;
+        L.order = nullptr;
+        L.item_list = (const int32_t*)list;
+        L.item_count = q + count_word;
+        launch_scan(ctx, L, G, fullblocks, stream);
+        HIPCHK(ctx, hipGetLastError());
+        return PNJL_OK;
+    };
+    if (!multi && !mr3 && o->auto_multiseed_fallback)
+        if ((rc = next_stage(pnjl::K_MULTI, 6, 0, mlist, pnjl::Q_MCOUNT))) return rc;
+    if (omega_judge) {
+        if ((rc = next_stage(pnjl::K_JUDGE, 2, 0, jlist, pnjl::Q_JCOUNT))) return rc;
+        if ((rc = next_stage(pnjl::K_JUDGE, pnjl::JUDGE_SEEDS - 2, 2, j2list, pnjl::Q_J2COUNT))) return rc;
+    }
     pnjl::FinArgs F{};
     F.n = n;
-    F.plan = multi ? pnjl::PLAN_MULTI : (mr3 ? pnjl::PLAN_MR3 : (surr ? pnjl::PLAN_SURR : pnjl::PLAN_DEFAULT));
+    F.plan = A.plan;
     F.auto_ms = o->auto_multiseed_fallback;
     F.recS = (const double*)recS;
     F.recP = (const double*)recP;
@@ -814,12 +809,9 @@ int pnjl_gap_residual_batch(pnjl_ctx* ctx, int64_t n, const double* x, const dou
     R.tab = tab;
     R.pn = pn;
     R.tn = tn;
-    rc = ensure_scratch(ctx, (size_t)n * pnjl::SCRATCH_FIELDS * sizeof(double));
-    if (rc) return rc;
-    R.scratch = ctx->scratch;
     const int LPP = 16;
     int64_t blocks = (n * LPP + 255) / 256;
-    size_t lds = sizeof(double) * (pnjl::kMathTab + (size_t)pn + tn + (size_t)pn * tn + pn);
+    const size_t lds = sizeof(double) * ((size_t)pnjl::table_len(pn, tn) + (256 / LPP) * pnjl::STATE_STRIDE);
     HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
     hipLaunchKernelGGL(pnjl::residual_kernel<LPP>, dim3((unsigned)blocks), dim3(256), lds, st, R);
     HIPCHK(ctx, hipGetLastError());
@@ -891,11 +883,6 @@ int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* 
             for (int w : {8, 12, 16})
                 if (!(c == 1 && w == 12))
                     if (int rc = trial(pnjl_launch_cfg{lanes[i], c, w, 0}, nullptr)) return rc;
-    // 3) lanes of the later (fallback / MultiSeed / judge) stages
-    const pnjl_launch_cfg b2 = bcfg;
-    for (int t : {4, 16, 32})
-        if (t != b2.lanes_per_point)
-            if (int rc = trial(pnjl_launch_cfg{b2.lanes_per_point, b2.chunk, b2.waves_per_cu, t}, nullptr)) return rc;
     ctx->cfg = bcfg;
     if (best) *best = bcfg;
     if (ns_per_point) *ns_per_point = bms * 1e6 / (double)sample;

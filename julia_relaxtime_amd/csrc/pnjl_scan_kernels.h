@@ -6,9 +6,6 @@
 // wavefront.  Each lane integrates every LPP-th momentum node; the group's
 // partial sums are combined with xor-butterfly shuffles, so every lane of a
 // group holds bit-identical sums and runs the identical control logic.
-// Items (points, or continuity chains) are pulled from a device work queue
-// (one atomic counter, `chunk` items per pop), so points that need 3 and 30
-// Newton iterations share the launch and no group idles while work is left.
 //
 // One trip of the loop = one node pass at the group's current state x
 // (Omega, F = grad P, J = dF/dx and the thermodynamic sums, all fused) plus
@@ -20,13 +17,26 @@
 //   solve_multi + default_omega_selector                          ImplicitSolver.jl:532-559
 //   ContinuitySeed / PhaseAwareContinuitySeed chains               SeedStrategies.jl:303-345, 762-888
 //
-// Register budget: the node pass is the hot loop and keeps one flavour's 13
-// accumulators live; reduced sums are parked in a per-group scratch area in
-// HBM ([group][field], every lane writes and re-reads only its own writes),
-// and the control step (a separate non-inlined function) rebuilds Omega, F
-// and the packed symmetric Jacobian from them.  Trust-region state and
-// candidate records also live in that scratch, so the common Newton path
-// holds only x, xold and a 5x5 LU in registers.
+// Per-point policies run as a few persistent launches ("stages"), each over
+// a queue of items pulled with one atomic counter: stage 1 takes every point
+// -- the learned-seed plan's learned attempt continues in the same group
+// with the plain (DefaultSeed) attempt where the seed fails or the judge
+// sends the point to the reference policy, and those judged points are
+// queued first, since they run longest; a point whose plain attempt fails is
+// appended to the MultiSeed list, a judged root to the Omega judge's list.
+// Stage 2 runs the six MultiSeed candidates of every listed point, and the
+// Omega judge's stages its candidates.  A group leaves as soon as its
+// stage's queue is drained.  Chain policies (continuity, phase-aware and
+// branch chains) run one chain per group with the sequential fallback logic
+// of solve().
+//
+// Storage: the group's hot state -- the reduced node sums of the last pass,
+// the trust-region state, the attempt's seed, the previous chain solution
+// and x_old -- lives in LDS next to the node tables (STATE_STRIDE doubles
+// per group), and every function is inlined, so no call frames or spills
+// go to memory.  Candidate records (x, Omega, thermodynamics, flags), which
+// are written once per attempt end or accepted trust-region step, live in a
+// small per-group area in HBM.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -34,6 +44,11 @@
 
 #include "../../include/pnjl_hip.h"
 #include "pnjl_device.h"
+
+// Every function below evaluates exactly the written operations (no
+// compiler contraction of a*b+c into an FMA): the same arithmetic rounds the
+// same way in every kernel it is inlined into, whatever the launch geometry.
+#pragma clang fp contract(off)
 
 namespace pnjl {
 
@@ -45,7 +60,7 @@ __device__ __forceinline__ double qnan() { return __longlong_as_double(0x7ff8000
 struct ScanArgs {
     DevParams P;
     int64_t n_points;
-    int64_t n_items;      // points, or chains for the chain policies
+    int64_t n_items;      // chain policies: chains
     int policy;
     int chain_len;
     const double* T;
@@ -68,66 +83,82 @@ struct ScanArgs {
     int32_t* flags;
     int32_t* steps;
     int32_t* passes;
-    // node table (staged in LDS): pp[pn] | cc[tn] | C[pn*tn] | Cs[pn]
+    // node table (staged in LDS): math tables | pp[pn] | cc[tn] | C[pn*tn] | Cs[pn]
     const double* tab;
     int pn, tn;
     // solver options
     double xtol, ftol, resmax, phi_tol;
     int max_iter, tr_fb, auto_ms, surr_cap;
     int judge_mode;                         // PNJL_JUDGE_* (SURROGATE inside the mask)
-    // work queue and scratch
+    // queue counters (Q_* words) and the per-group record area in HBM
     unsigned long long* queue;
     int chunk;
-    double* scratch;
+    double* records;                        // REC_AREA doubles per group
     int64_t n_groups;
     // phase-boundary tables: nb sets of {xi, T_CEP, n, pad, T[kMaxBndRows], mu[kMaxBndRows]}
     const double* bnd;
     int nb;
     int64_t max_trips;    // hard cap on trips per point (safety exit)
-    // staged pipeline for the per-point policies (stage != STAGE_NONE): each
-    // item is one single-seed attempt; results go to rec_out, failures are
-    // appended to next_list for the following stage
-    int stage;
-    int mult;                               // attempts per listed point
-    const int32_t* item_list;               // points of this stage (null: all points)
-    const unsigned long long* item_count;   // device count of item_list
-    double* rec_out;                        // REC_SIZE doubles per record
-    int32_t* next_list;
-    unsigned long long* next_count;
-    int32_t* mslot;                         // point -> its index in the MultiSeed list
-    // learned-seed judge: a converged root inside the ambiguity mask is
-    // appended to judge_list for the Omega check against the MultiSeed
-    // candidates (HADRON and hint-QUARK first; where one of them fails, the
-    // other four candidates via judge2_list)
+    // ---- per-point pipeline (plan != PLAN_CHAINS) ----
+    int plan;                               // Plan
+    int kind;                               // ItemKind of this stage's items
+    int mult;                               // items per listed point (1, or 2 / 3 / 4 / 6 seeds)
+    int k_base;                             // seed index of a point's first item
+    const int32_t* order;                   // stage 1: item -> point (null: identity)
+    const int32_t* item_list;               // later stages: list index -> point
+    const unsigned long long* item_count;   // later stages: list length (on the device)
+    double* recS;                           // learned-seed attempt, per point
+    double* recP;                           // primary (DefaultSeed) attempt, per point
+    double* recM;                           // MultiSeed candidates: 6 per listed point (or 6 / 3 per point)
+    int32_t* mlist;                         // MultiSeed list index -> point
+    int32_t* mslot;                         // point -> its MultiSeed list index
+    // learned-seed judge: a converged root inside the ambiguity mask gets the
+    // Omega check against the MultiSeed candidates (HADRON and hint-QUARK
+    // first; where one of them fails, the other four)
     const uint8_t* judge_mask;              // [nT][nmu][nxi] bins over [jlo, jhi]
     int jdim[3];
     double jlo[3], jhi[3];
-    int32_t* judge_list;
-    unsigned long long* judge_count;
-    int32_t* jslot;                         // point -> its index in the judge list (-1 none)
-    int32_t* jb;                            // judge index -> 1 once its point is in judge2_list
-    int32_t* judge2_list;
-    unsigned long long* judge2_count;
+    double* recJ;                           // JUDGE_SEEDS records per judged point
+    int32_t* jlist;                         // judge index -> point
+    int32_t* jslot;                         // point -> its judge index (-1 none)
+    int32_t* jb;                            // judge index -> 1 once it is in the second judge list
+    int32_t* j2list;                        // second judge list: points whose HADRON or hint-QUARK run failed
 };
 
-// stages of the per-point pipeline
-enum Stage {
-    STAGE_NONE = 0,
-    STAGE_SURR = 1,
-    STAGE_PRIMARY = 2,
-    STAGE_MULTI = 3,
-    STAGE_MR3 = 4,
-    STAGE_JUDGE = 5,    // MultiSeed candidates 0, 1 (HADRON, hint-QUARK) of a judged point
-    STAGE_JUDGE2 = 6    // candidates 2..5, where candidate 0 or 1 did not converge
+// plans of the per-point pipeline
+enum Plan {
+    PLAN_DEFAULT = 0,   // primary attempt (DefaultSeed / HADRON / QUARK / EXPLICIT), MultiSeed on failure
+    PLAN_SURR = 1,      // learned seed, primary on failure or inside the mask, MultiSeed, [Omega judge]
+    PLAN_MULTI = 2,     // the six MultiSeed candidates of every point
+    PLAN_MR3 = 3,       // learned, HADRON and QUARK seeds of every point
+    PLAN_CHAINS = 4     // chain policies (one chain per group, sequential fallbacks)
+};
+
+// item kinds of the stages
+enum ItemKind {
+    K_PRIMARY = 1,      // the point's primary attempt
+    K_SURR = 2,         // learned seed (then the primary attempt in the same group)
+    K_MULTI = 3,        // MultiSeed candidate k of a listed point
+    K_JUDGE = 4,        // Omega judge: MultiSeed candidate k of a judged point (0, 1; then 2..5)
+    K_MPOL = 5,         // MultiSeed candidate k of every point (PLAN_MULTI)
+    K_MR3 = 6           // k = 0 learned, 1 HADRON, 2 QUARK (PLAN_MR3)
 };
 // judge records per judged point: the six MultiSeed candidates
 constexpr int JUDGE_SEEDS = 6;
+
+// queue words
+enum QueueWord {
+    Q_HEAD = 0,         // next item of the running stage
+    Q_MCOUNT = 4,       // MultiSeed list length
+    Q_JCOUNT = 5,       // judge list length
+    Q_J2COUNT = 6       // second judge list length
+};
 
 constexpr int kMaxBndRows = 128;
 constexpr int kBndStride = 4 + 2 * kMaxBndRows;
 
 // ---------------------------------------------------------------------------
-// scratch layout (doubles, [group][field])
+// candidate records (doubles) and the group's state
 // ---------------------------------------------------------------------------
 enum RecField {
     RF_X = 0,          // 5
@@ -149,24 +180,34 @@ enum RecField {
     RF_XCONV = 26,
     REC_SIZE = 28
 };
-enum ScratchBlock {
+// the group's record area in HBM (offsets of its records)
+enum RecBlock {
     SB_PRIM = 0,                 // primary (Newton) candidate of the attempt
     SB_TRC = SB_PRIM + REC_SIZE, // trust-region candidate at the accepted x
     SB_SINGLE = SB_TRC + REC_SIZE,
     SB_BEST = SB_SINGLE + REC_SIZE,
-    SB_TR = SB_BEST + REC_SIZE,  // r[5] Jsym[15] d[5] p[5] delta
+    REC_AREA = SB_BEST + REC_SIZE
+};
+// the group's hot state in LDS
+enum StateField {
+    SB_SUMS = 0,                 // reduced node sums of the last pass: u, s, d (13 each)
+    SB_TR = SB_SUMS + 39,        // r[5] Jsym[15] d[5] p[5] delta
     SB_X0 = SB_TR + 31,          // seed of the attempt
     SB_PREV = SB_X0 + 5,         // chain: previous solution
-    SB_SUMS = SB_PREV + 5,       // reduced node sums of the last pass: u, s, d (13 each)
-    SCRATCH_FIELDS = SB_SUMS + 39
+    SB_XOLD = SB_PREV + 5,       // x of the previous Newton / trust-region step
+    SB_ST = SB_XOLD + 5,         // the group's solver state (struct St, 24 doubles)
+    // an odd number of doubles per group: the groups of a wavefront hit
+    // distinct LDS banks when they access the same field
+    STATE_STRIDE = SB_ST + 24 + ((SB_ST + 24) % 2 == 0 ? 1 : 0)
 };
 enum TrField { TR_R = 0, TR_J = 5, TR_D = 20, TR_P = 25, TR_DELTA = 30 };
 
-// a group's scratch record: SCRATCH_FIELDS contiguous doubles, so every field
-// is one base address plus an immediate offset
+// a group's storage: hot state in LDS (at), candidate records in HBM (rec)
 struct Scr {
-    double* p;
-    __device__ __forceinline__ double& at(int f) const { return p[f]; }
+    double* l;
+    double* g;
+    __device__ __forceinline__ double& at(int f) const { return l[f]; }
+    __device__ __forceinline__ double& rec(int f) const { return g[f]; }
 };
 
 // packed index of the symmetric 5x5 Jacobian (upper triangle, row-major)
@@ -302,11 +343,10 @@ __device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, 
     flav_finish(acc, invT);
 }
 
-// Node pass at state x: u, s, and d unless M_d == M_u (then d aliases u).
-// Not inlined: the node loop gets the whole register budget, and the caller's
-// live state is saved once per pass instead of competing with the loop.
+// Node pass at state x: u, s, and d unless M_d == M_u (then d aliases u);
+// the reduced sums go to the group's LDS state.
 template <int LPP>
-__device__ __noinline__ void node_pass(const DevParams& P, const double* lds, int pn, int tn, const Scr& s,
+__device__ __forceinline__ void node_pass(const DevParams& P, const double* lds, int pn, int tn, const Scr& s,
                                           const double* x, double T, double mu, double xi, bool sym, int gl)
 {
     double M[3];
@@ -327,7 +367,7 @@ __device__ __forceinline__ int slot_of(int i, bool sym) { return i == 0 ? 0 : (i
 // ---------------------------------------------------------------------------
 // Omega, F and the packed symmetric Jacobian from the sums in scratch
 // ---------------------------------------------------------------------------
-__device__ void assemble_core(const DevParams& P, const Scr& s, const double* x, double T, bool sym, double& omega,
+__device__ __forceinline__ void assemble_core(const DevParams& P, const Scr& s, const double* x, double T, bool sym, double& omega,
                               double F[5], double Jp[15])
 {
     const double G = P.G, K = P.K;
@@ -411,7 +451,7 @@ __device__ void assemble_core(const DevParams& P, const Scr& s, const double* x,
 // Candidate record at x: thermodynamics (calculate_thermo + densities, in
 // closed form from the sums) and the physicality gate
 // (_default_is_physical_solution + _all_finite_thermo, ImplicitSolver.jl:50-64).
-__device__ void put_rec(const DevParams& P, const Scr& s, int blk, const double* x, double mu, double T, bool sym,
+__device__ __noinline__ void put_rec(const DevParams& P, const Scr& s, int blk, const double* x, double mu, double T, bool sym,
                         double phi_tol, double omega, double res, int it, bool fconv, bool xconv)
 {
     double M[3];
@@ -436,9 +476,9 @@ __device__ void put_rec(const DevParams& P, const Scr& s, int blk, const double*
         rsum += 6.0 * (sfp - sfm);
         Sab += s.at(b + SF_Sfab);
         Ssum += s.at(b + SF_S);
-        s.at(blk + RF_NQ + i) = twoNc * sfp;
-        s.at(blk + RF_NQB + i) = twoNc * sfm;
-        s.at(blk + RF_M + i) = M[i];
+        s.rec(blk + RF_NQ + i) = twoNc * sfp;
+        s.rec(blk + RF_NQB + i) = twoNc * sfm;
+        s.rec(blk + RF_M + i) = M[i];
     }
     double pressure = -omega;
     double rho_norm = rsum / (3.0 * P.rho0);
@@ -450,43 +490,43 @@ __device__ void put_rec(const DevParams& P, const Scr& s, int blk, const double*
     for (int i = 0; i < 3; i++) ph = ph && isfinite(M[i]) && (M[i] > 0.0);
     ph = ph && isfinite(omega) && isfinite(pressure) && isfinite(rho_norm) && isfinite(entropy) && isfinite(energy);
 #pragma unroll
-    for (int i = 0; i < 5; i++) s.at(blk + RF_X + i) = x[i];
-    s.at(blk + RF_OMEGA) = omega;
-    s.at(blk + RF_P) = pressure;
-    s.at(blk + RF_RHO) = rho_norm;
-    s.at(blk + RF_S) = entropy;
-    s.at(blk + RF_E) = energy;
-    s.at(blk + RF_RES) = res;
-    s.at(blk + RF_IT) = (double)it;
-    s.at(blk + RF_FCONV) = fconv ? 1.0 : 0.0;
-    s.at(blk + RF_XCONV) = xconv ? 1.0 : 0.0;
-    s.at(blk + RF_PHYS) = ph ? 1.0 : 0.0;
-    s.at(blk + RF_EXC) = 0.0;
-    s.at(blk + RF_FLAGS) = 0.0;
+    for (int i = 0; i < 5; i++) s.rec(blk + RF_X + i) = x[i];
+    s.rec(blk + RF_OMEGA) = omega;
+    s.rec(blk + RF_P) = pressure;
+    s.rec(blk + RF_RHO) = rho_norm;
+    s.rec(blk + RF_S) = entropy;
+    s.rec(blk + RF_E) = energy;
+    s.rec(blk + RF_RES) = res;
+    s.rec(blk + RF_IT) = (double)it;
+    s.rec(blk + RF_FCONV) = fconv ? 1.0 : 0.0;
+    s.rec(blk + RF_XCONV) = xconv ? 1.0 : 0.0;
+    s.rec(blk + RF_PHYS) = ph ? 1.0 : 0.0;
+    s.rec(blk + RF_EXC) = 0.0;
+    s.rec(blk + RF_FLAGS) = 0.0;
 }
 
-__device__ void put_exc_rec(const Scr& s, int blk, const double* x0)
+__device__ __noinline__ void put_exc_rec(const Scr& s, int blk, const double* x0)
 {
 #pragma unroll
-    for (int i = 0; i < 5; i++) s.at(blk + RF_X + i) = x0[i];
-    for (int f = RF_OMEGA; f <= RF_RES; f++) s.at(blk + f) = qnan();
-    s.at(blk + RF_IT) = 0.0;
-    s.at(blk + RF_FCONV) = 0.0;
-    s.at(blk + RF_XCONV) = 0.0;
-    s.at(blk + RF_PHYS) = 0.0;
-    s.at(blk + RF_EXC) = 1.0;
-    s.at(blk + RF_FLAGS) = (double)PNJL_PT_EXCEPTION;
+    for (int i = 0; i < 5; i++) s.rec(blk + RF_X + i) = x0[i];
+    for (int f = RF_OMEGA; f <= RF_RES; f++) s.rec(blk + f) = qnan();
+    s.rec(blk + RF_IT) = 0.0;
+    s.rec(blk + RF_FCONV) = 0.0;
+    s.rec(blk + RF_XCONV) = 0.0;
+    s.rec(blk + RF_PHYS) = 0.0;
+    s.rec(blk + RF_EXC) = 1.0;
+    s.rec(blk + RF_FLAGS) = (double)PNJL_PT_EXCEPTION;
 }
 
-__device__ void copy_rec(const Scr& s, int dst, int src)
+__device__ __forceinline__ void copy_rec(const Scr& s, int dst, int src)
 {
-    for (int f = 0; f < REC_SIZE; f++) s.at(dst + f) = s.at(src + f);
+    for (int f = 0; f < REC_SIZE; f++) s.rec(dst + f) = s.rec(src + f);
 }
 
 __device__ __forceinline__ bool rec_good(const Scr& s, int blk, double resmax)
 {
-    double r = s.at(blk + RF_RES);
-    return s.at(blk + RF_FCONV) != 0.0 && isfinite(r) && r <= resmax && s.at(blk + RF_PHYS) != 0.0;
+    double r = s.rec(blk + RF_RES);
+    return s.rec(blk + RF_FCONV) != 0.0 && isfinite(r) && r <= resmax && s.rec(blk + RF_PHYS) != 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -598,7 +638,7 @@ __device__ __forceinline__ double sumsq5(const double* v)
 // Newton direction p = -J^{-1} F; on a singular J the Levenberg-modified
 // direction p = -(J'J + lambda I)^{-1} J'F, lambda = 1e6 sqrt(n eps) |J'J|_1
 // (NLsolve newton_; the norm is entrywise).
-__device__ __noinline__ void newton_dir(const double Jp[15], const double F[5], double p[5])
+__device__ __forceinline__ void newton_dir(const double Jp[15], const double F[5], double p[5])
 {
     double A[5][5];
     unpack5(Jp, A);
@@ -640,7 +680,7 @@ __device__ __noinline__ void newton_dir(const double Jp[15], const double F[5], 
 // NLsolve dogleg! with J (packed, symmetric) read from the TR block.  (An
 // exactly singular J, where the reference takes a pseudo-inverse step, uses
 // the scaled steepest-descent branch.)
-__device__ __noinline__ void dogleg5(const Scr& s, double p[5], const double r[5], const double d[5], double delta)
+__device__ __forceinline__ void dogleg5(const Scr& s, double p[5], const double r[5], const double d[5], double delta)
 {
     double pi_[5];
     {
@@ -743,7 +783,7 @@ __device__ __forceinline__ void multi_seed(int k, double T, double mu, double x0
 }
 
 // phase of (T_MeV, mu_MeV) for the chain's xi: 0 unknown, 1 hadron, 2 quark, 3 crossover
-__device__ int phase_of(const ScanArgs& A, double xi, double T, double mu)
+__device__ __forceinline__ int phase_of(const ScanArgs& A, double xi, double T, double mu)
 {
     const double* B = nullptr;
     for (int k = 0; k < A.nb; k++) {
@@ -783,23 +823,28 @@ enum Phase { PH_IDLE = 0, PH_NEWTON = 1, PH_TR = 2 };
 constexpr int ATT_SURR = -2;
 constexpr int ATT_PRIMARY = -1;  // 0..5: MultiSeed candidates
 
+// The group's solver state.  It lives in the group's LDS block (every lane
+// of the group reads and writes the same values), which keeps it out of the
+// registers the node loop needs.
 struct St {
-    int64_t item, item_end;   // current chunk of items
+    int64_t item, item_end;   // current chunk of items (chains: of chains)
     int64_t pt, pt_end;       // current point, end of its chain (exclusive)
     int64_t trips;
+    int64_t idx;              // pipeline: list index of the item's point
     double T, mu, xi;
-    double x[5], xold[5];
+    double x[5];
+    double prev_mass;         // branch chain: M_u of the previous point
     int phase, it, attempt, flags, steps, passes;
     int prev_phase;
-    int k;                    // staged pipeline: attempt index within the listed point
-    int64_t item_ix;          // staged pipeline: record index of the attempt
+    int kind, k;              // pipeline: item kind and seed index
     bool fresh;               // next pass is the first evaluation of the phase
     bool sym;                 // seed has phi_u == phi_d: keep them equal
     bool have_best;
     bool have_prev;
     bool plan_multi;          // this point runs MultiSeed as its plan
-    double prev_mass;         // branch chain: M_u of the previous point
 };
+static_assert(sizeof(St) <= 24 * sizeof(double), "St must fit its LDS slot");
+static_assert(SB_ST + 24 <= STATE_STRIDE && STATE_STRIDE % 2 == 1, "state layout");
 
 __host__ __device__ inline bool branch_policy(int p)
 {
@@ -811,7 +856,7 @@ __host__ __device__ inline bool chain_policy(int p)
     return p == PNJL_SEED_CONTINUITY || p == PNJL_SEED_PHASE_AWARE || branch_policy(p);
 }
 
-__device__ void start_attempt(St& st, const Scr& s, int attempt, const double x0[5])
+__device__ __forceinline__ void start_attempt(St& st, const Scr& s, int attempt, const double x0[5])
 {
     st.attempt = attempt;
     st.phase = PH_NEWTON;
@@ -820,33 +865,36 @@ __device__ void start_attempt(St& st, const Scr& s, int attempt, const double x0
 #pragma unroll
     for (int i = 0; i < 5; i++) {
         st.x[i] = x0[i];
-        st.xold[i] = qnan();
+        s.at(SB_XOLD + i) = qnan();
         s.at(SB_X0 + i) = x0[i];
     }
     st.sym = (x0[0] == x0[1]);
 }
 
-__device__ void write_point(const ScanArgs& A, const Scr& s, const St& st, int blk, int gl)
+// ---------------------------------------------------------------------------
+// chains: one chain per group, solve()'s sequential fallbacks
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void write_point(const ScanArgs& A, const Scr& s, const St& st, int blk, int gl)
 {
     if (gl != 0) return;
     int64_t i = st.pt;
 #pragma unroll
-    for (int k = 0; k < 5; k++) A.x_out[i * 5 + k] = s.at(blk + RF_X + k);
-    A.omega[i] = s.at(blk + RF_OMEGA);
-    A.pressure[i] = s.at(blk + RF_P);
-    A.rho_norm[i] = s.at(blk + RF_RHO);
-    A.entropy[i] = s.at(blk + RF_S);
-    A.energy[i] = s.at(blk + RF_E);
+    for (int k = 0; k < 5; k++) A.x_out[i * 5 + k] = s.rec(blk + RF_X + k);
+    A.omega[i] = s.rec(blk + RF_OMEGA);
+    A.pressure[i] = s.rec(blk + RF_P);
+    A.rho_norm[i] = s.rec(blk + RF_RHO);
+    A.entropy[i] = s.rec(blk + RF_S);
+    A.energy[i] = s.rec(blk + RF_E);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        A.masses[i * 3 + k] = s.at(blk + RF_M + k);
-        if (A.n_q) A.n_q[i * 3 + k] = s.at(blk + RF_NQ + k);
-        if (A.n_qbar) A.n_qbar[i * 3 + k] = s.at(blk + RF_NQB + k);
+        A.masses[i * 3 + k] = s.rec(blk + RF_M + k);
+        if (A.n_q) A.n_q[i * 3 + k] = s.rec(blk + RF_NQ + k);
+        if (A.n_qbar) A.n_qbar[i * 3 + k] = s.rec(blk + RF_NQB + k);
     }
-    A.resnorm[i] = s.at(blk + RF_RES);
-    A.iterations[i] = (int32_t)s.at(blk + RF_IT);
-    A.converged[i] = (int32_t)s.at(blk + RF_CONV);
-    if (A.flags) A.flags[i] = (int32_t)s.at(blk + RF_FLAGS) | st.flags;
+    A.resnorm[i] = s.rec(blk + RF_RES);
+    A.iterations[i] = (int32_t)s.rec(blk + RF_IT);
+    A.converged[i] = (int32_t)s.rec(blk + RF_CONV);
+    if (A.flags) A.flags[i] = (int32_t)s.rec(blk + RF_FLAGS) | st.flags;
     if (A.steps) A.steps[i] = st.steps;
     if (A.passes) A.passes[i] = st.passes;
 }
@@ -855,53 +903,53 @@ __device__ void write_point(const ScanArgs& A, const Scr& s, const St& st, int b
 // (update! only after a converged solve, run_gap_transport_scan.jl:439-442,
 // tests/perf/pnjl/scan_perf.jl:69-72).  A branch chain ends at its first
 // point that fails or jumps (_is_solution_jump, DualBranchScan.jl:410-415);
-// the rest of the chain is written as skipped and not solved.
-__device__ void finish_point(const ScanArgs& A, const Scr& s, St& st, int blk, int gl)
+// the rest of the chain is written as skipped (by all lanes of the group)
+// and not solved.
+template <int LPP>
+__device__ __forceinline__ void finish_point(const ScanArgs& A, const Scr& s, St& st, int blk, int gl)
 {
-    bool ok = s.at(blk + RF_CONV) != 0.0 && !((int)(s.at(blk + RF_FLAGS)) & PNJL_PT_EXCEPTION) &&
+    bool ok = s.rec(blk + RF_CONV) != 0.0 && !((int)(s.rec(blk + RF_FLAGS)) & PNJL_PT_EXCEPTION) &&
               !(st.flags & PNJL_PT_NO_ROOT);
     const bool branch = branch_policy(A.policy);
     if (branch && ok && st.have_prev) {
-        const double dphi = fabs(s.at(SB_PREV) - s.at(blk + RF_X));
-        const double dm = fabs(st.prev_mass - s.at(blk + RF_M)) * 197.327;
+        const double dphi = fabs(s.at(SB_PREV) - s.rec(blk + RF_X));
+        const double dm = fabs(st.prev_mass - s.rec(blk + RF_M)) * 197.327;
         if (dphi > 0.5 || dm > 50.0) ok = false;
     }
     if (branch && !ok) st.flags |= PNJL_PT_BRANCH_END;
     write_point(A, s, st, blk, gl);
     if (chain_policy(A.policy) && ok) {
 #pragma unroll
-        for (int i = 0; i < 5; i++) s.at(SB_PREV + i) = s.at(blk + RF_X + i);
+        for (int i = 0; i < 5; i++) s.at(SB_PREV + i) = s.rec(blk + RF_X + i);
         st.have_prev = true;
-        st.prev_mass = s.at(blk + RF_M);
+        st.prev_mass = s.rec(blk + RF_M);
         if (A.policy == PNJL_SEED_PHASE_AWARE) st.prev_phase = phase_of(A, st.xi, st.T * 197.327, st.mu * 197.327);
     }
     if (branch && !ok) {
-        if (gl == 0) {
-            for (int64_t i = st.pt + 1; i < st.pt_end; i++) {
+        for (int64_t i = st.pt + 1 + gl; i < st.pt_end; i += LPP) {
 #pragma unroll
-                for (int k = 0; k < 5; k++) A.x_out[i * 5 + k] = qnan();
-                A.omega[i] = A.pressure[i] = A.rho_norm[i] = A.entropy[i] = A.energy[i] = qnan();
+            for (int k = 0; k < 5; k++) A.x_out[i * 5 + k] = qnan();
+            A.omega[i] = A.pressure[i] = A.rho_norm[i] = A.entropy[i] = A.energy[i] = qnan();
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    A.masses[i * 3 + k] = qnan();
-                    if (A.n_q) A.n_q[i * 3 + k] = qnan();
-                    if (A.n_qbar) A.n_qbar[i * 3 + k] = qnan();
-                }
-                A.resnorm[i] = qnan();
-                A.iterations[i] = 0;
-                A.converged[i] = 0;
-                if (A.flags) A.flags[i] = PNJL_PT_SKIPPED;
-                if (A.steps) A.steps[i] = 0;
-                if (A.passes) A.passes[i] = 0;
+            for (int k = 0; k < 3; k++) {
+                A.masses[i * 3 + k] = qnan();
+                if (A.n_q) A.n_q[i * 3 + k] = qnan();
+                if (A.n_qbar) A.n_qbar[i * 3 + k] = qnan();
             }
+            A.resnorm[i] = qnan();
+            A.iterations[i] = 0;
+            A.converged[i] = 0;
+            if (A.flags) A.flags[i] = PNJL_PT_SKIPPED;
+            if (A.steps) A.steps[i] = 0;
+            if (A.passes) A.passes[i] = 0;
         }
-        st.pt = st.pt_end - 1;      // next_point moves on to the next chain
+        st.pt = st.pt_end - 1;      // next_chain_point moves on to the next chain
     }
     st.phase = PH_IDLE;
 }
 
-// Sets up point st.pt according to the policy (first attempt).
-__device__ __noinline__ void begin_point(const ScanArgs& A, const Scr& s, St& st)
+// Sets up point st.pt of a chain according to the policy (first attempt).
+__device__ __noinline__ void begin_chain_point(const ScanArgs& A, const Scr& s, St& st)
 {
     int64_t i = st.pt;
     st.T = A.T[i];
@@ -915,29 +963,6 @@ __device__ __noinline__ void begin_point(const ScanArgs& A, const Scr& s, St& st
     st.plan_multi = false;
     double x0[5];
     switch (A.policy) {
-    case PNJL_SEED_HADRON:
-        default_seed(1, st.T, st.mu, x0);
-        start_attempt(st, s, ATT_PRIMARY, x0);
-        break;
-    case PNJL_SEED_QUARK:
-        default_seed(2, st.T, st.mu, x0);
-        start_attempt(st, s, ATT_PRIMARY, x0);
-        break;
-    case PNJL_SEED_EXPLICIT:
-#pragma unroll
-        for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
-        start_attempt(st, s, ATT_PRIMARY, x0);
-        break;
-    case PNJL_SEED_MULTI:
-        st.plan_multi = true;
-        multi_seed(0, st.T, st.mu, x0);
-        start_attempt(st, s, 0, x0);
-        break;
-    case PNJL_SEED_SURROGATE:
-#pragma unroll
-        for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
-        start_attempt(st, s, ATT_SURR, x0);
-        break;
     case PNJL_SEED_CONTINUITY:
         if (st.have_prev) {
 #pragma unroll
@@ -957,7 +982,7 @@ __device__ __noinline__ void begin_point(const ScanArgs& A, const Scr& s, St& st
         }
         start_attempt(st, s, ATT_PRIMARY, x0);
         break;
-    case PNJL_SEED_PHASE_AWARE:
+    default:  // PNJL_SEED_PHASE_AWARE
         if (!st.have_prev) {
             st.plan_multi = true;
             multi_seed(0, st.T, st.mu, x0);
@@ -975,13 +1000,81 @@ __device__ __noinline__ void begin_point(const ScanArgs& A, const Scr& s, St& st
             start_attempt(st, s, ATT_PRIMARY, x0);
         }
         break;
-    default:  // PNJL_SEED_DEFAULT
-        default_seed(0, st.T, st.mu, x0);
-        start_attempt(st, s, ATT_PRIMARY, x0);
-        break;
     }
 }
 
+// A chain point's attempt ended with its result in record `blk`: the
+// automatic MultiSeed fallback runs its six candidates one after another.
+template <int LPP>
+__device__ __noinline__ void chain_attempt_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl,
+                                                   bool conv, bool exc)
+{
+    double x0[5];
+    if (st.attempt == ATT_PRIMARY) {
+        copy_rec(s, SB_SINGLE, blk);
+        if (conv || !A.auto_ms || exc) {
+            finish_point<LPP>(A, s, st, SB_SINGLE, gl);
+            return;
+        }
+        st.have_best = false;
+        multi_seed(0, st.T, st.mu, x0);
+        start_attempt(st, s, 0, x0);
+        return;
+    }
+    // MultiSeed candidate k = st.attempt
+    if (st.attempt == 0 && st.plan_multi) copy_rec(s, SB_SINGLE, blk);
+    if (conv && (!st.have_best || s.rec(blk + RF_OMEGA) < s.rec(SB_BEST + RF_OMEGA))) {
+        copy_rec(s, SB_BEST, blk);
+        st.have_best = true;
+    }
+    if (st.attempt < 5) {
+        int k = st.attempt + 1;
+        multi_seed(k, st.T, st.mu, x0);
+        start_attempt(st, s, k, x0);
+        return;
+    }
+    if (st.have_best) {
+        s.rec(SB_BEST + RF_FLAGS) = (double)((int)s.rec(SB_BEST + RF_FLAGS) | PNJL_PT_MULTISEED);
+        finish_point<LPP>(A, s, st, SB_BEST, gl);
+    } else {
+        if (st.plan_multi) {
+            st.flags |= PNJL_PT_NO_ROOT;
+            s.rec(SB_SINGLE + RF_CONV) = 0.0;
+        }
+        finish_point<LPP>(A, s, st, SB_SINGLE, gl);
+    }
+}
+
+// next chain point for the group; false when the queue is drained
+template <int LPP>
+__device__ __forceinline__ bool next_chain_point(const ScanArgs& A, St& st, int gl)
+{
+    if (st.pt + 1 < st.pt_end) {
+        st.pt++;
+        return true;
+    }
+    if (st.item >= st.item_end) {
+        unsigned long long base = 0;
+        if (gl == 0) base = atomicAdd(A.queue + Q_HEAD, (unsigned long long)A.chunk);
+        base = __shfl(base, 0, LPP);
+        st.item = (int64_t)base;
+        st.item_end = st.item + A.chunk;
+        if (st.item_end > A.n_items) st.item_end = A.n_items;
+    }
+    if (st.item >= st.item_end) return false;
+    int64_t it = st.item++;
+    st.pt = it * A.chain_len;
+    st.pt_end = st.pt + A.chain_len;
+    if (st.pt_end > A.n_points) st.pt_end = A.n_points;
+    st.have_prev = false;
+    st.prev_phase = 0;
+    st.prev_mass = 0.0;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// per-point pipeline: the two-level work queue
+// ---------------------------------------------------------------------------
 // ambiguity mask lookup (bins over the training box; outside the box: judged)
 __device__ __forceinline__ bool in_judge_region(const ScanArgs& A, double T, double mu, double xi)
 {
@@ -998,128 +1091,212 @@ __device__ __forceinline__ bool in_judge_region(const ScanArgs& A, double T, dou
     return A.judge_mask[idx] != 0;
 }
 
-// Staged pipeline: an attempt ended -> its record goes to rec_out; a failed
-// learned seed or a failed primary attempt appends the point to the next stage.
-__device__ void staged_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl, bool conv, bool exc)
+// Next item of the stage for the group (st.kind, st.k, st.idx, st.pt set);
+// false when the stage's queue is drained.
+template <int LPP>
+__device__ __forceinline__ bool pipeline_next(const ScanArgs& A, St& st, int gl, int64_t n_items)
 {
-    if (gl == 0) {
-        double* r = A.rec_out + st.item_ix * REC_SIZE;
-        for (int f = 0; f < REC_SIZE; f++) r[f] = s.at(blk + f);
-        if (A.passes) atomicAdd(&A.passes[st.pt], st.passes);
-        if (A.steps) atomicAdd(&A.steps[st.pt], st.steps);
-        bool append = false;
-        if (A.stage == STAGE_SURR) append = !conv;
-        else if (A.stage == STAGE_PRIMARY) append = !conv && A.auto_ms && !exc;
-        if (append) {
-            unsigned long long j = atomicAdd(A.next_count, 1ULL);
-            A.next_list[j] = (int32_t)st.pt;
-            if (A.stage == STAGE_PRIMARY) A.mslot[st.pt] = (int32_t)j;
+    if (st.item >= st.item_end) {
+        unsigned long long base = 0;
+        if (gl == 0) base = atomicAdd(A.queue + Q_HEAD, (unsigned long long)A.chunk);
+        base = __shfl(base, 0, LPP);
+        st.item = (int64_t)base;
+        st.item_end = st.item + A.chunk;
+        if (st.item_end > n_items) st.item_end = n_items;
+    }
+    if (st.item >= st.item_end) return false;
+    const int64_t i = st.item++;
+    const int64_t j = i / A.mult;
+    st.k = A.k_base + (int)(i - j * A.mult);
+    st.idx = j;
+    st.kind = A.kind;
+    st.pt = A.item_list ? (int64_t)A.item_list[j] : (A.order ? (int64_t)A.order[j] : j);
+    return true;
+}
+
+__device__ __forceinline__ void reset_point(const ScanArgs& A, St& st)
+{
+    const int64_t i = st.pt;
+    st.T = A.T[i];
+    st.mu = A.mu[i];
+    st.xi = A.xi[i];
+    st.flags = 0;
+    st.steps = 0;
+    st.passes = 0;
+    st.trips = 0;
+}
+
+// the primary attempt of point st.pt (DefaultSeed with the policy's hint, or the given seed)
+__device__ __forceinline__ void begin_primary(const ScanArgs& A, const Scr& s, St& st)
+{
+    double x0[5];
+    st.kind = K_PRIMARY;
+    switch (A.policy) {
+    case PNJL_SEED_HADRON: default_seed(1, st.T, st.mu, x0); break;
+    case PNJL_SEED_QUARK: default_seed(2, st.T, st.mu, x0); break;
+    case PNJL_SEED_EXPLICIT:
+#pragma unroll
+        for (int k = 0; k < 5; k++) x0[k] = A.seed[st.pt * 5 + k];
+        break;
+    default: default_seed(0, st.T, st.mu, x0); break;
+    }
+    start_attempt(st, s, ATT_PRIMARY, x0);
+}
+
+// Sets up the attempt of the item the group just took.
+__device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st, int gl)
+{
+    reset_point(A, st);
+    const int64_t p = st.pt;
+    double x0[5];
+    switch (st.kind) {
+    case K_SURR:
+        if (A.judge_mode == PNJL_JUDGE_REFERENCE && A.judge_mask && in_judge_region(A, st.T, st.mu, st.xi)) {
+            // ambiguity region: the reference policy takes the point unseen
+            if (gl == 0) {
+                double* r = A.recS + p * REC_SIZE;
+                r[RF_CONV] = 0.0;
+                r[RF_FLAGS] = (double)PNJL_PT_SURR_MASKED;
+            }
+            begin_primary(A, s, st);
+            return;
         }
-        if ((A.stage == STAGE_SURR || A.stage == STAGE_PRIMARY) && A.jslot) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) x0[k] = A.seed[p * 5 + k];
+        start_attempt(st, s, ATT_SURR, x0);
+        return;
+    case K_MULTI:
+    case K_JUDGE:
+    case K_MPOL:
+        multi_seed(st.k, st.T, st.mu, x0);
+        start_attempt(st, s, st.k, x0);
+        return;
+    case K_MR3:
+        // learned seed, then the chirally broken and restored reference seeds
+        if (st.k == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) x0[k] = A.seed[p * 5 + k];
+        } else {
+            seed_const(st.k == 1 ? 0 : 1, x0);
+        }
+        start_attempt(st, s, st.k, x0);
+        return;
+    default:  // K_PRIMARY
+        begin_primary(A, s, st);
+        return;
+    }
+}
+
+// An attempt of a pipeline item ended with its record in `blk`: the record
+// goes to the item's slot; a failed primary attempt appends its point to
+// the MultiSeed list, a judged root to the Omega judge's list, a failed
+// judge candidate (HADRON, hint-QUARK) to the second judge list; a failed
+// learned-seed attempt continues with the primary attempt in the same group.
+__device__ __noinline__ void item_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl, bool conv,
+                                       bool exc)
+{
+    const int64_t p = st.pt;
+    const bool cont = (st.kind == K_SURR) && !conv;
+    if (gl == 0) {
+        double* dst;
+        switch (st.kind) {
+        case K_SURR: dst = A.recS + p * REC_SIZE; break;
+        case K_PRIMARY: dst = A.recP + p * REC_SIZE; break;
+        case K_MULTI: dst = A.recM + (st.idx * 6 + st.k) * REC_SIZE; break;
+        case K_JUDGE: dst = A.recJ + ((int64_t)A.jslot[p] * JUDGE_SEEDS + st.k) * REC_SIZE; break;
+        case K_MPOL: dst = A.recM + (p * 6 + st.k) * REC_SIZE; break;
+        default: dst = A.recM + (p * 3 + st.k) * REC_SIZE; break;   // K_MR3
+        }
+        for (int f = 0; f < REC_SIZE; f++) dst[f] = s.rec(blk + f);
+        if (A.passes) atomicAdd(&A.passes[p], st.passes);
+        if (A.steps) atomicAdd(&A.steps[p], st.steps);
+        if (st.kind == K_PRIMARY && !conv && A.auto_ms && !exc) {
+            const unsigned long long j = atomicAdd(A.queue + Q_MCOUNT, 1ULL);
+            A.mlist[j] = (int32_t)p;
+            A.mslot[p] = (int32_t)j;
+        }
+        if ((st.kind == K_SURR || st.kind == K_PRIMARY) && A.jslot && !cont) {
             // learned-seed plan: a converged root inside the ambiguity mask (the
             // learned root, or the plain root of a point whose learned seed
             // failed) is checked against the MultiSeed candidates' roots
             int32_t js = -1;
             if (conv && in_judge_region(A, st.T, st.mu, st.xi)) {
-                unsigned long long j = atomicAdd(A.judge_count, 1ULL);
-                A.judge_list[j] = (int32_t)st.pt;
+                const unsigned long long j = atomicAdd(A.queue + Q_JCOUNT, 1ULL);
+                A.jlist[j] = (int32_t)p;
                 A.jb[j] = 0;
                 js = (int32_t)j;
             }
-            if (A.jslot) A.jslot[st.pt] = js;
+            A.jslot[p] = js;
         }
-        if (A.stage == STAGE_JUDGE && !conv) {
+        if (st.kind == K_JUDGE && st.k < 2 && !conv) {
             // HADRON or hint-QUARK failed: the other MultiSeed candidates run
-            const int64_t js = st.item_ix / JUDGE_SEEDS;
+            const int32_t js = A.jslot[p];
             if (atomicCAS(&A.jb[js], 0, 1) == 0) {
-                unsigned long long j = atomicAdd(A.judge2_count, 1ULL);
-                A.judge2_list[j] = (int32_t)st.pt;
+                const unsigned long long j = atomicAdd(A.queue + Q_J2COUNT, 1ULL);
+                A.j2list[j] = (int32_t)p;
             }
         }
+    }
+    if (cont) {
+        st.flags = 0;
+        st.steps = 0;
+        st.passes = 0;
+        st.trips = 0;
+        begin_primary(A, s, st);
+        return;
     }
     st.phase = PH_IDLE;
 }
 
-// An attempt (one single-seed solve) ended with its result in record `blk`.
-__device__ __noinline__ void attempt_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl)
+// A learned-seed run that lands farther from its seed (in the group's X0)
+// than DualBranchScan's jump test allows a continuation seed (|d phi_u| >
+// 0.5 or |d M_u| > 50 MeV, DualBranchScan.jl:410-415) has left the seed's
+// basin: its root is not trusted and the plain policy takes the point.
+__device__ __forceinline__ bool learned_jump(const ScanArgs& A, const Scr& s, int blk)
 {
-    bool exc = s.at(blk + RF_EXC) != 0.0;
+    double x0[5], M0[3];
+#pragma unroll
+    for (int i = 0; i < 5; i++) x0[i] = s.at(SB_X0 + i);
+    mass_vec(A.P, x0, M0);
+    return fabs(s.rec(blk + RF_X) - x0[0]) > 0.5 || fabs(s.rec(blk + RF_M) - M0[0]) * 197.327 > 50.0;
+}
+
+template <int LPP>
+__device__ __forceinline__ void attempt_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl)
+{
+    const bool exc = s.rec(blk + RF_EXC) != 0.0;
     bool conv = !exc && rec_good(s, blk, A.resmax);
-    s.at(blk + RF_CONV) = conv ? 1.0 : 0.0;
-    if (A.stage != STAGE_NONE) {
-        staged_done(A, s, st, blk, gl, conv, exc);
-        return;
-    }
-    double x0[5];
-    if (st.attempt == ATT_SURR) {
-        if (conv) {
-            finish_point(A, s, st, blk, gl);
-            return;
-        }
-        // learned seed abandoned: the plain reference policy takes over
-        st.flags |= PNJL_PT_SURR_FALLBACK;
-        default_seed(0, st.T, st.mu, x0);
-        start_attempt(st, s, ATT_PRIMARY, x0);
-        return;
-    }
-    if (st.attempt == ATT_PRIMARY) {
-        copy_rec(s, SB_SINGLE, blk);
-        if (conv || !A.auto_ms || exc) {
-            finish_point(A, s, st, SB_SINGLE, gl);
-            return;
-        }
-        st.have_best = false;
-        multi_seed(0, st.T, st.mu, x0);
-        start_attempt(st, s, 0, x0);
-        return;
-    }
-    // MultiSeed candidate k = st.attempt
-    if (st.attempt == 0 && st.plan_multi) copy_rec(s, SB_SINGLE, blk);
-    if (conv && (!st.have_best || s.at(blk + RF_OMEGA) < s.at(SB_BEST + RF_OMEGA))) {
-        copy_rec(s, SB_BEST, blk);
-        st.have_best = true;
-    }
-    if (st.attempt < 5) {
-        int k = st.attempt + 1;
-        multi_seed(k, st.T, st.mu, x0);
-        start_attempt(st, s, k, x0);
-        return;
-    }
-    if (st.have_best) {
-        s.at(SB_BEST + RF_FLAGS) = (double)((int)s.at(SB_BEST + RF_FLAGS) | PNJL_PT_MULTISEED);
-        finish_point(A, s, st, SB_BEST, gl);
-    } else {
-        if (st.plan_multi) {
-            st.flags |= PNJL_PT_NO_ROOT;
-            s.at(SB_SINGLE + RF_CONV) = 0.0;
-        }
-        finish_point(A, s, st, SB_SINGLE, gl);
-    }
+    if (conv && A.plan == PLAN_SURR && st.kind == K_SURR && learned_jump(A, s, blk)) conv = false;
+    s.rec(blk + RF_CONV) = conv ? 1.0 : 0.0;
+    if (A.plan == PLAN_CHAINS)
+        chain_attempt_done<LPP>(A, s, st, blk, gl, conv, exc);
+    else
+        item_done(A, s, st, blk, gl, conv, exc);
 }
 
 // _choose_candidate (ImplicitSolver.jl:72-101) between PRIM and TRC -> block
-__device__ int choose_candidate(const ScanArgs& A, const Scr& s)
+__device__ __noinline__ int choose_candidate(const ScanArgs& A, const Scr& s)
 {
     bool pg = rec_good(s, SB_PRIM, A.resmax);
     bool fg = rec_good(s, SB_TRC, A.resmax);
-    double pres = s.at(SB_PRIM + RF_RES), fres = s.at(SB_TRC + RF_RES);
+    double pres = s.rec(SB_PRIM + RF_RES), fres = s.rec(SB_TRC + RF_RES);
     bool pick_fb;
     if (fg && !pg) pick_fb = true;
     else if (pg && !fg) pick_fb = false;
     else if (fg && pg) {
-        double po = s.at(SB_PRIM + RF_OMEGA), fo = s.at(SB_TRC + RF_OMEGA);
+        double po = s.rec(SB_PRIM + RF_OMEGA), fo = s.rec(SB_TRC + RF_OMEGA);
         if (fo < po) pick_fb = true;
         else if (fo > po) pick_fb = false;
         else pick_fb = fres < pres;
     } else {
-        bool pf = s.at(SB_PRIM + RF_FCONV) != 0.0, ff = s.at(SB_TRC + RF_FCONV) != 0.0;
+        bool pf = s.rec(SB_PRIM + RF_FCONV) != 0.0, ff = s.rec(SB_TRC + RF_FCONV) != 0.0;
         if (ff && !pf) pick_fb = true;
         else if (pf && !ff) pick_fb = false;
         else if (isfinite(fres) && isfinite(pres)) pick_fb = fres < pres;
         else pick_fb = false;
     }
     if (pick_fb) {
-        s.at(SB_TRC + RF_FLAGS) = (double)PNJL_PT_TR_USED;
+        s.rec(SB_TRC + RF_FLAGS) = (double)PNJL_PT_TR_USED;
         return SB_TRC;
     }
     return SB_PRIM;
@@ -1129,7 +1306,7 @@ __device__ int choose_candidate(const ScanArgs& A, const Scr& s)
 // eta 1e-4), after an evaluation at st.x.  Returns true if another pass is
 // needed; otherwise the TR candidate is final in SB_TRC.
 __device__ __noinline__ bool tr_step(const ScanArgs& A, const Scr& s, St& st, double omega, const double F[5],
-                                     const double Jp[15])
+                                        const double Jp[15])
 {
     double r[5], d[5], p[5], delta;
     bool converged = false, stopped;
@@ -1193,8 +1370,9 @@ __device__ __noinline__ bool tr_step(const ScanArgs& A, const Scr& s, St& st, do
             bool xnan = false;
 #pragma unroll
             for (int i = 0; i < 5; i++) {
-                xnan = xnan || (st.xold[i] != st.xold[i]);
-                double dd = fabs(st.x[i] - st.xold[i]);
+                const double xo = s.at(SB_XOLD + i);
+                xnan = xnan || (xo != xo);
+                double dd = fabs(st.x[i] - xo);
                 cheb = (dd > cheb || dd != dd) ? dd : cheb;
             }
             bool xconv = !xnan && cheb < A.xtol;
@@ -1203,8 +1381,8 @@ __device__ __noinline__ bool tr_step(const ScanArgs& A, const Scr& s, St& st, do
             put_rec(A.P, s, SB_TRC, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(r), st.it, fconv, xconv);
         } else {
 #pragma unroll
-            for (int i = 0; i < 5; i++) st.x[i] = st.xold[i];
-            s.at(SB_TRC + RF_XCONV) = 0.0;
+            for (int i = 0; i < 5; i++) st.x[i] = s.at(SB_XOLD + i);
+            s.rec(SB_TRC + RF_XCONV) = 0.0;
             converged = false;
         }
         if (rho < 0.1) delta = delta / 2.0;
@@ -1224,19 +1402,20 @@ __device__ __noinline__ bool tr_step(const ScanArgs& A, const Scr& s, St& st, do
             s.at(SB_TR + TR_R + i) = r[i];
             s.at(SB_TR + TR_D + i) = d[i];
             s.at(SB_TR + TR_P + i) = p[i];
-            st.xold[i] = st.x[i];
-            st.x[i] = st.xold[i] + p[i];
+            s.at(SB_XOLD + i) = st.x[i];
+            st.x[i] = st.x[i] + p[i];
         }
         s.at(SB_TR + TR_DELTA) = delta;
         if (st.sym) st.x[1] = st.x[0];
         return true;
     }
-    s.at(SB_TRC + RF_IT) = (double)st.it;
+    s.rec(SB_TRC + RF_IT) = (double)st.it;
     return false;
 }
 
-// One control step after a pass at st.x (sums in scratch).
-__device__ __noinline__ void control_step(const ScanArgs& A, const Scr& s, St& st, int gl)
+// One control step after a pass at st.x (sums in the group's LDS state).
+template <int LPP>
+__device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St& st, int gl)
 {
     double omega, F[5], Jp[15];
     assemble_core(A.P, s, st.x, st.T, st.sym, omega, F, Jp);
@@ -1250,7 +1429,7 @@ __device__ __noinline__ void control_step(const ScanArgs& A, const Scr& s, St& s
 #pragma unroll
                 for (int i = 0; i < 5; i++) x0[i] = s.at(SB_X0 + i);
                 put_exc_rec(s, SB_PRIM, x0);
-                attempt_done(A, s, st, SB_PRIM, gl);
+                attempt_done<LPP>(A, s, st, SB_PRIM, gl);
                 return;
             }
             fconv = norm_inf5(F) < A.ftol;
@@ -1260,8 +1439,9 @@ __device__ __noinline__ void control_step(const ScanArgs& A, const Scr& s, St& s
             bool xnan = false;
 #pragma unroll
             for (int i = 0; i < 5; i++) {
-                xnan = xnan || (st.xold[i] != st.xold[i]);
-                double dd = fabs(st.x[i] - st.xold[i]);
+                const double xo = s.at(SB_XOLD + i);
+                xnan = xnan || (xo != xo);
+                double dd = fabs(st.x[i] - xo);
                 cheb = (dd > cheb || dd != dd) ? dd : cheb;
             }
             xconv = !xnan && cheb < A.xtol;
@@ -1276,8 +1456,8 @@ __device__ __noinline__ void control_step(const ScanArgs& A, const Scr& s, St& s
             newton_dir(Jp, F, p);
 #pragma unroll
             for (int i = 0; i < 5; i++) {
-                st.xold[i] = st.x[i];
-                st.x[i] = st.xold[i] + p[i];
+                s.at(SB_XOLD + i) = st.x[i];
+                st.x[i] = st.x[i] + p[i];
             }
             if (st.sym) st.x[1] = st.x[0];
             return;
@@ -1293,168 +1473,46 @@ __device__ __noinline__ void control_step(const ScanArgs& A, const Scr& s, St& s
 #pragma unroll
             for (int i = 0; i < 5; i++) {
                 st.x[i] = s.at(SB_X0 + i);
-                st.xold[i] = qnan();
+                s.at(SB_XOLD + i) = qnan();
             }
             st.sym = (st.x[0] == st.x[1]);
             return;
         }
-        attempt_done(A, s, st, SB_PRIM, gl);
+        attempt_done<LPP>(A, s, st, SB_PRIM, gl);
         return;
     }
     // trust region
     if (st.fresh && !all_finite5(F)) {
         // the fallback throws: the primary candidate is returned
         st.fresh = false;
-        attempt_done(A, s, st, SB_PRIM, gl);
+        attempt_done<LPP>(A, s, st, SB_PRIM, gl);
         return;
     }
     if (tr_step(A, s, st, omega, F, Jp)) return;
-    attempt_done(A, s, st, choose_candidate(A, s), gl);
-}
-
-// Staged pipeline: sets up attempt st.k of point st.pt for the kernel's stage.
-__device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st, int gl)
-{
-    int64_t i = st.pt;
-    st.T = A.T[i];
-    st.mu = A.mu[i];
-    st.xi = A.xi[i];
-    st.flags = 0;
-    st.steps = 0;
-    st.passes = 0;
-    st.trips = 0;
-    st.have_best = false;
-    st.plan_multi = false;
-    double x0[5];
-    if (A.stage == STAGE_SURR) {
-        if (A.judge_mode == PNJL_JUDGE_REFERENCE && A.judge_mask && in_judge_region(A, st.T, st.mu, st.xi)) {
-            // ambiguity region: the reference policy takes the point unseen
-            if (gl == 0) {
-                double* r = A.rec_out + st.item_ix * REC_SIZE;
-                r[RF_CONV] = 0.0;
-                r[RF_FLAGS] = (double)PNJL_PT_SURR_MASKED;
-                unsigned long long j = atomicAdd(A.next_count, 1ULL);
-                A.next_list[j] = (int32_t)st.pt;
-            }
-            st.phase = PH_IDLE;
-            return;
-        }
-#pragma unroll
-        for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
-        start_attempt(st, s, ATT_SURR, x0);
-        return;
-    }
-    if (A.stage == STAGE_MULTI) {
-        multi_seed(st.k, st.T, st.mu, x0);
-        start_attempt(st, s, st.k, x0);
-        return;
-    }
-    if (A.stage == STAGE_JUDGE || A.stage == STAGE_JUDGE2) {
-        // the Omega check of a learned-seed root: the MultiSeed candidates
-        const int c = (A.stage == STAGE_JUDGE) ? st.k : 2 + st.k;
-        multi_seed(c, st.T, st.mu, x0);
-        start_attempt(st, s, c, x0);
-        return;
-    }
-    if (A.stage == STAGE_MR3) {
-        // learned seed, then the chirally broken and restored reference seeds
-        if (st.k == 0) {
-#pragma unroll
-            for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
-        } else {
-            seed_const(st.k == 1 ? 0 : 1, x0);
-        }
-        start_attempt(st, s, st.k, x0);
-        return;
-    }
-    // STAGE_PRIMARY
-    switch (A.policy) {
-    case PNJL_SEED_HADRON: default_seed(1, st.T, st.mu, x0); break;
-    case PNJL_SEED_QUARK: default_seed(2, st.T, st.mu, x0); break;
-    case PNJL_SEED_EXPLICIT:
-#pragma unroll
-        for (int k = 0; k < 5; k++) x0[k] = A.seed[i * 5 + k];
-        break;
-    default: default_seed(0, st.T, st.mu, x0); break;
-    }
-    start_attempt(st, s, ATT_PRIMARY, x0);
+    attempt_done<LPP>(A, s, st, choose_candidate(A, s), gl);
 }
 
 // ---------------------------------------------------------------------------
 // the persistent kernel
 // ---------------------------------------------------------------------------
-// next work point for the group; false when the queue is drained
-template <int LPP>
-__device__ __forceinline__ bool next_point(const ScanArgs& A, St& st, int gl, bool chains, int64_t n_items)
-{
-    if (A.stage != STAGE_NONE) {
-        if (st.item >= st.item_end) {
-            unsigned long long base = 0;
-            if (gl == 0) base = atomicAdd(A.queue, (unsigned long long)A.chunk);
-            base = __shfl(base, 0, LPP);
-            st.item = (int64_t)base;
-            st.item_end = st.item + A.chunk;
-            if (st.item_end > n_items) st.item_end = n_items;
-        }
-        if (st.item >= st.item_end) return false;
-        int64_t it = st.item++;
-        int64_t j = it / A.mult;
-        st.k = (int)(it - j * A.mult);
-        st.pt = A.item_list ? (int64_t)A.item_list[j] : j;
-        // records: per point for single-attempt stages, per item otherwise;
-        // judge records: JUDGE_SEEDS per judge index
-        if (A.stage == STAGE_JUDGE) st.item_ix = j * JUDGE_SEEDS + st.k;
-        else if (A.stage == STAGE_JUDGE2) st.item_ix = (int64_t)A.jslot[st.pt] * JUDGE_SEEDS + 2 + st.k;
-        else st.item_ix = (A.mult == 1) ? st.pt : it;
-        return true;
-    }
-    if (chains && st.pt + 1 < st.pt_end) {
-        st.pt++;
-        return true;
-    }
-    if (st.item >= st.item_end) {
-        unsigned long long base = 0;
-        if (gl == 0) base = atomicAdd(A.queue, (unsigned long long)A.chunk);
-        base = __shfl(base, 0, LPP);
-        st.item = (int64_t)base;
-        st.item_end = st.item + A.chunk;
-        if (st.item_end > A.n_items) st.item_end = A.n_items;
-    }
-    if (st.item >= st.item_end) return false;
-    int64_t it = st.item++;
-    if (chains) {
-        st.pt = it * A.chain_len;
-        st.pt_end = st.pt + A.chain_len;
-        if (st.pt_end > A.n_points) st.pt_end = A.n_points;
-        st.have_prev = false;
-        st.prev_phase = 0;
-        st.prev_mass = 0.0;
-    } else {
-        st.pt = it;
-        st.pt_end = it + 1;
-    }
-    return true;
-}
+__host__ __device__ inline int table_len(int pn, int tn) { return kMathTab + pn + tn + pn * tn + pn; }
 
-// 3 waves per SIMD: every function fits 168 VGPRs; the node pass is a
-// separate function so that its loop keeps all of them (no spills), the
-// control path spills a little once per trip.
+// 3 waves per SIMD; every function is inlined and the group's state is in
+// LDS, so the whole loop fits the register budget without spills.
 template <int LPP>
 __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
 {
     extern __shared__ double lds[];
-    const int tab_len = kMathTab + A.pn + A.tn + A.pn * A.tn + A.pn;
+    const int tab_len = table_len(A.pn, A.tn);
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) lds[i] = A.tab[i];
     __syncthreads();
 
     const int gl = threadIdx.x % LPP;
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPP;
-    if (g >= A.n_groups) return;  // beyond the scratch the host allocated
-    const Scr s{A.scratch + g * SCRATCH_FIELDS};
-    const bool chains = (A.stage == STAGE_NONE) && chain_policy(A.policy);
-    const int64_t n_items = (A.stage == STAGE_NONE) ? A.n_items
-                            : (A.item_list ? (int64_t)(*A.item_count) : A.n_points) * A.mult;
-    St st;
+    if (g >= A.n_groups) return;  // beyond the record area the host allocated
+    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE, A.records + g * REC_AREA};
+    const bool chains = (A.plan == PLAN_CHAINS);
+    St& st = *reinterpret_cast<St*>(s.l + SB_ST);
     st.phase = PH_IDLE;
     st.item = 0;
     st.item_end = 0;
@@ -1464,16 +1522,21 @@ __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
     st.prev_phase = 0;
     st.prev_mass = 0.0;
     st.k = 0;
-    st.item_ix = 0;
+    st.kind = 0;
+    st.idx = 0;
+    const int64_t n_items = chains ? A.n_items
+                                   : (A.item_list ? (int64_t)(*A.item_count) : A.n_points) * (int64_t)A.mult;
 
     while (true) {
         if (st.phase == PH_IDLE) {
-            if (!next_point<LPP>(A, st, gl, chains, n_items)) break;
-            if (A.stage == STAGE_NONE)
-                begin_point(A, s, st);
-            else
+            if (chains) {
+                if (!next_chain_point<LPP>(A, st, gl)) break;
+                begin_chain_point(A, s, st);
+            } else {
+                if (!pipeline_next<LPP>(A, st, gl, n_items)) break;
                 begin_item(A, s, st, gl);
-            if (st.phase == PH_IDLE) continue;   // handed on without an attempt
+            }
+            if (st.phase == PH_IDLE) continue;
         }
         {
             double x[5];
@@ -1482,18 +1545,36 @@ __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
             node_pass<LPP>(A.P, lds, A.pn, A.tn, s, x, st.T, st.mu, st.xi, st.sym, gl);
             st.passes += st.sym ? 2 : 3;
         }
-        control_step(A, s, st, gl);
+        control_step<LPP>(A, s, st, gl);
         if (st.phase != PH_IDLE && ++st.trips > A.max_trips) {
             // safety exit: report the point as failed and move on
             double xn[5] = {qnan(), qnan(), qnan(), qnan(), qnan()};
             put_exc_rec(s, SB_SINGLE, xn);
-            s.at(SB_SINGLE + RF_CONV) = 0.0;
-            if (A.stage == STAGE_NONE)
-                finish_point(A, s, st, SB_SINGLE, gl);
+            s.rec(SB_SINGLE + RF_CONV) = 0.0;
+            if (chains)
+                finish_point<LPP>(A, s, st, SB_SINGLE, gl);
             else
-                staged_done(A, s, st, SB_SINGLE, gl, false, true);
+                item_done(A, s, st, SB_SINGLE, gl, false, true);
         }
     }
+}
+
+// the learned-seed plan's initial order: points the judge sends to the plain
+// policy first (they run longest), then the others
+__global__ void surr_order_kernel(ScanArgs A, int32_t* order, unsigned long long* counters)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.n_points) return;
+    const bool masked = in_judge_region(A, A.T[p], A.mu[p], A.xi[p]);
+    if (masked)
+        order[atomicAdd(counters + 0, 1ULL)] = (int32_t)p;
+    else
+        order[A.n_points - 1 - (int64_t)atomicAdd(counters + 1, 1ULL)] = (int32_t)p;
+}
+
+__global__ void zero_u64(unsigned long long* q, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) q[i] = 0ULL;
 }
 
 // ---------------------------------------------------------------------------
@@ -1506,20 +1587,19 @@ struct ResArgs {
     double *F, *J, *omega;
     const double* tab;
     int pn, tn;
-    double* scratch;   // n * SCRATCH_FIELDS doubles
 };
 
 template <int LPP>
 __global__ void __launch_bounds__(256, 3) residual_kernel(ResArgs R)
 {
     extern __shared__ double lds[];
-    const int tab_len = kMathTab + R.pn + R.tn + R.pn * R.tn + R.pn;
+    const int tab_len = table_len(R.pn, R.tn);
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) lds[i] = R.tab[i];
     __syncthreads();
     const int gl = threadIdx.x % LPP;
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPP;
     if (g >= R.n) return;
-    const Scr s{R.scratch + g * SCRATCH_FIELDS};
+    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE, nullptr};
     double x[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) x[i] = R.x[g * 5 + i];
@@ -1538,11 +1618,6 @@ __global__ void __launch_bounds__(256, 3) residual_kernel(ResArgs R)
     }
 }
 
-__global__ void zero_u64(unsigned long long* p, int n)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ULL;
-}
-
 __global__ void zero_i32(int32_t* p, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1550,10 +1625,8 @@ __global__ void zero_i32(int32_t* p, int64_t n)
 }
 
 // ---------------------------------------------------------------------------
-// staged pipeline: per-point selection of the final result
+// per-point selection of the final result
 // ---------------------------------------------------------------------------
-enum Plan { PLAN_DEFAULT = 0, PLAN_SURR = 1, PLAN_MULTI = 2, PLAN_MR3 = 3 };
-
 struct FinArgs {
     int64_t n;
     int plan;

@@ -174,48 +174,108 @@ class Surrogate:
         self.judge_mask = z["judge_mask"] if "judge_mask" in z.files else None
         return self
 
-    def build_judge_mask(self, ctx, T, mu, xi, plain_roots=None, p_num=64, t_num=4, opts=None, bins=(64, 64, 8),
-                         dilate=1, multi_root=False):
-        """Ambiguity mask of the learned seed's judge, learned on probe points
-        (the training set, optionally with more random points): a bin is set
-        where the learned-seed root, the plain (DefaultSeed) root and the
-        MultiSeed (argmin-Omega) root do not all agree, or a solve failed;
-        with `multi_root`, also where the learned, HADRON and QUARK seeds reach
-        distinct roots.  The set is dilated by `dilate` bins.  Stored in
-        self.judge_mask and installed in `ctx`."""
-        from ._lib import SEED_DEFAULT, SEED_MULTI, SEED_SURROGATE
+    def build_judge_mask(self, ctx, T, mu, xi, plain_roots=None, p_num=64, t_num=4, opts=None, bins=(128, 128, 16),
+                         dilate=1, multi_root=False, refine=8, refine_seed=6, neighbourhood="full"):
+        """Ambiguity mask of the learned seed's judge, learned on probe points:
+        a bin is set where the learned-seed root, the plain (DefaultSeed) root
+        and the MultiSeed (argmin-Omega) root are not all the same root, or a
+        solve failed; with `multi_root`, also where the learned, HADRON and
+        QUARK seeds reach distinct roots.
+
+        Two passes: the given probes (the training set, optionally with more
+        random points), then -- when `refine` > 0 -- `refine` further random
+        probes in every bin within two bins of a set one (default_rng
+        (refine_seed)), where ambiguity is likely.  The set bins are dilated by
+        `dilate` bins over their 26 neighbours (`neighbourhood` "full") or 6
+        faces ("face").  Stored in self.judge_mask and installed in `ctx`;
+        returns (mask, ambiguous probes of the first pass)."""
         T, mu, xi = _d(T), _d(mu), _d(xi)
-        if plain_roots is None:
-            plain_roots = ctx.solve_batch(T, mu, xi, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=opts)["x"]
-        seeds = self.infer(T, mu, xi)
-        ctx.set_judge_mask(None, None, None)
-        q = ctx.solve_batch(T, mu, xi, policy=SEED_SURROGATE, seed=seeds, p_num=p_num, t_num=t_num, opts=opts)
-        m = ctx.solve_batch(T, mu, xi, policy=SEED_MULTI, p_num=p_num, t_num=t_num, opts=opts)
-
-        def differ(a, b):
-            return np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)), axis=1) > 1e-8
-
-        amb = differ(q["x"], m["x"]) | differ(_d(plain_roots), m["x"]) | (q["converged"] != 1) | (m["converged"] != 1)
-        if multi_root:
-            # also every probe where the HADRON and QUARK seeds reach distinct roots
-            from ._lib import PT_FIRST_ORDER, SEED_MULTIROOT3
-            r3 = ctx.solve_batch(T, mu, xi, policy=SEED_MULTIROOT3, seed=seeds, p_num=p_num, t_num=t_num, opts=opts)
-            amb |= (r3["flags"] & PT_FIRST_ORDER) != 0
         nb = np.asarray(bins)
-        u = (np.stack([T, mu, xi], 1) - self.in_lo) / (self.in_hi - self.in_lo)
-        b = np.clip((u * nb).astype(int), 0, nb - 1)
+        amb = self._ambiguous(ctx, T, mu, xi, plain_roots, p_num, t_num, opts, multi_root)
         mask = np.zeros(tuple(nb), dtype=np.uint8)
+        b = self._bins(T, mu, xi, nb)
         mask[b[amb, 0], b[amb, 1], b[amb, 2]] = 1
-        for _ in range(dilate):
-            grown = mask.copy()
-            for ax in range(3):
-                grown |= np.roll(mask, 1, ax) * (np.arange(nb[ax]) > 0).reshape([-1 if k == ax else 1 for k in range(3)])
-                grown |= np.roll(mask, -1, ax) * (np.arange(nb[ax]) < nb[ax] - 1).reshape(
-                    [-1 if k == ax else 1 for k in range(3)])
-            mask = grown.astype(np.uint8)
+        if refine > 0 and mask.any():
+            region = _dilate(mask, 2, full=True)
+            cells = np.argwhere(region)
+            rng = np.random.default_rng(refine_seed)
+            pick = np.repeat(np.arange(cells.shape[0]), refine)
+            u = (cells[pick] + rng.random((pick.size, 3))) / nb
+            P = self.in_lo + u * (self.in_hi - self.in_lo)
+            amb2 = self._ambiguous(ctx, P[:, 0], P[:, 1], P[:, 2], None, p_num, t_num, opts, multi_root)
+            b2 = self._bins(P[:, 0], P[:, 1], P[:, 2], nb)
+            mask[b2[amb2, 0], b2[amb2, 1], b2[amb2, 2]] = 1
+        mask = _dilate(mask, dilate, full=(neighbourhood == "full"))
         self.judge_mask = mask
         ctx.set_judge_mask(mask, self.in_lo, self.in_hi)
         return mask, amb
+
+    def _bins(self, T, mu, xi, nb):
+        u = (np.stack([T, mu, xi], 1) - self.in_lo) / (self.in_hi - self.in_lo)
+        return np.clip((u * nb).astype(int), 0, nb - 1)
+
+    def _ambiguous(self, ctx, T, mu, xi, plain_roots, p_num, t_num, opts, multi_root):
+        """Probes at which the learned-seed, DefaultSeed and MultiSeed solves do
+        not all reach the same root (DualBranchScan's sameness test and Omega
+        to 1e-10), or one of them failed."""
+        from ._lib import SEED_DEFAULT, SEED_MULTI, SEED_SURROGATE
+        saved = getattr(self, "judge_mask", None)
+        ctx.set_judge_mask(None, None, None)
+        try:
+            if plain_roots is None:
+                pr = ctx.solve_batch(T, mu, xi, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=opts)
+            else:
+                pr = plain_roots
+            seeds = self.infer(T, mu, xi)
+            q = ctx.solve_batch(T, mu, xi, policy=SEED_SURROGATE, seed=seeds, p_num=p_num, t_num=t_num, opts=opts)
+            m = ctx.solve_batch(T, mu, xi, policy=SEED_MULTI, p_num=p_num, t_num=t_num, opts=opts)
+            amb = (q["converged"] != 1) | (m["converged"] != 1) | ~same_root(q, m)
+            if isinstance(pr, dict):
+                amb |= (pr["converged"] != 1) | ~same_root(pr, m)
+            else:
+                amb |= np.max(np.abs(_d(pr) - m["x"]) / np.maximum(1.0, np.abs(m["x"])), axis=1) > 1e-8
+            if multi_root:
+                # also every probe where the HADRON and QUARK seeds reach distinct roots
+                from ._lib import PT_FIRST_ORDER, SEED_MULTIROOT3
+                r3 = ctx.solve_batch(T, mu, xi, policy=SEED_MULTIROOT3, seed=seeds, p_num=p_num, t_num=t_num,
+                                     opts=opts)
+                amb |= (r3["flags"] & PT_FIRST_ORDER) != 0
+        finally:
+            if saved is not None:
+                ctx.set_judge_mask(saved, self.in_lo, self.in_hi)
+        return amb
+
+
+def same_root(a, b, omega_rtol=1e-10):
+    """Per point: do solve results a and b (dicts with x, masses, omega) hold
+    the same root?  DualBranchScan's sameness test (|d phi_u| < 0.01, |d Phi|
+    < 0.01, |d M_u| < 0.01 fm^-1; DualBranchScan.jl:409-419) and Omega equal
+    to omega_rtol -- two Newton runs that stop within the solver's
+    tolerances of one root can differ in x by more than 1e-9 where the
+    Hessian is nearly singular (next to the critical end point), never in
+    Omega, which is stationary there."""
+    ax, bx = np.asarray(a["x"]), np.asarray(b["x"])
+    am, bm = np.asarray(a["masses"]), np.asarray(b["masses"])
+    ao, bo = np.asarray(a["omega"]), np.asarray(b["omega"])
+    return ((np.abs(ax[:, 0] - bx[:, 0]) < 0.01) & (np.abs(ax[:, 3] - bx[:, 3]) < 0.01)
+            & (np.abs(am[:, 0] - bm[:, 0]) < 0.01) & (np.abs(ao - bo) <= omega_rtol * np.abs(bo)))
+
+
+def _dilate(mask, n, full=True):
+    """n steps of binary dilation over the 26 (full) or 6 (face) neighbours."""
+    m = mask.astype(bool)
+    for _ in range(n):
+        g = m.copy()
+        p = np.pad(m, 1)
+        nb = m.shape
+        for d0 in (-1, 0, 1):
+            for d1 in (-1, 0, 1):
+                for d2 in (-1, 0, 1):
+                    if not full and abs(d0) + abs(d1) + abs(d2) != 1:
+                        continue
+                    g |= p[1 + d0:1 + d0 + nb[0], 1 + d1:1 + d1 + nb[1], 1 + d2:1 + d2 + nb[2]]
+        m = g
+    return m.astype(np.uint8)
 
 
 def training_points(n=200_000, seed=4):

@@ -1160,6 +1160,17 @@ static int same_root(const cand_t* a, const cand_t* b)
            fabs(a->M[0] - b->M[0]) * 197.327 < 0.01 * 197.327;
 }
 
+/* A learned-seed run that lands farther from its seed than DualBranchScan's
+ * jump test allows a continuation seed (|d phi_u| > 0.5 or |d M_u| > 50 MeV,
+ * DualBranchScan.jl:410-415) has left the seed's basin: its root is not
+ * trusted and the plain policy takes the point. */
+static int learned_jump(const double* seed, const cand_t* c)
+{
+    double M0[3];
+    mass_vec(seed, M0, NULL, NULL);
+    return fabs(c->x[0] - seed[0]) > 0.5 || fabs(c->M[0] - M0[0]) * 197.327 > 50.0;
+}
+
 static void solve_surrogate(const prob_t* pb, const double* seed, const pnjl_solve_opts* o, point_t* out)
 {
     const int in_mask = g_judge.mask && judge_region(pb->T, pb->mu_vec[0], pb->xi);
@@ -1179,7 +1190,7 @@ static void solve_surrogate(const prob_t* pb, const double* seed, const pnjl_sol
     single_t s;
     solve_single(&capped, seed, &oc, &s);
     int steps = s.steps, judged;
-    if (!s.exception && s.converged) {
+    if (!s.exception && s.converged && !learned_jump(seed, &s.c)) {
         from_single(&s, out);
         judged = 1;
     } else {

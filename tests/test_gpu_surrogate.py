@@ -113,21 +113,22 @@ def test_learned_seed_policies_match_oracle(gpu_ctx, oracle, policy, judge):
     """SURROGATE (both judge modes) and MULTIROOT3 through the C ABI against
     the oracle's restatement, same seeds and judge mask, 1e-10 parity."""
     from julia_relaxtime_amd import _lib
-    from test_gpu_gap import assert_parity
+    from test_gpu_gap import assert_parity, oracle_rerun
     T, mu, xi, seeds, mask, lo, hi = _policy_case(oracle)
     gpu_ctx.set_judge_mask(mask, lo, hi)
     oracle.set_judge_mask(mask, lo, hi)
+    oo = oracle.default_opts(ftol=1e-10, max_iter=40, surrogate_judge=judge)
+    sel = judge == 1 or policy == 8
     try:
         r = gpu_ctx.solve_batch(T, mu, xi, policy=policy, seed=seeds, p_num=64, t_num=4,
                                 opts=_lib.default_opts(ftol=1e-10, max_iter=40, surrogate_judge=judge))
-        o = oracle.solve_batch(T, mu, xi, policy=policy, seed=seeds, p_num=64, t_num=4, nthreads=8,
-                               opts=oracle.default_opts(ftol=1e-10, max_iter=40, surrogate_judge=judge))
+        o = oracle.solve_batch(T, mu, xi, policy=policy, seed=seeds, p_num=64, t_num=4, nthreads=8, opts=oo)
+        assert o["converged"].mean() > 0.95
+        assert_parity(r, o, T, mu, oracle_rerun(oracle, xi, policy, 64, 4, oo, seed=seeds), selection=sel,
+                      label=f"policy {policy} judge {judge}")
     finally:
         gpu_ctx.set_judge_mask(None, None, None)
         oracle.set_judge_mask(None)
-    assert o["converged"].mean() > 0.95
-    sel = judge == 1 or policy == 8
-    assert_parity(r, o, selection=sel)
     # selection policies: of two runs tied on one root either may win, and
     # with it its own trust-region bit
     keep = ~_lib.PT_TR_USED if sel else ~0
@@ -170,10 +171,27 @@ def _c4_grid():
     return g[0].ravel() / HB, g[1].ravel() / HB, g[2].ravel()
 
 
+def plain_root_unstable(oracle, T, mu, xi, P, idx, opts):
+    """At points idx: is the plain DefaultSeed root P rounding-sensitive?  True
+    where the oracle's DefaultSeed root differs from P (the GPU adds its
+    node sums in another order) or moves when T or mu moves by one ulp."""
+    from test_gpu_gap import _xdiff, oracle_rerun, ulp_unstable
+    idx = np.asarray(idx, dtype=np.int64)
+    if idx.size == 0:
+        return np.zeros(0, dtype=bool)
+    run = oracle_rerun(oracle, xi, 0, 64, 4, opts)
+    o = run(T[idx], mu[idx], idx)
+    u = (o["converged"] != P["converged"][idx]) | (_xdiff(o["x"], P["x"][idx]) > 1e-9)
+    full = {"x": np.zeros((T.size, 5)), "converged": np.zeros(T.size, dtype=np.int32)}
+    full["x"][idx], full["converged"][idx] = o["x"], o["converged"]
+    return u | ulp_unstable(run, T, mu, idx, full, rtol=1e-9)
+
+
 def test_c4_learned_seed_scan_reproduces_reference_roots(gpu_ctx, oracle, c4_model):
     """256x256x17 grid: the learned-seed scan (reference judge) returns the
-    plain solver's root at every point but a handful, at |res| < 1e-10, with
-    far fewer node passes; a sample is checked against the oracle."""
+    plain solver's root at |res| < 1e-10 at every point where that root is
+    not itself rounding-sensitive (no mask misses), with far fewer node
+    passes; a sample is checked against the oracle point by point."""
     import torch
     from julia_relaxtime_amd import _lib
     s, loss = c4_model
@@ -190,8 +208,19 @@ def test_c4_learned_seed_scan_reproduces_reference_roots(gpu_ctx, oracle, c4_mod
                                                               opts=opts).items()}
     assert np.all(Q["converged"] == 1)
     assert Q["residual_norm"].max() <= 1e-10
-    same = np.max(np.abs(Q["x"] - P["x"]) / np.maximum(1, np.abs(P["x"])), axis=1) < 1e-9
-    assert (~same).sum() <= 50, (~same).sum()
+    from julia_relaxtime_amd.surrogate import same_root
+    same = same_root(Q, P)
+    apart = same & (np.max(np.abs(Q["x"] - P["x"]) / np.maximum(1, np.abs(P["x"])), axis=1) >= 1e-9)
+    oo = oracle.default_opts(ftol=1e-10, max_iter=40)
+    off = np.flatnonzero(~same)
+    unstable = plain_root_unstable(oracle, T, mu, xi, P, off, oo)
+    miss = off[~unstable]
+    higher = off[Q["omega"][off] > P["omega"][off]]
+    print(f"[c4 reference judge] {n} points: {off.size} on another root than the plain solve, {int(unstable.sum())} "
+          f"of them where the plain root is rounding-sensitive, {miss.size} mask misses ({higher.size} with higher "
+          f"Omega); {int(apart.sum())} on the same root with x more than 1e-9 apart (nearly singular Hessian)",
+          flush=True)
+    assert miss.size == 0, [(float(T[i] * HB), float(mu[i] * HB), float(xi[i])) for i in miss[:10]]
     assert np.all(np.abs(Q["omega"][same] - P["omega"][same]) <= 1e-10 * np.abs(P["omega"][same]))
     assert Q["passes"].sum() < 0.4 * P["passes"].sum()
     seed_err = np.abs(seeds.cpu().numpy() - Q["x"]).max(axis=1)
@@ -201,20 +230,24 @@ def test_c4_learned_seed_scan_reproduces_reference_roots(gpu_ctx, oracle, c4_mod
     special = np.flatnonzero((Q["flags"] & (_lib.PT_SURR_MASKED | _lib.PT_SURR_FALLBACK)) != 0)
     idx = np.unique(np.r_[rng.choice(n, 1500, replace=False), rng.choice(special, min(500, special.size),
                                                                          replace=False)])
+    sd = seeds.cpu().numpy()
+    from test_gpu_gap import assert_parity, oracle_rerun
     oracle.set_judge_mask(s.judge_mask, s.in_lo, s.in_hi)
     try:
-        o = oracle.solve_batch(T[idx], mu[idx], xi[idx], policy=7, seed=seeds.cpu().numpy()[idx], p_num=64,
-                               t_num=4, nthreads=8, opts=oracle.default_opts(ftol=1e-10, max_iter=40))
+        o = oracle.solve_batch(T[idx], mu[idx], xi[idx], policy=7, seed=sd[idx], p_num=64, t_num=4, nthreads=8,
+                               opts=oo)
+        assert_parity({k: v[idx] for k, v in Q.items()}, o, T[idx], mu[idx],
+                      oracle_rerun(oracle, xi[idx], 7, 64, 4, oo, seed=sd[idx]), selection=True,
+                      label="c4 sample")
     finally:
         oracle.set_judge_mask(None)
-    from test_gpu_gap import assert_parity
-    assert_parity({k: v[idx] for k, v in Q.items()}, o, max_chaotic=0.01, selection=True)
 
 
-def test_c4_omega_judge_keeps_the_stable_root(gpu_ctx, c4_model):
+def test_c4_omega_judge_keeps_the_stable_root(gpu_ctx, oracle, c4_model):
     """Omega judge: inside the mask the root with the lowest Omega among the
     learned root and the MultiSeed candidates is kept -- never a higher-Omega
-    root than the plain solver's except where the mask misses."""
+    root than the plain solver's, except where the plain root itself is
+    rounding-sensitive."""
     import torch
     from julia_relaxtime_amd import _lib
     s, _ = c4_model
@@ -229,9 +262,13 @@ def test_c4_omega_judge_keeps_the_stable_root(gpu_ctx, c4_model):
         Td, mud, xid, policy=7, seed=seeds, p_num=64, t_num=4,
         opts=_lib.default_opts(ftol=1e-10, max_iter=40, surrogate_judge=_lib.JUDGE_OMEGA)).items()}
     assert np.all(Q["converged"] == 1)
-    higher = Q["omega"] > P["omega"] + 1e-9 * np.abs(P["omega"])
+    from julia_relaxtime_amd.surrogate import same_root
+    higher = np.flatnonzero((Q["omega"] > P["omega"] + 1e-9 * np.abs(P["omega"])) & ~same_root(Q, P))
     lower = Q["omega"] < P["omega"] - 1e-9 * np.abs(P["omega"])
-    assert higher.mean() < 2e-3, higher.sum()
+    unstable = plain_root_unstable(oracle, T, mu, xi, P, higher, oracle.default_opts(ftol=1e-10, max_iter=40))
+    print(f"[c4 omega judge] {T.size} points: {int(lower.sum())} below the plain root's Omega, {higher.size} above "
+          f"it, {int(unstable.sum())} of those where the plain root is rounding-sensitive", flush=True)
+    assert np.all(unstable), [(float(T[i] * HB), float(mu[i] * HB), float(xi[i])) for i in higher[~unstable][:10]]
     assert lower.sum() > 1000          # the plain solver's metastable roots are replaced
     # mostly through the judge (a learned root may also sit on the stable branch by itself)
     assert np.mean((Q["flags"][lower] & (_lib.PT_SURR_OMEGA | _lib.PT_FIRST_ORDER)) != 0) > 0.9

@@ -59,8 +59,13 @@ def test_learned_seed_cutoff_and_fallback(oracle, case):
         assert np.array_equal(r[k][~fb], ex[k][~fb]), k
         assert np.array_equal(r[k][fb], plain[k][fb], equal_nan=True), k
     assert np.all(r["iterations"][~fb] <= 8)
-    # the fallback points are those whose explicit run needs more than 8 steps or fails
-    slow = (ex["iterations"] > 8) | (ex["converged"] != 1)
+    # the fallback points are those whose explicit run needs more than 8 steps or
+    # fails, or lands farther from its seed than DualBranchScan's jump test allows
+    # (|d phi_u| > 0.5 or |d M_u| > 50 MeV, DualBranchScan.jl:410-415)
+    P = oracle.default_params()
+    m_seed = P.m_ud0 - 4 * P.G * seeds[:, 0] + 2 * P.K * seeds[:, 1] * seeds[:, 2]
+    jump = (np.abs(ex["x"][:, 0] - seeds[:, 0]) > 0.5) | (np.abs(ex["masses"][:, 0] - m_seed) * 197.327 > 50.0)
+    slow = (ex["iterations"] > 8) | (ex["converged"] != 1) | ((ex["converged"] == 1) & jump)
     assert np.array_equal(fb, slow)
 
 
