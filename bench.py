@@ -251,12 +251,14 @@ def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
 
 
 # HBM bytes per scan_kernel launch of the learned-seed scan (c4_surrogate):
-# FETCH_SIZE + WRITE_SIZE of its three launches per step from two rocprofv3
-# --pmc runs of this bench (8.1 + 7.5, 14.8 + 13.0, 13.8 + 15.2 GiB), divided
-# by 3.  WRITE_SIZE reads exactly 1 GiB for a 1 GiB copy; FETCH_SIZE is
-# unscaled.  Against ~0.3 KB of algorithmic bytes per point the excess is the
-# kernel's call-frame and per-group scratch traffic.
-C4_TRAFFIC_BYTES_PER_LAUNCH = (8.1 + 7.5 + 14.8 + 13.0 + 13.8 + 15.2) / 3 * 2 ** 30
+# FETCH_SIZE + WRITE_SIZE of its two launches per step, from two rocprofv3
+# --pmc runs of this bench over three timed steps (FETCH 9.71 + 4.51 GiB,
+# WRITE 10.06 + 4.55 GiB per step), divided by the two launches.  A 1 GiB
+# device copy reads 0.5 GiB in FETCH_SIZE and exactly 1 GiB in WRITE_SIZE,
+# so FETCH is scaled by 2.  Against ~0.3 KB of algorithmic bytes per point
+# the excess is the kernel's candidate records and the stack frames of its
+# out-of-line cold functions.
+C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (9.71 + 4.51) + 10.06 + 4.55) / 2 * 2 ** 30
 
 # launch geometry of the learned seed's set-up solves (training targets, judge
 # mask): fixed, so that the trained network does not depend on what the
@@ -685,6 +687,64 @@ def run_maxwell(args, w):
     print(json.dumps(line), flush=True)
 
 
+def root_agreement(ctx, w, opts, T, mu, xi, Td, mud, xid, Q, P, cfg, plain_policy):
+    """Points where the learned-seed scan Q reports another root than the plain
+    scan P (DualBranchScan's sameness test, DualBranchScan.jl:409-419, and
+    Omega to 1e-10), split into points whose plain root is itself rounding-
+    sensitive -- it changes when the plain scan runs at the learned scan's
+    launch geometry (another summation order) or when T or mu moves by one
+    ulp -- and mask misses; and the points on the same root whose x differ by
+    more than 1e-9 (next to the critical end point the Hessian is nearly
+    singular)."""
+    from julia_relaxtime_amd.surrogate import same_root
+    saved = ctx.launch_cfg()
+    ctx.set_launch_cfg(**cfg)
+    P2 = {k: v.cpu().numpy() for k, v in ctx.solve_batch(Td, mud, xid, policy=plain_policy, p_num=w["p_num"],
+                                                          t_num=w["t_num"], opts=opts).items()}
+    ctx.set_launch_cfg(**saved)
+    same = same_root(Q, P)
+    off = np.flatnonzero(~same)
+    unstable = ~same_root({k: P2[k][off] for k in ("x", "masses", "omega")},
+                          {k: P[k][off] for k in ("x", "masses", "omega")})
+    if off.size:
+        for dT, dm in ((np.inf, None), (-np.inf, None), (None, np.inf), (None, -np.inf)):
+            Tp = T[off] if dT is None else np.nextafter(T[off], dT)
+            mp = mu[off] if dm is None else np.nextafter(mu[off], dm)
+            r = ctx.solve_batch(Tp, mp, xi[off], policy=plain_policy, p_num=w["p_num"], t_num=w["t_num"], opts=opts)
+            unstable |= ~same_root(r, {k: P[k][off] for k in ("x", "masses", "omega")})
+    apart = same & (np.max(np.abs(Q["x"] - P["x"]) / np.maximum(1, np.abs(P["x"])), axis=1) >= 1e-9)
+    return dict(points_on_another_root=int(off.size), of_which_plain_root_rounding_sensitive=int(unstable.sum()),
+                mask_misses=int((~unstable).sum()),
+                mask_misses_higher_omega=int(np.sum(Q["omega"][off[~unstable]] > P["omega"][off[~unstable]])),
+                same_root_x_apart_1e9=int(apart.sum()), same_root_as_plain=float(same.mean()),
+                same_root_as_plain_equal_geometry=float(same_root(Q, P2).mean()))
+
+
+def judge_everywhere(ctx, w, opts, model, Td, mud, xid, seeds, cfg, stream, args):
+    """The learned-seed scan with the Omega judge on every point (the whole box
+    in the mask): each converged learned root is checked against the roots of
+    the HADRON and hint-QUARK seeds (all six MultiSeed candidates where one of
+    them fails) and the minimum-Omega root is kept.  Timed like the step, for
+    the saving of the learned mask."""
+    from julia_relaxtime_amd import _lib
+    full = np.ones_like(model.judge_mask)
+    o = _lib.default_opts(ftol=1e-10, max_iter=w["max_iter"], surrogate_judge=_lib.JUDGE_OMEGA)
+    ctx.set_judge_mask(full, model.in_lo, model.in_hi)
+    ctx.set_launch_cfg(**cfg)
+    holder = {}
+    try:
+        def step():
+            holder["out"] = ctx.solve_batch(Td, mud, xid, policy=_lib.SEED_SURROGATE, seed=seeds, p_num=w["p_num"],
+                                            t_num=w["t_num"], opts=o, out=holder.get("out"), stream=stream, sync=True)
+            return float(np.sum(ctx.last_scan_ms()))
+        ms, _ = timed_steps(step, max(1, min(args.steps, 3)), 1)
+    finally:
+        ctx.set_judge_mask(model.judge_mask, model.in_lo, model.in_hi)
+    R = holder["out"]
+    solved = int(((R["converged"] == 1) & (R["residual_norm"] < 1e-10)).sum().item())
+    return dict(judge_everywhere_ms_per_step=ms, judge_everywhere_points_per_s=solved / (ms * 1e-3))
+
+
 def timed_steps(step, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -800,6 +860,8 @@ def main():
                          "(profiling runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (calibrates HBM byte counters)")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling runs: skip the root-agreement and judge-everywhere comparisons")
     ap.add_argument("--flop-probe", default=None, choices=sorted(FLOP_PROBES),
                     help="run one FLOP probe (for rocprofv3 FP64 counters) instead of a workload")
     args = ap.parse_args()
@@ -886,12 +948,14 @@ def main():
         Q = {k: v.cpu().numpy() for k, v in state["out"].items()}
         flops = scan_flops(state["out"])
         solved = int(np.sum((Q["converged"] == 1) & (Q["residual_norm"] < 1e-10)))
-        same = np.max(np.abs(Q["x"] - P["x"]) / np.maximum(1, np.abs(P["x"])), axis=1) < 1e-9
         seed_err = np.abs(state["seeds"].cpu().numpy() - Q["x"]).max(axis=1)
+        if w["policy"] == "SURROGATE" and not args.profile:
+            extra.update(root_agreement(ctx, w, opts, T, mu, xi, Td, mud, xid, Q, P, cfg, plain_policy))
+            extra.update(judge_everywhere(ctx, w, opts, model, Td, mud, xid, state["seeds"], cfg, stream, args))
         extra.update(
             plain_points_per_s=solved_plain / (ms_plain * 1e-3), plain_ms_per_step=ms_plain,
             speedup_vs_plain=(solved / (ms_step * 1e-3)) / (solved_plain / (ms_plain * 1e-3)),
-            same_root_as_plain=float(same.mean()), points_not_on_plain_root=int((~same).sum()),
+            break_even_grids=(extra["setup_s"] * 1e3 / (ms_plain - ms_step)) if ms_plain > ms_step else None,
             fallback_fraction=float(np.mean((Q["flags"] & _lib.PT_SURR_FALLBACK) != 0)),
             judged_fraction=float(np.mean((Q["flags"] & _lib.PT_SURR_MASKED) != 0)),
             seed_to_root_err_median=float(np.median(seed_err)), seed_to_root_err_p90=float(np.quantile(seed_err, 0.9)),

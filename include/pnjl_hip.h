@@ -266,9 +266,9 @@ typedef struct pnjl_launch_cfg {
     int chunk;            /* points taken from the work queue per pop        */
     int waves_per_cu;     /* persistent grid size = 256 CUs x waves_per_cu   */
     int tail_lanes_per_point; /* lanes per point of the later pipeline stages
-                             (fallback attempts, MultiSeed candidates,
-                             judge runs: few, long-running items); 0 = same
-                             as lanes_per_point                         */
+                             (MultiSeed candidates, Omega-judge candidates:
+                             few, long-running items; they run on the full
+                             persistent grid); 0 = same as lanes_per_point */
 } pnjl_launch_cfg;
 
 int pnjl_set_launch_cfg(pnjl_ctx* ctx, const pnjl_launch_cfg* cfg);
@@ -279,7 +279,7 @@ int pnjl_get_launch_cfg(const pnjl_ctx* ctx, pnjl_launch_cfg* cfg);
  * once, best of two timed runs) -- first every lanes_per_point (points per
  * wavefront and nodes per lane) at chunk 1 and 12 waves/CU, then chunk
  * {1,2,4} x waves/CU {8,12,16} for the two best lane counts, then the later
- * stages' lanes {4,16,32} -- keeps the fastest in the context and returns it
+ * stages' lanes {16,32,64} -- keeps the fastest in the context and returns it
  * in *best (and its pipeline time per point in *ns_per_point).  No
  * counterpart in the reference.                                            */
 int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* mu, const double* xi,

@@ -296,9 +296,12 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     const int64_t need_groups = std::max<int64_t>(1, (work + A.chunk - 1) / A.chunk);
     if ((int64_t)blocks * G.groups_per_block > need_groups)
         blocks = (int)std::max<int64_t>(1, (need_groups + G.groups_per_block - 1) / G.groups_per_block);
-    // the later pipeline stages run the full persistent grid: records for the larger of the two
-    const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / waves_per_block);
-    const int64_t groups = std::max<int64_t>((int64_t)blocks, (int64_t)fullblocks) * G.groups_per_block;
+    // the later pipeline stages (few, long items) run the full persistent grid
+    // at their own lanes per point: records for the largest grid
+    const ScanGeom G2 = scan_geom(cfg.tail_lanes_per_point > 0 ? cfg.tail_lanes_per_point : G.lpp, pn, tn);
+    const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / (G2.threads / 64));
+    const int64_t groups = std::max<int64_t>((int64_t)blocks * G.groups_per_block,
+                                             (int64_t)fullblocks * G2.groups_per_block);
     A.n_groups = groups;
     rc = ensure_scratch(ctx, (size_t)groups * pnjl::REC_AREA * sizeof(double));
     if (rc) return rc;
@@ -386,7 +389,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
         L.order = nullptr;
         L.item_list = (const int32_t*)list;
         L.item_count = q + count_word;
-        launch_scan(ctx, L, G, fullblocks, stream);
+        launch_scan(ctx, L, G2, fullblocks, stream);
         HIPCHK(ctx, hipGetLastError());
         return PNJL_OK;
     };
@@ -883,6 +886,11 @@ int pnjl_autotune(pnjl_ctx* ctx, int64_t sample, const double* T, const double* 
             for (int w : {8, 12, 16})
                 if (!(c == 1 && w == 12))
                     if (int rc = trial(pnjl_launch_cfg{lanes[i], c, w, 0}, nullptr)) return rc;
+    // 3) lanes of the later stages (MultiSeed and judge candidates: few, long items)
+    const pnjl_launch_cfg b2 = bcfg;
+    for (int t : {16, 32, 64})
+        if (t != b2.lanes_per_point)
+            if (int rc = trial(pnjl_launch_cfg{b2.lanes_per_point, b2.chunk, b2.waves_per_cu, t}, nullptr)) return rc;
     ctx->cfg = bcfg;
     if (best) *best = bcfg;
     if (ns_per_point) *ns_per_point = bms * 1e6 / (double)sample;
