@@ -210,6 +210,29 @@ int pnjl_solve_fixedmu_batch(pnjl_ctx* ctx, int64_t n,
                              const pnjl_solve_out* out,
                              int flags, void* stream);
 
+/* pnjl_solve_fixedmu_batch without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, policies and results as pnjl_solve_fixedmu_batch, but
+ * every array is a device pointer (flags must hold PNJL_FLAG_DEVICE_PTRS)
+ * and n <= 8192 points per call (larger n: PNJL_ERR_ARG).  The node table,
+ * work-queue counters, candidate records and per-point lists live in one
+ * fixed scratch array of ~41 MB per GPU, held from the library's load until
+ * the process ends; its address is looked up once per device by
+ * pnjl_ctx_create.  Every call on a GPU shares that GPU's array, so calls
+ * that overlap in time on one GPU race (serialise them on one stream).  A
+ * call whose p_num/t_num differ from the previous call's on that GPU
+ * rebuilds the node table in the array and first waits for the device.
+ * The phase-boundary tables and judge mask are the context's (set up
+ * beforehand with pnjl_set_phase_boundary / pnjl_set_judge_mask).          */
+int pnjl_solve_fixedmu_batch_noalloc(pnjl_ctx* ctx, int64_t n,
+                                     const double* T, const double* mu, const double* xi,
+                                     const double* seed, int seed_policy,
+                                     int p_num, int t_num,
+                                     const pnjl_solve_opts* opts,
+                                     const pnjl_solve_out* out,
+                                     int flags, void* stream);
+
 /* Implicit thermodynamic derivatives at given roots x[n*5] of the FixedMu gap
  * equations (solve first; xi and the node counts as in the solve): out[n*36],
  * per point, in this order --

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_PKG, "libpnjl_hip.so")
 # status codes
 PNJL_OK, PNJL_ERR_ARG, PNJL_ERR_HIP, PNJL_ERR_NO_DEVICE, PNJL_ERR_CHANNEL, PNJL_ERR_STATE = range(6)
 FLAG_DEVICE_PTRS, FLAG_ASYNC, FLAG_ONE_PER_WAVE = 0x1, 0x2, 0x4
+# points per call of pnjl_solve_fixedmu_batch_noalloc
+NOALLOC_MAX_POINTS = 8192
 
 # seed policies
 SEED_DEFAULT, SEED_HADRON, SEED_QUARK, SEED_MULTI, SEED_EXPLICIT = 0, 1, 2, 3, 4
@@ -132,6 +134,7 @@ def load():
     L.pnjl_get_launch_cfg.argtypes = [vp, C.POINTER(LaunchCfg)]
     L.pnjl_solve_fixedmu_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
                                            C.POINTER(SolveOpts), C.POINTER(SolveOut), C.c_int, vp]
+    L.pnjl_solve_fixedmu_batch_noalloc.argtypes = L.pnjl_solve_fixedmu_batch.argtypes
     L.pnjl_gap_residual_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp,
                                           C.c_int, vp]
     L.pnjl_thermo_derivatives_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, vp, C.c_int, vp]
@@ -139,6 +142,7 @@ def load():
                                 C.POINTER(SolveOpts), C.POINTER(LaunchCfg), C.POINTER(C.c_double)]
     for name in ("pnjl_ctx_create", "pnjl_gauleg", "pnjl_nodes", "pnjl_set_phase_boundary", "pnjl_set_judge_mask",
                  "pnjl_set_launch_cfg", "pnjl_get_launch_cfg", "pnjl_solve_fixedmu_batch",
+                 "pnjl_solve_fixedmu_batch_noalloc",
                  "pnjl_gap_residual_batch", "pnjl_thermo_derivatives_batch", "pnjl_autotune"):
         getattr(L, name).restype = C.c_int
     _register_extra(L)
@@ -287,7 +291,19 @@ class Context:
         self._check(rc, "pnjl_solve_fixedmu_batch")
         return arrs
 
-    def _solve_torch(self, T, mu, xi, policy, seed, p_num, t_num, o, out, stream, sync):
+    def solve_batch_noalloc(self, T, mu, xi, policy=SEED_DEFAULT, seed=None, p_num=64, t_num=8, opts=None, out=None,
+                            stream=None, sync=True):
+        """solve_batch through pnjl_solve_fixedmu_batch_noalloc: torch.cuda
+        tensors only, at most NOALLOC_MAX_POINTS points; the library allocates
+        nothing (pass `out` to allocate nothing here either)."""
+        if not _is_torch(T):
+            raise PnjlError("solve_batch_noalloc takes device tensors only")
+        o = opts if opts is not None else default_opts()
+        return self._solve_torch(T, mu, xi, policy, seed, p_num, t_num, o, out, stream, sync,
+                                 entry="pnjl_solve_fixedmu_batch_noalloc")
+
+    def _solve_torch(self, T, mu, xi, policy, seed, p_num, t_num, o, out, stream, sync,
+                     entry="pnjl_solve_fixedmu_batch"):
         import torch
         n = T.numel()
         dev = T.device
@@ -301,10 +317,10 @@ class Context:
                         for k, v in out.items()})
         st = stream if stream is not None else torch.cuda.current_stream(dev)
         flags = FLAG_DEVICE_PTRS | (0 if sync else FLAG_ASYNC)
-        rc = self._L.pnjl_solve_fixedmu_batch(self.h, n, T.data_ptr(), mu.data_ptr(), xi.data_ptr(),
-                                              seed.data_ptr() if seed is not None else None, policy, p_num, t_num,
-                                              C.byref(o), C.byref(s), flags, C.c_void_p(st.cuda_stream))
-        self._check(rc, "pnjl_solve_fixedmu_batch")
+        rc = getattr(self._L, entry)(self.h, n, T.data_ptr(), mu.data_ptr(), xi.data_ptr(),
+                                     seed.data_ptr() if seed is not None else None, policy, p_num, t_num,
+                                     C.byref(o), C.byref(s), flags, C.c_void_p(st.cuda_stream))
+        self._check(rc, entry)
         return out
 
     def residual_batch(self, x, T, mu, xi, p_num=64, t_num=8):

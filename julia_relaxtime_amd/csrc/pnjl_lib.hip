@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -24,11 +25,11 @@ using namespace pnjl_host;
 namespace pnjl_host {
 
 // n-point Gauss-Legendre rule on [-1,1], ascending (Newton iteration on the
-// Legendre three-term recurrence from the Tricomi-type initial guesses).
-void gl_rule(int n, std::vector<double>& x, std::vector<double>& w)
+// Legendre three-term recurrence from the Tricomi-type initial guesses), into
+// x[n], w[n].
+void gl_rule_raw(int n, double* x, double* w)
 {
-    x.assign(n, 0.0);
-    w.assign(n, 0.0);
+    for (int i = 0; i < n; i++) x[i] = w[i] = 0.0;
     const double pi = 3.14159265358979323846;
     for (int i = 0; i < (n + 1) / 2; i++) {
         double z = cos(pi * (i + 0.75) / (n + 0.5));
@@ -63,14 +64,29 @@ void gl_rule(int n, std::vector<double>& x, std::vector<double>& w)
     if (n % 2 == 1) x[n / 2] = 0.0;
 }
 
-void gauleg_host(double a, double b, int n, std::vector<double>& x, std::vector<double>& w)
+void gl_rule(int n, std::vector<double>& x, std::vector<double>& w)
 {
-    gl_rule(n, x, w);
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    gl_rule_raw(n, x.data(), w.data());
+}
+
+// Gauss-Legendre rule on [a, b] into x[n], w[n]
+void gauleg_raw(double a, double b, int n, double* x, double* w)
+{
+    gl_rule_raw(n, x, w);
     double sc = (b - a) / 2.0, sh = (b + a) / 2.0;
     for (int i = 0; i < n; i++) {
         x[i] = sc * x[i] + sh;
         w[i] = sc * w[i];
     }
+}
+
+void gauleg_host(double a, double b, int n, std::vector<double>& x, std::vector<double>& w)
+{
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    gauleg_raw(a, b, n, x.data(), w.data());
 }
 
 // meshes of cached_nodes (Integrals.jl:87-97)
@@ -106,6 +122,32 @@ void math_tables(double* t)
     }
 }
 
+// Node table of (pn, tn) into t[table_len(pn, tn)]: math tables | p^2 [pn] |
+// cos^2 [tn] | C [pn*tn] | Cs [pn], with C the coefficients of cached_nodes
+// (Integrals.jl:87-97) and Cs their sums over the angle nodes.  work holds
+// 2 pn + 2 tn doubles.  Allocates nothing.
+void fill_tab(int pn, int tn, double* t, double* work)
+{
+    math_tables(t);
+    double *px = work, *pw = work + pn, *tx = work + 2 * pn, *tw = work + 2 * pn + tn;
+    gauleg_raw(0.0, 10.0, pn, px, pw);
+    gauleg_raw(0.0, 1.0, tn, tx, tw);
+    const double pi = 3.14159265358979323846;
+    const double twopi2 = (2.0 * pi) * (2.0 * pi);
+    double* nt = t + pnjl::kMathTab;
+    double* C = nt + pn + tn;
+    for (int j = 0; j < pn; j++) nt[j] = px[j] * px[j];
+    for (int l = 0; l < tn; l++) nt[pn + l] = tx[l] * tx[l];
+    for (int l = 0; l < tn; l++)
+        for (int j = 0; j < pn; j++)
+            C[(size_t)j + (size_t)l * pn] = (pw[j] * (tw[l] * 2.0)) * (px[j] * px[j]) / twopi2;
+    for (int j = 0; j < pn; j++) {
+        double s = 0.0;
+        for (int l = 0; l < tn; l++) s += C[(size_t)j + (size_t)l * pn];
+        C[(size_t)pn * tn + j] = s;
+    }
+}
+
 int get_tab(pnjl_ctx* ctx, int pn, int tn, double** out)
 {
     auto key = std::make_pair(pn, tn);
@@ -114,20 +156,8 @@ int get_tab(pnjl_ctx* ctx, int pn, int tn, double** out)
         *out = it->second;
         return PNJL_OK;
     }
-    std::vector<double> p, c, C;
-    build_meshes(pn, tn, p, c, C);
-    // device table: math tables[kMathTab] | pp[pn] | cc[tn] | C[pn*tn] | Cs[pn]
-    std::vector<double> tab(pnjl::kMathTab + (size_t)pn + tn + (size_t)pn * tn + pn);
-    math_tables(tab.data());
-    double* nt = tab.data() + pnjl::kMathTab;
-    for (int j = 0; j < pn; j++) nt[j] = p[j] * p[j];
-    for (int l = 0; l < tn; l++) nt[pn + l] = c[(size_t)l * pn] * c[(size_t)l * pn];
-    for (size_t k = 0; k < (size_t)pn * tn; k++) nt[pn + tn + k] = C[k];
-    for (int j = 0; j < pn; j++) {
-        double s = 0.0;
-        for (int l = 0; l < tn; l++) s += C[(size_t)j + (size_t)l * pn];
-        nt[pn + tn + (size_t)pn * tn + j] = s;
-    }
+    std::vector<double> tab((size_t)pnjl::table_len(pn, tn)), work(2 * (size_t)pn + 2 * (size_t)tn);
+    fill_tab(pn, tn, tab.data(), work.data());
     double* d = nullptr;
     HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(double)));
     HIPCHK(ctx, hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -226,19 +256,94 @@ void launch_scan(pnjl_ctx* ctx, const pnjl::ScanArgs& A, const ScanGeom& G, int 
     if (k >= 0) (void)hipEventRecord(ctx->ev_scan[2 * k + 1], st);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of pnjl_solve_fixedmu_batch_noalloc: one file-scope device
+// array per GPU, carved into the regions a solve needs beyond its inputs and
+// outputs.  Calls of the no-alloc entry point on one GPU share it: calls that
+// overlap in time on one GPU race.
+constexpr int64_t kNaPoints = 8192;    // points per call
+constexpr int64_t kNaGroups = 16384;   // scan-kernel groups (candidate-record areas)
+constexpr size_t kNaTabLen = 12544;    // doubles >= table_len(pn, tn) for pn * tn <= 4096, pn <= 4096
+constexpr size_t na_round(size_t d) { return (d + 31) / 32 * 32; }   // 256-byte boundaries
+constexpr size_t kNaQueue = 0;                                                      // kQueueWords u64
+constexpr size_t kNaTab = na_round(kNaQueue + kQueueWords);
+constexpr size_t kNaRec = na_round(kNaTab + kNaTabLen);
+constexpr size_t kNaRecS = na_round(kNaRec + (size_t)kNaGroups * pnjl::REC_AREA);
+constexpr size_t kNaRecP = na_round(kNaRecS + (size_t)kNaPoints * pnjl::REC_SIZE);
+constexpr size_t kNaRecM = na_round(kNaRecP + (size_t)kNaPoints * pnjl::REC_SIZE);
+constexpr size_t kNaRecJ = na_round(kNaRecM + (size_t)kNaPoints * 6 * pnjl::REC_SIZE);
+constexpr size_t kNaLists = na_round(kNaRecJ + (size_t)kNaPoints * pnjl::JUDGE_SEEDS * pnjl::REC_SIZE);
+constexpr int kNaNumLists = 7;         // mlist, mslot, order, jlist, jslot, jb, j2list (int32, kNaPoints each)
+constexpr size_t kNaDoubles = na_round(kNaLists + (size_t)kNaNumLists * kNaPoints / 2);
+static_assert(kNaTabLen >= (size_t)pnjl::kMathTab + 4096 + 1 + 4096 + 4096, "node table region");
+
+}  // namespace pnjl_host
+
+static __device__ __align__(256) double scratch[pnjl_host::kNaDoubles];
+
+namespace pnjl_host {
+
+constexpr int kNaMaxDevices = 64;
+struct NaDevice {
+    double* base = nullptr;   // device address of `scratch` on this GPU
+    int pn = 0, tn = 0;       // node table held in its table region
+};
+static NaDevice g_na[kNaMaxDevices];
+static std::mutex g_na_mu;
+static double g_na_host[kNaTabLen + 2 * 4096 + 2 * 64];   // host image of a table + fill_tab work
+
+// once per device, with that device current: the device address of `scratch`
+int na_init(pnjl_ctx* ctx)
+{
+    if (ctx->device < 0 || ctx->device >= kNaMaxDevices) return PNJL_OK;   // the entry point reports it
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    NaDevice& d = g_na[ctx->device];
+    if (d.base) return PNJL_OK;
+    double* ptr = nullptr;
+    HIPCHK(ctx, hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_err(ctx, PNJL_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    d.base = ptr;
+    return PNJL_OK;
+}
+
+// the node table of (pn, tn) in the device's scratch, rebuilt when the node
+// counts change (after the device has finished every earlier call)
+int na_table(pnjl_ctx* ctx, int pn, int tn, hipStream_t st, double** out)
+{
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    NaDevice& d = g_na[ctx->device];
+    double* dev = d.base + kNaTab;
+    if (d.pn != pn || d.tn != tn) {
+        HIPCHK(ctx, hipDeviceSynchronize());
+        fill_tab(pn, tn, g_na_host, g_na_host + kNaTabLen);
+        HIPCHK(ctx, hipMemcpyAsync(dev, g_na_host, (size_t)pnjl::table_len(pn, tn) * sizeof(double),
+                                   hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        d.pn = pn;
+        d.tn = tn;
+    }
+    *out = dev;
+    return PNJL_OK;
+}
+
 int check_policy(int p)
 {
     return (p >= PNJL_SEED_DEFAULT && p <= PNJL_SEED_BRANCH_QUARK) ? PNJL_OK : PNJL_ERR_ARG;
 }
 
-// core of pnjl_solve_fixedmu_batch with device pointers
+// core of pnjl_solve_fixedmu_batch with device pointers.  fixed: the node
+// table, queue words, candidate records and per-point lists live in the
+// device's no-alloc scratch (n <= kNaPoints, the grid capped at kNaGroups
+// groups) instead of buffers the context allocates; the lanes per point, and
+// with them every point's result, are the same either way.
 int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const double* xi, const double* seed,
               int policy, int pn, int tn, const pnjl_solve_opts* o, const pnjl_solve_out* out,
-              const pnjl_launch_cfg& cfg, hipStream_t stream)
+              const pnjl_launch_cfg& cfg, hipStream_t stream, bool fixed = false)
 {
     double* tab = nullptr;
-    int rc = get_tab(ctx, pn, tn, &tab);
+    int rc = fixed ? na_table(ctx, pn, tn, stream, &tab) : get_tab(ctx, pn, tn, &tab);
     if (rc) return rc;
+    double* const na = fixed ? g_na[ctx->device].base : nullptr;
     ctx->n_scans = 0;
     pnjl::ScanArgs A{};
     A.P = ctx->dp;
@@ -283,7 +388,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     A.auto_ms = o->auto_multiseed_fallback;
     A.surr_cap = o->surrogate_max_steps;
     A.judge_mode = o->surrogate_judge;
-    A.queue = ctx->queue;
+    A.queue = fixed ? (unsigned long long*)(na + kNaQueue) : ctx->queue;
     A.chunk = std::max(1, cfg.chunk);
     A.bnd = ctx->bnd;
     A.nb = ctx->nb;
@@ -299,14 +404,23 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     // the later pipeline stages (few, long items) run the full persistent grid
     // at their own lanes per point: records for the largest grid
     const ScanGeom G2 = scan_geom(cfg.tail_lanes_per_point > 0 ? cfg.tail_lanes_per_point : G.lpp, pn, tn);
-    const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / (G2.threads / 64));
+    int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / (G2.threads / 64));
+    if (fixed) {
+        blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, kNaGroups / G.groups_per_block));
+        fullblocks = (int)std::max<int64_t>(1, std::min<int64_t>(fullblocks, kNaGroups / G2.groups_per_block));
+    }
     const int64_t groups = std::max<int64_t>((int64_t)blocks * G.groups_per_block,
                                              (int64_t)fullblocks * G2.groups_per_block);
     A.n_groups = groups;
-    rc = ensure_scratch(ctx, (size_t)groups * pnjl::REC_AREA * sizeof(double));
-    if (rc) return rc;
-    A.records = ctx->scratch;
-    unsigned long long* q = ctx->queue;
+    if (fixed) {
+        if (groups > kNaGroups) return set_err(ctx, PNJL_ERR_ARG, "no-alloc solve: launch grid beyond the scratch");
+        A.records = na + kNaRec;
+    } else {
+        rc = ensure_scratch(ctx, (size_t)groups * pnjl::REC_AREA * sizeof(double));
+        if (rc) return rc;
+        A.records = ctx->scratch;
+    }
+    unsigned long long* q = A.queue;
     if (chains) {
         hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, kQueueWords);
         HIPCHK(ctx, hipGetLastError());
@@ -321,21 +435,28 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     const bool omega_judge = surr && ctx->jmask && o->surrogate_judge == PNJL_JUDGE_OMEGA;
     void *recS = nullptr, *recP = nullptr, *recM = nullptr, *mlist = nullptr, *mslot = nullptr, *order = nullptr;
     void *recJ = nullptr, *jlist = nullptr, *jslot = nullptr, *jb = nullptr, *j2list = nullptr;
-    if ((rc = stage_buf(ctx, 40, (size_t)n * rec * 6, &recM))) return rc;
+    // staging slot k of `bytes`, or its fixed region (offset in doubles) of the no-alloc scratch
+    auto buf = [&](size_t k, size_t bytes, size_t region, void** p) -> int {
+        if (!fixed) return stage_buf(ctx, k, bytes, p);
+        *p = na + region;
+        return PNJL_OK;
+    };
+    auto list = [](int i) { return kNaLists + (size_t)i * kNaPoints / 2; };
+    if ((rc = buf(40, (size_t)n * rec * 6, kNaRecM, &recM))) return rc;
     if (!multi && !mr3) {
-        if ((rc = stage_buf(ctx, 41, (size_t)n * rec, &recP))) return rc;
-        if ((rc = stage_buf(ctx, 42, (size_t)n * 4, &mlist))) return rc;
-        if ((rc = stage_buf(ctx, 43, (size_t)n * 4, &mslot))) return rc;
+        if ((rc = buf(41, (size_t)n * rec, kNaRecP, &recP))) return rc;
+        if ((rc = buf(42, (size_t)n * 4, list(0), &mlist))) return rc;
+        if ((rc = buf(43, (size_t)n * 4, list(1), &mslot))) return rc;
     }
     if (surr) {
-        if ((rc = stage_buf(ctx, 44, (size_t)n * rec, &recS))) return rc;
-        if ((rc = stage_buf(ctx, 45, (size_t)n * 4, &order))) return rc;
+        if ((rc = buf(44, (size_t)n * rec, kNaRecS, &recS))) return rc;
+        if ((rc = buf(45, (size_t)n * 4, list(2), &order))) return rc;
         if (omega_judge) {
-            if ((rc = stage_buf(ctx, 46, (size_t)n * pnjl::JUDGE_SEEDS * rec, &recJ))) return rc;
-            if ((rc = stage_buf(ctx, 47, (size_t)n * 4, &jlist))) return rc;
-            if ((rc = stage_buf(ctx, 48, (size_t)n * 4, &jslot))) return rc;
-            if ((rc = stage_buf(ctx, 49, (size_t)n * 4, &jb))) return rc;
-            if ((rc = stage_buf(ctx, 50, (size_t)n * 4, &j2list))) return rc;
+            if ((rc = buf(46, (size_t)n * pnjl::JUDGE_SEEDS * rec, kNaRecJ, &recJ))) return rc;
+            if ((rc = buf(47, (size_t)n * 4, list(3), &jlist))) return rc;
+            if ((rc = buf(48, (size_t)n * 4, list(4), &jslot))) return rc;
+            if ((rc = buf(49, (size_t)n * 4, list(5), &jb))) return rc;
+            if ((rc = buf(50, (size_t)n * 4, list(6), &j2list))) return rc;
         }
     }
     A.recS = (double*)recS;
@@ -493,6 +614,11 @@ int pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out)
     if (!ev_ok || hipMalloc(&c->queue, kQueueWords * sizeof(unsigned long long)) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
+        return PNJL_ERR_HIP;
+    }
+    // the device address of the no-alloc entry point's scratch, once per device
+    if (na_init(c) != PNJL_OK) {
+        pnjl_ctx_destroy(c);
         return PNJL_ERR_HIP;
     }
     *out = c;
@@ -655,7 +781,8 @@ int pnjl_get_launch_cfg(const pnjl_ctx* ctx, pnjl_launch_cfg* cfg)
 
 static int solve_impl(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const double* xi,
                       const double* seed, int policy, int pn, int tn, const pnjl_solve_opts* opts,
-                      const pnjl_solve_out* out, int flags, void* stream, const pnjl_launch_cfg& cfg)
+                      const pnjl_solve_out* out, int flags, void* stream, const pnjl_launch_cfg& cfg,
+                      bool fixed = false)
 {
     if (!ctx) return PNJL_ERR_ARG;
     if (n < 0 || !T || !mu || !xi || !out || check_policy(policy))
@@ -680,7 +807,7 @@ static int solve_impl(pnjl_ctx* ctx, int64_t n, const double* T, const double* m
     hipStream_t st = (hipStream_t)stream;
     int rc;
     if (flags & PNJL_FLAG_DEVICE_PTRS) {
-        rc = solve_dev(ctx, n, T, mu, xi, seed, policy, pn, tn, &o, out, cfg, st);
+        rc = solve_dev(ctx, n, T, mu, xi, seed, policy, pn, tn, &o, out, cfg, st, fixed);
         if (rc) return rc;
     } else {
         // Host arrays travel packed: the inputs are gathered into one page-locked
@@ -762,6 +889,26 @@ int pnjl_solve_fixedmu_batch(pnjl_ctx* ctx, int64_t n, const double* T, const do
 {
     if (!ctx) return PNJL_ERR_ARG;
     return solve_impl(ctx, n, T, mu, xi, seed, seed_policy, p_num, t_num, opts, out, flags, stream, ctx->cfg);
+}
+
+int pnjl_solve_fixedmu_batch_noalloc(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const double* xi,
+                                     const double* seed, int seed_policy, int p_num, int t_num,
+                                     const pnjl_solve_opts* opts, const pnjl_solve_out* out, int flags,
+                                     void* stream)
+{
+    if (!ctx) return PNJL_ERR_ARG;
+    if (!(flags & PNJL_FLAG_DEVICE_PTRS))
+        return set_err(ctx, PNJL_ERR_ARG,
+                       "pnjl_solve_fixedmu_batch_noalloc: device pointers only (PNJL_FLAG_DEVICE_PTRS); host "
+                       "arrays would need staging buffers");
+    if (n > kNaPoints)
+        return set_err(ctx, PNJL_ERR_ARG, "pnjl_solve_fixedmu_batch_noalloc: n = " + std::to_string(n) +
+                                              " exceeds the maximum of " + std::to_string(kNaPoints) +
+                                              " points per call");
+    if (ctx->device < 0 || ctx->device >= kNaMaxDevices || !g_na[ctx->device].base)
+        return set_err(ctx, PNJL_ERR_STATE, "pnjl_solve_fixedmu_batch_noalloc: no scratch on this device");
+    return solve_impl(ctx, n, T, mu, xi, seed, seed_policy, p_num, t_num, opts, out, flags, stream, ctx->cfg,
+                      true);
 }
 
 int pnjl_gap_residual_batch(pnjl_ctx* ctx, int64_t n, const double* x, const double* T, const double* mu,
