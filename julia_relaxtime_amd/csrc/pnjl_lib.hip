@@ -225,7 +225,8 @@ ScanGeom scan_geom(int lpp, int pn, int tn)
 {
     const size_t tab = (size_t)pnjl::table_len(pn, tn) * sizeof(double);
     const size_t group = (size_t)pnjl::STATE_STRIDE * sizeof(double);
-    const size_t limit = 64 * 1024;
+    // 64 KB per block, less the block's copy of the launch arguments (static LDS)
+    const size_t limit = 64 * 1024 - (sizeof(pnjl::ScanArgs) + 255) / 256 * 256;
     for (int l = lpp; l <= 64; l *= 2) {
         for (int waves = 4; waves >= 1; waves--) {
             const int threads = 64 * waves;

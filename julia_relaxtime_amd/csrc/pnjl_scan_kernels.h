@@ -196,11 +196,20 @@ enum StateField {
     SB_PREV = SB_X0 + 5,         // chain: previous solution
     SB_XOLD = SB_PREV + 5,       // x of the previous Newton / trust-region step
     SB_ST = SB_XOLD + 5,         // the group's solver state (struct St, 24 doubles)
+    SB_FJ = SB_ST + 24,          // F[5] and packed J[15] of the last pass, for the trust-region step
     // an odd number of doubles per group: the groups of a wavefront hit
     // distinct LDS banks when they access the same field
-    STATE_STRIDE = SB_ST + 24 + ((SB_ST + 24) % 2 == 0 ? 1 : 0)
+    STATE_STRIDE = SB_FJ + 20 + ((SB_FJ + 20) % 2 == 0 ? 1 : 0)
 };
 enum TrField { TR_R = 0, TR_J = 5, TR_D = 20, TR_P = 25, TR_DELTA = 30 };
+
+// The launch arguments as the out-of-line (cold) functions see them: a copy
+// in LDS that every block writes at its start.  The kernel's inlined code
+// reads its by-value parameter from kernarg memory into scalar registers;
+// handing that parameter's address to a call instead would make the
+// compiler copy the whole struct to per-lane private memory and read every
+// field from there on every trip.
+__shared__ ScanArgs cold_args;
 
 // a group's storage: hot state in LDS (at), candidate records in HBM (rec)
 struct Scr {
@@ -451,9 +460,10 @@ __device__ __forceinline__ void assemble_core(const DevParams& P, const Scr& s, 
 // Candidate record at x: thermodynamics (calculate_thermo + densities, in
 // closed form from the sums) and the physicality gate
 // (_default_is_physical_solution + _all_finite_thermo, ImplicitSolver.jl:50-64).
-__device__ __noinline__ void put_rec(const DevParams& P, const Scr& s, int blk, const double* x, double mu, double T, bool sym,
+__device__ __noinline__ void put_rec(Scr s, int blk, const double* x, double mu, double T, bool sym,
                         double phi_tol, double omega, double res, int it, bool fconv, bool xconv)
 {
+    const DevParams& P = cold_args.P;
     double M[3];
     mass_vec(P, x, M);
     const double Phi = x[3], Phib = x[4];
@@ -505,10 +515,11 @@ __device__ __noinline__ void put_rec(const DevParams& P, const Scr& s, int blk, 
     s.rec(blk + RF_FLAGS) = 0.0;
 }
 
-__device__ __noinline__ void put_exc_rec(const Scr& s, int blk, const double* x0)
+// x0: the point's seed (null: NaN)
+__device__ __noinline__ void put_exc_rec(Scr s, int blk, const double* x0)
 {
 #pragma unroll
-    for (int i = 0; i < 5; i++) s.rec(blk + RF_X + i) = x0[i];
+    for (int i = 0; i < 5; i++) s.rec(blk + RF_X + i) = x0 ? x0[i] : qnan();
     for (int f = RF_OMEGA; f <= RF_RES; f++) s.rec(blk + f) = qnan();
     s.rec(blk + RF_IT) = 0.0;
     s.rec(blk + RF_FCONV) = 0.0;
@@ -949,8 +960,9 @@ __device__ __forceinline__ void finish_point(const ScanArgs& A, const Scr& s, St
 }
 
 // Sets up point st.pt of a chain according to the policy (first attempt).
-__device__ __noinline__ void begin_chain_point(const ScanArgs& A, const Scr& s, St& st)
+__device__ __noinline__ void begin_chain_point(Scr s, St& st)
 {
+    const ScanArgs& A = cold_args;
     int64_t i = st.pt;
     st.T = A.T[i];
     st.mu = A.mu[i];
@@ -1006,9 +1018,9 @@ __device__ __noinline__ void begin_chain_point(const ScanArgs& A, const Scr& s, 
 // A chain point's attempt ended with its result in record `blk`: the
 // automatic MultiSeed fallback runs its six candidates one after another.
 template <int LPP>
-__device__ __noinline__ void chain_attempt_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl,
-                                                   bool conv, bool exc)
+__device__ __noinline__ void chain_attempt_done(Scr s, St& st, int blk, int gl, bool conv, bool exc)
 {
+    const ScanArgs& A = cold_args;
     double x0[5];
     if (st.attempt == ATT_PRIMARY) {
         copy_rec(s, SB_SINGLE, blk);
@@ -1144,8 +1156,9 @@ __device__ __forceinline__ void begin_primary(const ScanArgs& A, const Scr& s, S
 }
 
 // Sets up the attempt of the item the group just took.
-__device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st, int gl)
+__device__ __noinline__ void begin_item(Scr s, St& st, int gl)
 {
+    const ScanArgs& A = cold_args;
     reset_point(A, st);
     const int64_t p = st.pt;
     double x0[5];
@@ -1192,9 +1205,9 @@ __device__ __noinline__ void begin_item(const ScanArgs& A, const Scr& s, St& st,
 // the MultiSeed list, a judged root to the Omega judge's list, a failed
 // judge candidate (HADRON, hint-QUARK) to the second judge list; a failed
 // learned-seed attempt continues with the primary attempt in the same group.
-__device__ __noinline__ void item_done(const ScanArgs& A, const Scr& s, St& st, int blk, int gl, bool conv,
-                                       bool exc)
+__device__ __noinline__ void item_done(Scr s, St& st, int blk, int gl, bool conv, bool exc)
 {
+    const ScanArgs& A = cold_args;
     const int64_t p = st.pt;
     const bool cont = (st.kind == K_SURR) && !conv;
     if (gl == 0) {
@@ -1269,14 +1282,15 @@ __device__ __forceinline__ void attempt_done(const ScanArgs& A, const Scr& s, St
     if (conv && A.plan == PLAN_SURR && st.kind == K_SURR && learned_jump(A, s, blk)) conv = false;
     s.rec(blk + RF_CONV) = conv ? 1.0 : 0.0;
     if (A.plan == PLAN_CHAINS)
-        chain_attempt_done<LPP>(A, s, st, blk, gl, conv, exc);
+        chain_attempt_done<LPP>(s, st, blk, gl, conv, exc);
     else
-        item_done(A, s, st, blk, gl, conv, exc);
+        item_done(s, st, blk, gl, conv, exc);
 }
 
 // _choose_candidate (ImplicitSolver.jl:72-101) between PRIM and TRC -> block
-__device__ __noinline__ int choose_candidate(const ScanArgs& A, const Scr& s)
+__device__ __noinline__ int choose_candidate(Scr s)
 {
+    const ScanArgs& A = cold_args;
     bool pg = rec_good(s, SB_PRIM, A.resmax);
     bool fg = rec_good(s, SB_TRC, A.resmax);
     double pres = s.rec(SB_PRIM + RF_RES), fres = s.rec(SB_TRC + RF_RES);
@@ -1305,15 +1319,18 @@ __device__ __noinline__ int choose_candidate(const ScanArgs& A, const Scr& s)
 // trust-region step bookkeeping (NLsolve trust_region_, autoscale, factor 1,
 // eta 1e-4), after an evaluation at st.x.  Returns true if another pass is
 // needed; otherwise the TR candidate is final in SB_TRC.
-__device__ __noinline__ bool tr_step(const ScanArgs& A, const Scr& s, St& st, double omega, const double F[5],
-                                        const double Jp[15])
+// F and the packed J of the pass are in the group's SB_FJ state.
+__device__ __noinline__ bool tr_step(Scr s, St& st, double omega)
 {
+    const ScanArgs& A = cold_args;
+    const double* F = s.l + SB_FJ;
+    const double* Jp = s.l + SB_FJ + 5;
     double r[5], d[5], p[5], delta;
     bool converged = false, stopped;
     if (st.fresh) {
         st.fresh = false;
         bool fconv = norm_inf5(F) < A.ftol;
-        put_rec(A.P, s, SB_TRC, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(F), 0, fconv, false);
+        put_rec(s, SB_TRC, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(F), 0, fconv, false);
 #pragma unroll
         for (int i = 0; i < 5; i++) r[i] = F[i];
 #pragma unroll
@@ -1378,7 +1395,7 @@ __device__ __noinline__ bool tr_step(const ScanArgs& A, const Scr& s, St& st, do
             bool xconv = !xnan && cheb < A.xtol;
             bool fconv = norm_inf5(r) < A.ftol;
             converged = xconv || fconv;
-            put_rec(A.P, s, SB_TRC, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(r), st.it, fconv, xconv);
+            put_rec(s, SB_TRC, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(r), st.it, fconv, xconv);
         } else {
 #pragma unroll
             for (int i = 0; i < 5; i++) st.x[i] = s.at(SB_XOLD + i);
@@ -1425,10 +1442,7 @@ __device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St
             st.fresh = false;
             if (!all_finite5(F)) {
                 // NLsolve check_isfinite throws on the seed
-                double x0[5];
-#pragma unroll
-                for (int i = 0; i < 5; i++) x0[i] = s.at(SB_X0 + i);
-                put_exc_rec(s, SB_PRIM, x0);
+                put_exc_rec(s, SB_PRIM, s.l + SB_X0);
                 attempt_done<LPP>(A, s, st, SB_PRIM, gl);
                 return;
             }
@@ -1464,7 +1478,7 @@ __device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St
         }
         // Newton finished: candidate at x
         double res = norm_inf5(F);
-        put_rec(A.P, s, SB_PRIM, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, res, st.it, fconv, xconv);
+        put_rec(s, SB_PRIM, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, res, st.it, fconv, xconv);
         bool good = rec_good(s, SB_PRIM, A.resmax);
         if (A.tr_fb && !good && st.attempt != ATT_SURR) {
             st.phase = PH_TR;
@@ -1488,8 +1502,12 @@ __device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St
         attempt_done<LPP>(A, s, st, SB_PRIM, gl);
         return;
     }
-    if (tr_step(A, s, st, omega, F, Jp)) return;
-    attempt_done<LPP>(A, s, st, choose_candidate(A, s), gl);
+#pragma unroll
+    for (int i = 0; i < 5; i++) s.at(SB_FJ + i) = F[i];
+#pragma unroll
+    for (int k = 0; k < 15; k++) s.at(SB_FJ + 5 + k) = Jp[k];
+    if (tr_step(s, st, omega)) return;
+    attempt_done<LPP>(A, s, st, choose_candidate(s), gl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1497,14 +1515,18 @@ __device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St
 // ---------------------------------------------------------------------------
 __host__ __device__ inline int table_len(int pn, int tn) { return kMathTab + pn + tn + pn * tn + pn; }
 
-// 3 waves per SIMD; every function is inlined and the group's state is in
-// LDS, so the whole loop fits the register budget without spills.
+// 3 waves per SIMD.  The node pass and the Newton step are inlined; the cold
+// bookkeeping (attempt and item ends, the trust-region step) is out of line
+// and reads the block's LDS copy of the arguments, and no local array's
+// address reaches a call, so the loop keeps its state in registers and LDS
+// (a few loop-invariant values are reloaded from private memory per trip).
 template <int LPP>
 __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
 {
     extern __shared__ double lds[];
     const int tab_len = table_len(A.pn, A.tn);
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) lds[i] = A.tab[i];
+    if (threadIdx.x == 0) cold_args = A;
     __syncthreads();
 
     const int gl = threadIdx.x % LPP;
@@ -1531,10 +1553,10 @@ __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
         if (st.phase == PH_IDLE) {
             if (chains) {
                 if (!next_chain_point<LPP>(A, st, gl)) break;
-                begin_chain_point(A, s, st);
+                begin_chain_point(s, st);
             } else {
                 if (!pipeline_next<LPP>(A, st, gl, n_items)) break;
-                begin_item(A, s, st, gl);
+                begin_item(s, st, gl);
             }
             if (st.phase == PH_IDLE) continue;
         }
@@ -1548,13 +1570,12 @@ __global__ void __launch_bounds__(256, 3) scan_kernel(ScanArgs A)
         control_step<LPP>(A, s, st, gl);
         if (st.phase != PH_IDLE && ++st.trips > A.max_trips) {
             // safety exit: report the point as failed and move on
-            double xn[5] = {qnan(), qnan(), qnan(), qnan(), qnan()};
-            put_exc_rec(s, SB_SINGLE, xn);
+            put_exc_rec(s, SB_SINGLE, nullptr);
             s.rec(SB_SINGLE + RF_CONV) = 0.0;
             if (chains)
                 finish_point<LPP>(A, s, st, SB_SINGLE, gl);
             else
-                item_done(A, s, st, SB_SINGLE, gl, false, true);
+                item_done(s, st, SB_SINGLE, gl, false, true);
         }
     }
 }
