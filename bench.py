@@ -252,13 +252,12 @@ def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
 
 # HBM bytes per scan_kernel launch of the learned-seed scan (c4_surrogate):
 # FETCH_SIZE + WRITE_SIZE of its two launches per step, from two rocprofv3
-# --pmc runs of this bench over three timed steps (FETCH 9.71 + 4.51 GiB,
-# WRITE 10.06 + 4.55 GiB per step), divided by the two launches.  A 1 GiB
+# --pmc runs of this bench over three timed steps (FETCH 0.1401 + 0.0119 GiB,
+# WRITE 0.3704 + 0.0548 GiB per step), divided by the two launches.  A 1 GiB
 # device copy reads 0.5 GiB in FETCH_SIZE and exactly 1 GiB in WRITE_SIZE,
-# so FETCH is scaled by 2.  Against ~0.3 KB of algorithmic bytes per point
-# the excess is the kernel's candidate records and the stack frames of its
-# out-of-line cold functions.
-C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (9.71 + 4.51) + 10.06 + 4.55) / 2 * 2 ** 30
+# so FETCH is scaled by 2.  The algorithmic bytes are ~0.25 KB per point
+# (inputs, seeds, outputs); the rest is the per-point attempt records.
+C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (0.1401 + 0.0119) + 0.3704 + 0.0548) / 2 * 2 ** 30
 
 # launch geometry of the learned seed's set-up solves (training targets, judge
 # mask): fixed, so that the trained network does not depend on what the

@@ -449,9 +449,9 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
         if ((rc = buf(42, (size_t)n * 4, list(0), &mlist))) return rc;
         if ((rc = buf(43, (size_t)n * 4, list(1), &mslot))) return rc;
     }
+    if ((rc = buf(45, (size_t)n * 4, list(2), &order))) return rc;
     if (surr) {
         if ((rc = buf(44, (size_t)n * rec, kNaRecS, &recS))) return rc;
-        if ((rc = buf(45, (size_t)n * 4, list(2), &order))) return rc;
         if (omega_judge) {
             if ((rc = buf(46, (size_t)n * pnjl::JUDGE_SEEDS * rec, kNaRecJ, &recJ))) return rc;
             if ((rc = buf(47, (size_t)n * 4, list(3), &jlist))) return rc;
@@ -482,19 +482,18 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     }
     // queue words: q[0] the running stage's item counter (cleared before each
     // stage), q[Q_MCOUNT..Q_J2COUNT] the list lengths, q[kOrderWords..] the
-    // ordering pass's counters
+    // ordering pass's class sizes and cursors
     HIPCHK(ctx, hipEventRecord(ctx->ev0, stream));
     hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, kQueueWords);
     if (A.passes)
         hipLaunchKernelGGL(pnjl::zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A.passes, n);
     if (A.steps)
         hipLaunchKernelGGL(pnjl::zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A.steps, n);
-    if (surr && ctx->jmask && o->surrogate_judge == PNJL_JUDGE_REFERENCE) {
-        // the judge's plain points first: they run longest
-        hipLaunchKernelGGL(pnjl::surr_order_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A,
-                           (int32_t*)order, q + kOrderWords);
-        A.order = (const int32_t*)order;
-    }
+    // stage-1 order: judged points first, xi = 0 points after the others
+    const dim3 ogrid((unsigned)((n + pnjl::kOrderBlock - 1) / pnjl::kOrderBlock)), oblock(pnjl::kOrderBlock);
+    hipLaunchKernelGGL(pnjl::order_kernel<false>, ogrid, oblock, 0, stream, A, (int32_t*)order, q + kOrderWords);
+    hipLaunchKernelGGL(pnjl::order_kernel<true>, ogrid, oblock, 0, stream, A, (int32_t*)order, q + kOrderWords);
+    A.order = (const int32_t*)order;
     HIPCHK(ctx, hipGetLastError());
     // stage 1: every point (or every (point, seed) of the multi-seed plans)
     pnjl::ScanArgs S = A;
