@@ -101,24 +101,39 @@ __device__ __forceinline__ double rcp_nr(double d)
     return fma(r, e, r);
 }
 
-// exp(x) for -800 <= x <= 700 after clamping (Tang's table-driven method, ACM
-// TOMS 15(2), 1989): x = (64 k + j) ln2/64 + r with |r| <= ln2/128 (Cody-Waite
-// reduction with a split ln2/64), exp(x) = 2^k 2^(j/64) e^r and e^r from its
-// degree-5 Taylor polynomial (truncation below 4e-17).  About 1 ulp; results
-// below the double range flush through ldexp.
+// a * b + c with c a constant, as one VOP3 v_fma_f64 that reads c from
+// scalar registers.  Left to itself the compiler keeps such a constant in one
+// vector register for the whole loop and emits v_fmac (c's register tied to
+// the result), so every use first copies c: a move per polynomial step.
+// Same operation, same rounding as fma(a, b, c).
+__device__ __forceinline__ double fma_k(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
+// exp(x) (Tang's table-driven method, ACM TOMS 15(2), 1989): x = (64 k + j)
+// ln2/64 + r with |r| <= ln2/128 (Cody-Waite reduction with a split ln2/64),
+// exp(x) = 2^k 2^(j/64) e^r and e^r from its degree-5 Taylor polynomial
+// (truncation below 4e-17).  About 1 ulp; results below the double range
+// flush through ldexp.  kClamp: x is first clamped to [-800, 700]; without
+// it the caller guarantees x <= 700 (below -800 both forms give 0, and the
+// conversion of nd to int saturates on this hardware).
+template <bool kClamp = true>
 __device__ __forceinline__ double exp_tab(double x, const double* tab)
 {
     NODE_MATH_EXACT
     const double k64_ln2 = 0x1.71547652b82fep+6;     // 64 / ln 2
     const double ln2_64_hi = 0x1.62e42fee00000p-7;   // ln2/64, 21 trailing zero bits
     const double ln2_64_lo = 0x1.a39ef35793c76p-39;
-    x = fmin(fmax(x, -800.0), 700.0);
+    if (kClamp) x = fmin(fmax(x, -800.0), 700.0);
     const double nd = rint(x * k64_ln2);
     double r = fma(nd, -ln2_64_hi, x);
     r = fma(nd, -ln2_64_lo, r);
     const int n = (int)nd;
-    double p = fma(r, 1.0 / 120.0, 1.0 / 24.0);
-    p = fma(r, p, 1.0 / 6.0);
+    double p = fma_k(r, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma_k(r, p, 1.0 / 6.0);
     p = fma(r, p, 0.5);
     p = fma(r, p, 1.0);
     p = fma(r, p, 1.0);
@@ -140,10 +155,10 @@ __device__ __forceinline__ double log_tab(double x, const double* tab)
     const int j = (int)((uint32_t)(__double2hiint(m)) >> 14) & 63;
     const double2 t = *(const double2*)(tab + kLogTab + 2 * j);
     const double r = fma(m, t.x, -1.0);
-    double p = fma(r, 1.0 / 7.0, -1.0 / 6.0);
-    p = fma(r, p, 1.0 / 5.0);
-    p = fma(r, p, -0.25);
-    p = fma(r, p, 1.0 / 3.0);
+    double p = fma_k(r, 1.0 / 7.0, -1.0 / 6.0);
+    p = fma_k(r, p, 1.0 / 5.0);
+    p = fma_k(r, p, -0.25);
+    p = fma_k(r, p, 1.0 / 3.0);
     p = fma(r, p, -0.5);
     const double ed = (double)(e - 1);
     const double lo = fma(r * r, p, ed * ln2_lo);
@@ -231,7 +246,7 @@ __device__ __forceinline__ void node_accumulate(const double* tab, double q, dou
     const double b = fma(-E, invT, -muT);
 
     // quark: (c1, c2) = (Phi, Phib), t = e^a; antiquark: (Phib, Phi), e^b
-    const double ta = exp_tab(a, tab);
+    const double ta = exp_tab<kChecked>(a, tab);
     const double tb = ta * K2mu;
     const double ta2 = ta * ta, ta3 = ta2 * ta;
     const double tb2 = tb * tb, tb3 = tb2 * tb;
