@@ -112,6 +112,134 @@ function solve_batch(T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{Float6
             n_qbar=nqb, residual_norm=res, iterations=it, converged=conv .!= 0, flags=fl)
 end
 
+# seed constants of the GPU's DefaultSeed / MultiSeed policies (SeedStrategies.jl:56-107, 257-268)
+const HADRON_SEED = [-1.84329, -1.84329, -2.22701, 1.0e-5, 4.0e-5]
+const QUARK_SEED = [-0.73192, -0.73192, -1.79539, 0.60532, 0.60532]
+const WEAK_CHIRAL_CONF_SEED = [-0.5, -0.5, -1.2, 1e-3, 1e-3]
+const HT_0P8_SEED = [-0.5, -0.5, -1.2, 0.8, 0.8]
+const HT_0P9_SEED = [-0.3, -0.3, -0.9, 0.9, 0.9]
+const HT_0P95_SEED = [-0.2, -0.2, -0.7, 0.95, 0.95]
+const MULTI_CANDIDATES = ((HADRON_SEED, QUARK_SEED, :hadron), (HADRON_SEED, QUARK_SEED, :quark),
+                          (WEAK_CHIRAL_CONF_SEED, WEAK_CHIRAL_CONF_SEED, :hadron),
+                          (HT_0P8_SEED, HT_0P8_SEED, :hadron), (HT_0P9_SEED, HT_0P9_SEED, :hadron),
+                          (HT_0P95_SEED, HT_0P95_SEED, :hadron))
+
+# MultiSeed() with its default candidates and Omega selector (the GPU's SEED_MULTI)
+is_default_multiseed(s) = length(s.candidates) == 6 && nameof(s.selector) === :default_omega_selector &&
+    all(nameof(typeof(c)) === :DefaultSeed && c.hadron_seed == h && c.quark_seed == q && c.phase_hint === hint
+        for (c, (h, q, hint)) in zip(s.candidates, MULTI_CANDIDATES))
+
+"""
+Fields and order of the reference's `SolverResult` (ImplicitSolver.jl:178-193);
+`x_state`, `mu_vec` and `masses` are tuples here (the reference's SVectors
+convert from them, so `make_result=SolverResult` builds the reference's own type).
+"""
+struct SolverResult
+    mode::Any
+    converged::Bool
+    solution::Vector{Float64}
+    x_state::NTuple{5,Float64}
+    mu_vec::NTuple{3,Float64}
+    omega::Float64
+    pressure::Float64
+    rho_norm::Float64
+    entropy::Float64
+    energy::Float64
+    masses::NTuple{3,Float64}
+    iterations::Int
+    residual_norm::Float64
+    xi::Float64
+end
+
+# policy and explicit seed of a seed strategy, by the reference's type names
+# and fields (SeedStrategies.jl:193-888); strategies with host-side state the
+# GPU does not model (phase-aware seeds, custom seed vectors) take their seed
+# from `get_seed` (the reference's get_seed(strategy, [T, mu], mode)) as an
+# explicit seed
+function policy_for(s, T_fm, mu_fm, mode, get_seed)
+    s === nothing && return SEED_DEFAULT, nothing
+    t = nameof(typeof(s))
+    if t === :DefaultSeed && s.hadron_seed == HADRON_SEED && s.quark_seed == QUARK_SEED
+        return (s.phase_hint === :hadron ? SEED_HADRON : s.phase_hint === :quark ? SEED_QUARK : SEED_DEFAULT), nothing
+    elseif t === :MultiSeed
+        is_default_multiseed(s) && return SEED_MULTI, nothing
+        throw(ArgumentError("MultiSeed: the HIP backend runs the default candidates and Omega selector only"))
+    elseif t === :ContinuitySeed
+        s.previous_solution === nothing && return policy_for(s.fallback, T_fm, mu_fm, mode, get_seed)
+        return SEED_EXPLICIT, Float64.(s.previous_solution[1:5])
+    elseif t === :PhaseAwareContinuitySeed && s.bootstrap_multiseed && s.previous_solution === nothing
+        return SEED_MULTI, nothing
+    end
+    get_seed === nothing && throw(ArgumentError(
+        "seed strategy $t: pass get_seed (the reference's get_seed) so its seed can be used as an explicit seed"))
+    return SEED_EXPLICIT, Float64.(get_seed(s, [Float64(T_fm), Float64(mu_fm)], mode)[1:5])
+end
+
+"""
+    solve(mode, T_fm, mu_fm; xi=0.0, seed_strategy=nothing, p_num=64, t_num=8, iterations=1000,
+          trust_region_fallback=true, auto_multiseed_fallback=true, residual_norm_max=1e-6,
+          xtol=1e-9, ftol=1e-9, get_seed=nothing, make_result=SolverResult) -> SolverResult
+
+`PNJL.solve(::FixedMu, T_fm, μ_fm; ...)` (ImplicitSolver.jl:211-328) on the GPU,
+with the reference's keywords and result.  Non-convergence is `converged = false`;
+a non-finite residual at the seed throws `DomainError` (the reference's
+NLsolve IsFiniteException) and MultiSeed without a converged candidate throws
+the reference's "All seeds failed" error (ImplicitSolver.jl:553-556).
+The reference's solve switches with one keyword:
+
+    backend === :hip && return PNJLHip.solve(mode, T_fm, μ_fm; make_result=SolverResult,
+                                             get_seed=get_seed, kwargs...)
+"""
+function solve(mode, T_fm::Real, mu_fm::Real; xi::Real=0.0, seed_strategy=nothing, p_num::Integer=64,
+               t_num::Integer=8, iterations::Integer=1000, trust_region_fallback::Bool=true,
+               auto_multiseed_fallback::Bool=true, residual_norm_max::Real=1e-6, xtol::Real=1e-9,
+               ftol::Real=1e-9, nlsolve_method::Symbol=:newton, fallback_method::Symbol=:trust_region,
+               physicality_check=nothing, get_seed=nothing, make_result=SolverResult,
+               ctx::Context=default_context())
+    nameof(typeof(mode)) === :FixedMu || throw(ArgumentError("the HIP backend solves FixedMu() only"))
+    nlsolve_method === :newton || throw(ArgumentError("the HIP backend runs nlsolve_method=:newton"))
+    fallback_method === :trust_region || throw(ArgumentError("the HIP backend falls back to :trust_region"))
+    physicality_check === nothing || nameof(physicality_check) === :_default_is_physical_solution ||
+        throw(ArgumentError("the HIP backend applies the default physicality check"))
+    policy, x0 = policy_for(seed_strategy, T_fm, mu_fm, mode, get_seed)
+    r = solve_batch([Float64(T_fm)], [Float64(mu_fm)]; xi=[Float64(xi)], policy=policy,
+                    seeds=x0 === nothing ? nothing : reshape(x0, 5, 1), p_num=p_num, t_num=t_num,
+                    iterations=iterations, trust_region_fallback=trust_region_fallback,
+                    auto_multiseed_fallback=auto_multiseed_fallback, residual_norm_max=Float64(residual_norm_max),
+                    xtol=Float64(xtol), ftol=Float64(ftol), ctx=ctx)
+    fl = r.flags[1]
+    fl & PT_EXCEPTION != 0 && throw(DomainError(T_fm, "non-finite residual at the seed (NLsolve IsFiniteException)"))
+    fl & PT_NO_ROOT != 0 && error("All seeds failed to converge to a physical solution")
+    x = r.solution[:, 1]
+    mu = Float64(mu_fm)
+    return make_result(mode, r.converged[1], copy(x), Tuple(x), (mu, mu, mu), r.omega[1], r.pressure[1],
+                       r.rho_norm[1], r.entropy[1], r.energy[1], Tuple(r.masses[:, 1]), Int(r.iterations[1]),
+                       r.residual_norm[1], Float64(xi))
+end
+
+"""
+    solve_many(mode, T_fm, mu_fm; xi, seed_strategy=nothing, ..., make_result=SolverResult)
+
+`solve` over arrays of points in one launch (independent points; DefaultSeed
+or MultiSeed strategies): a Vector of results.  Scans that carry a
+continuation seed from point to point run as chains with `solve_batch`
+(policy SEED_CONTINUITY / SEED_PHASE_AWARE, `chain_len`).
+"""
+function solve_many(mode, T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{Float64};
+                    xi::AbstractVector{Float64}=zeros(length(T_fm)), seed_strategy=nothing, p_num::Integer=64,
+                    t_num::Integer=8, iterations::Integer=1000, make_result=SolverResult,
+                    ctx::Context=default_context(), kwargs...)
+    nameof(typeof(mode)) === :FixedMu || throw(ArgumentError("the HIP backend solves FixedMu() only"))
+    policy, x0 = policy_for(seed_strategy, T_fm[1], mu_fm[1], mode, nothing)
+    x0 === nothing || throw(ArgumentError("solve_many: point-independent strategies only"))
+    r = solve_batch(T_fm, mu_fm; xi=xi, policy=policy, p_num=p_num, t_num=t_num, iterations=iterations,
+                    ctx=ctx, kwargs...)
+    return [make_result(mode, r.converged[i], r.solution[:, i], Tuple(r.solution[:, i]),
+                        (mu_fm[i], mu_fm[i], mu_fm[i]), r.omega[i], r.pressure[i], r.rho_norm[i], r.entropy[i],
+                        r.energy[i], Tuple(r.masses[:, i]), Int(r.iterations[i]), r.residual_norm[i], xi[i])
+            for i in eachindex(T_fm)]
+end
+
 """
     thermo_derivatives_batch(T_fm, mu_fm, x; xi, p_num=64, t_num=8)
 
@@ -223,12 +351,17 @@ function solve_meson_mass(meson::Symbol, quark_params, thermo_params; k_norm::Fl
 end
 
 """
-    polarization_aniso(channel, k0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi, A1, A2, num_s)
+    polarization_aniso(channel, k0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi, A1, A2, num_s; backend=:hip)
 
-Same signature and meaning as the reference (channel :P or :S); computed on the GPU.
+Same signature and result (polarization_real, polarization_imag) as the
+reference (PolarizationAniso.jl:22-24; channel :P or :S, ArgumentError
+otherwise); computed on the GPU.  The reference's function switches with one
+keyword: `backend === :hip && return PNJLHip.polarization_aniso(args...)`.
 """
 function polarization_aniso(channel::Symbol, k0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi,
-                            A1_value, A2_value, num_s_quark::Int; ctx::Context=default_context())
+                            A1_value, A2_value, num_s_quark::Int; backend::Symbol=:hip,
+                            ctx::Context=default_context())
+    backend === :hip || throw(ArgumentError("PNJLHip.polarization_aniso computes backend=:hip only"))
     ch = channel === :P ? CHANNEL_P : channel === :S ? CHANNEL_S :
          throw(ArgumentError("Unsupported channel: $channel. Use :P or :S."))
     q = Ref(PolQuery(ch, num_s_quark, 0, 0, k0, 0.0, k_norm, m1, m2, mu1, mu2, T, Phi, Phibar, xi,
