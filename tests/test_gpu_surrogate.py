@@ -272,3 +272,65 @@ def test_c4_omega_judge_keeps_the_stable_root(gpu_ctx, oracle, c4_model):
     assert lower.sum() > 1000          # the plain solver's metastable roots are replaced
     # mostly through the judge (a learned root may also sit on the stable branch by itself)
     assert np.mean((Q["flags"][lower] & (_lib.PT_SURR_OMEGA | _lib.PT_FIRST_ORDER)) != 0) > 0.9
+
+
+def test_c5_multiroot3_with_learned_seeds_matches_oracle(gpu_ctx, oracle, c4_model):
+    """C5 grid (512x512x5: T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2 step
+    0.1) at p = 64, t = 4 with the learned seeds: MULTIROOT3 (learned, HADRON
+    and QUARK Newton solves, minimum-Omega root, first-order flag) over the
+    whole grid on the GPU, checked on a seeded sample of 1500 points plus up
+    to 500 first-order points against the oracle's MULTIROOT3 with the same
+    seeds (selection rule of solve_multi, ImplicitSolver.jl:532-559)."""
+    import torch
+    from julia_relaxtime_amd import _lib
+    from test_gpu_gap import assert_parity, oracle_rerun
+    s, _ = c4_model
+    g = np.meshgrid(np.linspace(10, 250, 512), np.linspace(200, 450, 512), [-0.2, -0.1, 0.0, 0.1, 0.2],
+                    indexing="ij")
+    T, mu, xi = g[0].ravel() / HB, g[1].ravel() / HB, g[2].ravel()
+    n = T.size
+    dev = torch.device("cuda", 0)
+    Td, mud, xid = (torch.tensor(a, device=dev) for a in (T, mu, xi))
+    opts = _lib.default_opts(ftol=1e-10, max_iter=40)
+    seeds = s.infer(Td, mud, xid)
+    Q = {k: v.cpu().numpy() for k, v in gpu_ctx.solve_batch(Td, mud, xid, policy=_lib.SEED_MULTIROOT3, seed=seeds,
+                                                              p_num=64, t_num=4, opts=opts).items()}
+    first = np.flatnonzero((Q["flags"] & _lib.PT_FIRST_ORDER) != 0)
+    print(f"[c5] {n} points: {int((Q['converged'] == 1).sum())} converged, {first.size} first-order "
+          f"(two distinct physical roots)", flush=True)
+    assert (Q["converged"] == 1).mean() > 0.99
+    assert first.size > 0
+    rng = np.random.default_rng(11)
+    idx = np.unique(np.r_[rng.choice(n, 1500, replace=False), rng.choice(first, min(500, first.size), replace=False)])
+    sd = seeds.cpu().numpy()
+    oo = oracle.default_opts(ftol=1e-10, max_iter=40)
+    o = oracle.solve_batch(T[idx], mu[idx], xi[idx], policy=_lib.SEED_MULTIROOT3, seed=sd[idx], p_num=64, t_num=4,
+                           nthreads=8, opts=oo)
+    assert_parity({k: v[idx] for k, v in Q.items()}, o, T[idx], mu[idx],
+                  oracle_rerun(oracle, xi[idx], _lib.SEED_MULTIROOT3, 64, 4, oo, seed=sd[idx]), selection=True,
+                  label="c5 sample")
+    # first-order flag (two distinct physical roots among the three runs): it
+    # differs only where one of the runs ends differently (a slow run that
+    # meets ftol at the 40-iteration cap in one arithmetic and not in the
+    # other, a trajectory that wanders).  A disagreement is allowed only where
+    # the flag itself is rounding-sensitive: the oracle's or the GPU's own
+    # flag changes when T or mu moves by one ulp (same seeds).
+    fo = (Q["flags"][idx] & _lib.PT_FIRST_ORDER) != 0
+    ofo = (o["flags"] & _lib.PT_FIRST_ORDER) != 0
+    dis = np.flatnonzero(fo != ofo)
+    u_or, u_gpu = np.zeros(dis.size, dtype=bool), np.zeros(dis.size, dtype=bool)
+    run = oracle_rerun(oracle, xi[idx], _lib.SEED_MULTIROOT3, 64, 4, oo, seed=sd[idx])
+    Ts, ms, xs, ss = T[idx], mu[idx], xi[idx], sd[idx]
+    for dT, dm in ((np.inf, None), (-np.inf, None), (None, np.inf), (None, -np.inf)):
+        Tp = Ts[dis] if dT is None else np.nextafter(Ts[dis], dT)
+        mp = ms[dis] if dm is None else np.nextafter(ms[dis], dm)
+        q = run(Tp, mp, dis)
+        u_or |= ((q["flags"] & _lib.PT_FIRST_ORDER) != 0) != ofo[dis]
+        g = gpu_ctx.solve_batch(Tp, mp, xs[dis], policy=_lib.SEED_MULTIROOT3, seed=ss[dis], p_num=64, t_num=4,
+                                opts=opts)
+        u_gpu |= ((g["flags"] & _lib.PT_FIRST_ORDER) != 0) != fo[dis]
+    unstable = u_or | u_gpu
+    print(f"[c5 sample] first-order flags agree at {idx.size - dis.size} of {idx.size}; of the {dis.size} "
+          f"disagreements the flag moves under one ulp of T or mu at {int(unstable.sum())} (oracle {int(u_or.sum())}, "
+          f"GPU {int(u_gpu.sum())})", flush=True)
+    assert unstable.all(), [(float(Ts[i] * HB), float(ms[i] * HB), float(xs[i])) for i in dis[~unstable][:10]]
