@@ -254,6 +254,28 @@ int pnjl_thermo_derivatives_batch(pnjl_ctx* ctx, int64_t n,
                                   const double* x, int p_num, int t_num,
                                   double* out, int flags, void* stream);
 
+/* Second-order implicit derivatives at given roots x[n*5] (solve first; xi
+ * and the node counts as in the solve): out[n*48], per point --
+ *   0-4 x, 5-9 dx_dT, 10-14 dx_dmu, 15-19 d2x_dT2, 20-24 d2x_dTdmu,
+ *   25-29 d2x_dmu2, 30-32 masses (u, d, s; bare masses 5.5 and 140.0 MeV as
+ *   the reference's derivative module writes them), 33-35 dM_dT,
+ *   36-38 dM_dmu, 39-41 d2M_dT2, 42-44 d2M_dTdmu, 45-47 d2M_dmu2.
+ * Along the root x(T, mu), differentiating H x_a + F_a = 0 (H = d2 Omega/dx2,
+ * F_a = d2 Omega/dx dtheta_a) once more along w_b = (x_b, e_b) gives
+ * x_ab = -H^-1 [(D_wb H) x_a + D_wb F_a]; the directional derivatives are the
+ * closed forms of H and F_a evaluated in dual-number arithmetic, exact to
+ * rounding like the reference's nested forward-mode AD.  All-NaN row where
+ * H is singular.  Host pointers unless PNJL_FLAG_DEVICE_PTRS.
+ * replaces: solve_with_derivatives (order 1, 2)
+ *             src/pnjl/solver/ImplicitSolver.jl:660-701
+ *           mass_derivatives (order 2)
+ *             src/pnjl/derivatives/ThermoDerivatives.jl:153-172            */
+#define PNJL_IMPLICIT2_FIELDS 48
+int pnjl_implicit_derivatives2_batch(pnjl_ctx* ctx, int64_t n,
+                                     const double* T, const double* mu, const double* xi,
+                                     const double* x, int p_num, int t_num,
+                                     double* out, int flags, void* stream);
+
 /* Residual F = grad_x P (5), Jacobian J = dF/dx (5x5, row-major) and Omega at
  * given states -- the building block of the Newton loop.
  * replaces: gap_conditions  src/pnjl/solver/Conditions.jl:72-81 and the

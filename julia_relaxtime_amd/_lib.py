@@ -30,6 +30,10 @@ PT_SURR_FALLBACK, PT_SURR_OMEGA, PT_FIRST_ORDER, PT_CHAIN_RESET = 0x10, 0x20, 0x
 PT_SURR_MASKED = 0x100
 PT_BRANCH_END, PT_SKIPPED = 0x200, 0x400
 
+# pnjl_implicit_derivatives2_batch: per-point output blocks (name, width)
+IMPLICIT2_LAYOUT = [("x", 5), ("dx_dT", 5), ("dx_dmu", 5), ("d2x_dT2", 5), ("d2x_dTdmu", 5), ("d2x_dmu2", 5),
+                    ("masses", 3), ("dM_dT", 3), ("dM_dmu", 3), ("d2M_dT2", 3), ("d2M_dTdmu", 3), ("d2M_dmu2", 3)]
+IMPLICIT2_FIELDS = sum(w for _, w in IMPLICIT2_LAYOUT)
 # pnjl_thermo_derivatives_batch: per-point output columns (name, width)
 THERMO_DERIV_FIELDS = 36
 THERMO_DERIV_LAYOUT = [("pressure", 1), ("energy", 1), ("rho", 1), ("rho_norm", 1), ("entropy", 1), ("dP_dT", 1),
@@ -138,12 +142,14 @@ def load():
     L.pnjl_gap_residual_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp,
                                           C.c_int, vp]
     L.pnjl_thermo_derivatives_batch.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_int, C.c_int, vp, C.c_int, vp]
+    L.pnjl_implicit_derivatives2_batch.argtypes = L.pnjl_thermo_derivatives_batch.argtypes
     L.pnjl_autotune.argtypes = [vp, C.c_int64, _DP, _DP, _DP, _DP, C.c_int, C.c_int, C.c_int,
                                 C.POINTER(SolveOpts), C.POINTER(LaunchCfg), C.POINTER(C.c_double)]
     for name in ("pnjl_ctx_create", "pnjl_gauleg", "pnjl_nodes", "pnjl_set_phase_boundary", "pnjl_set_judge_mask",
                  "pnjl_set_launch_cfg", "pnjl_get_launch_cfg", "pnjl_solve_fixedmu_batch",
                  "pnjl_solve_fixedmu_batch_noalloc",
-                 "pnjl_gap_residual_batch", "pnjl_thermo_derivatives_batch", "pnjl_autotune"):
+                 "pnjl_gap_residual_batch", "pnjl_thermo_derivatives_batch", "pnjl_implicit_derivatives2_batch",
+                 "pnjl_autotune"):
         getattr(L, name).restype = C.c_int
     _register_extra(L)
     _lib = L
@@ -348,6 +354,22 @@ class Context:
                                                    x.ctypes.data, p_num, t_num, out.ctypes.data, 0, None)
         self._check(rc, "pnjl_thermo_derivatives_batch")
         return unpack_thermo_derivs(out)
+
+    def implicit_derivatives2_batch(self, x, T, mu, xi, p_num=64, t_num=8):
+        """First and second implicit derivatives of the roots x (n, 5) in (T, mu)
+        and of the masses -> dict of the IMPLICIT2_LAYOUT blocks."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 5)
+        n = x.shape[0]
+        T, mu, xi = (np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (T, mu, xi))
+        out = np.zeros((n, IMPLICIT2_FIELDS))
+        rc = self._L.pnjl_implicit_derivatives2_batch(self.h, n, T.ctypes.data, mu.ctypes.data, xi.ctypes.data,
+                                                      x.ctypes.data, p_num, t_num, out.ctypes.data, 0, None)
+        self._check(rc, "pnjl_implicit_derivatives2_batch")
+        d, o = {}, 0
+        for name, w in IMPLICIT2_LAYOUT:
+            d[name] = out[:, o:o + w].copy()
+            o += w
+        return d
 
     def autotune(self, T, mu, xi, policy=SEED_DEFAULT, seed=None, p_num=64, t_num=8, opts=None, sample=None):
         T = np.ascontiguousarray(T, dtype=np.float64).ravel()

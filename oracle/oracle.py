@@ -95,6 +95,8 @@ def lib():
         L.oracle_thermo.argtypes = [_DP, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _DP]
         L.oracle_thermo_derivs.argtypes = [_DP, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _DP]
         L.oracle_thermo_derivs.restype = C.c_int
+        L.oracle_implicit_derivs2.argtypes = [_DP, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _DP]
+        L.oracle_implicit_derivs2.restype = C.c_int
         L.oracle_set_boundary.argtypes = [C.c_int, _DP, _DP, _DP, C.c_int, _DP, _DP, _DP]
         L.oracle_interp_mu_c.argtypes = [C.c_double, C.c_double]
         L.oracle_interp_mu_c.restype = C.c_double
@@ -215,6 +217,30 @@ def thermo_derivs(x, T, mu, xi=0.0, p_num=64, t_num=8):
     out = np.zeros(36)
     lib().oracle_thermo_derivs(_p(x), T, mu, xi, p_num, t_num, _p(out))
     return unpack_thermo_derivs(out)
+
+
+# field layout of oracle_implicit_derivs2 / pnjl_implicit_derivatives2_batch
+IMPLICIT2_LAYOUT = [("x", 5), ("dx_dT", 5), ("dx_dmu", 5), ("d2x_dT2", 5), ("d2x_dTdmu", 5), ("d2x_dmu2", 5),
+                    ("masses", 3), ("dM_dT", 3), ("dM_dmu", 3), ("d2M_dT2", 3), ("d2M_dTdmu", 3), ("d2M_dmu2", 3)]
+
+
+def unpack_implicit2(out):
+    """dict of the named blocks of a 48-value row (or an (n, 48) array)"""
+    out = np.asarray(out)
+    d, o = {}, 0
+    for name, w in IMPLICIT2_LAYOUT:
+        d[name] = out[..., o:o + w]
+        o += w
+    return d
+
+
+def implicit_derivs2(x, T, mu, xi=0.0, p_num=64, t_num=8):
+    """Second-order implicit derivatives at the root x (oracle_implicit_derivs2):
+    x_ab = -H^-1 [(D_wb H) x_a + D_wb F_a] by complex step on the closed forms."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(48)
+    lib().oracle_implicit_derivs2(_p(x), T, mu, xi, p_num, t_num, _p(out))
+    return unpack_implicit2(out)
 
 
 def alloc_out(n):

@@ -1802,3 +1802,278 @@ int oracle_thermo_derivs(const double* x, double T, double mu, double xi, int p_
     out[35] = ds_dmu;
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* second-order implicit derivatives                                         */
+/* solve_with_derivatives(order=2)  src/pnjl/solver/ImplicitSolver.jl:660-701 */
+/* mass_derivatives(order=2)  src/pnjl/derivatives/ThermoDerivatives.jl:153-172*/
+/* Along the root x(theta), theta = (T, mu), F(x(theta), theta) = 0 with      */
+/* F = grad_x Omega.  Differentiating H x_a + F_a = 0 (H = dF/dx, F_a =       */
+/* dF/dtheta_a) once more along w_b = (x_b, e_b):                             */
+/*   x_ab = -H^-1 [ (D_wb H) x_a + D_wb F_a ].                                */
+/* The directional derivatives D_w of the closed-form H and F_a come from the */
+/* complex-step method (Squire and Trapp, SIAM Review 40(1), 1998): the same  */
+/* closed forms evaluated in complex arithmetic at z + i h w give D_w f =     */
+/* Im f / h with no subtraction (h = 1e-20), so the result is exact to        */
+/* rounding like the reference's nested forward-mode AD.  Branches (the       */
+/* log-argument floor, the sign of a, |M|) follow the real part.              */
+/* ------------------------------------------------------------------------- */
+#include <complex.h>
+
+typedef double complex cplx;
+
+typedef struct {
+    cplx a, c1, c2, aa, ac1, ac2, c1c1, c1c2, c2c2;
+} lnF_c;
+
+static void lnF_eval_c(cplx a, cplx c1, cplx c2, lnF_c* o)
+{
+    cplx D, r1, r2, r3;
+    memset(o, 0, sizeof(*o));
+    if (creal(a) > 0) {
+        cplx e_a_m = cexp(-2.0 * a), e_2a_m = cexp(-a), e_neg_m = cexp(-3.0 * a);
+        D = e_neg_m + 3.0 * c1 * e_a_m + 3.0 * c2 * e_2a_m + 1.0;
+        if (creal(D) < POLYAKOV_EPS) {
+            o->a = 3.0;
+            return;
+        }
+        r1 = e_a_m / D;
+        r2 = e_2a_m / D;
+        r3 = 1.0 / D;
+    } else {
+        cplx ea = cexp(a), e2 = ea * ea, e3 = ea * e2;
+        D = 1.0 + 3.0 * c1 * ea + 3.0 * c2 * e2 + e3;
+        if (creal(D) < POLYAKOV_EPS) return;
+        r1 = ea / D;
+        r2 = e2 / D;
+        r3 = e3 / D;
+    }
+    cplx f = c1 * r1 + 2.0 * c2 * r2 + r3;
+    o->a = 3.0 * f;
+    o->c1 = 3.0 * r1;
+    o->c2 = 3.0 * r2;
+    o->aa = 3.0 * (c1 * r1 + 4.0 * c2 * r2 + 3.0 * r3) - 9.0 * f * f;
+    o->ac1 = 3.0 * r1 * (1.0 - 3.0 * f);
+    o->ac2 = 3.0 * r2 * (2.0 - 3.0 * f);
+    o->c1c1 = -9.0 * r1 * r1;
+    o->c1c2 = -9.0 * r1 * r2;
+    o->c2c2 = -9.0 * r2 * r2;
+}
+
+/* node sums of one flavour that H and dF/dtheta need */
+typedef struct {
+    cplx SM, SP, SPb, SMM, SMP, SMPb, SPP, SPPb, SPbPb, SMmu, SMT, SPmu, SPbmu, SPT, SPbT;
+} flav_c;
+
+static void flavour_sums_c(const nodes_t* nd, cplx M, cplx mu, cplx T, cplx Phi, cplx Phib, double xi, flav_c* o)
+{
+    memset(o, 0, sizeof(*o));
+    cplx invT = 1.0 / T, invT2 = invT * invT;
+    for (int idx = 0; idx < nd->n; idx++) {
+        double p = nd->p[idx], t = nd->c[idx], C = nd->C[idx];
+        double pt = p * t;
+        cplx E = csqrt(p * p + M * M + xi * (pt * pt));
+        cplx a = -(E - mu) * invT, b = -(E + mu) * invT;
+        lnF_c qp, qm;
+        lnF_eval_c(a, Phi, Phib, &qp);
+        lnF_eval_c(b, Phib, Phi, &qm);
+        cplx QE = -(qp.a + qm.a) * invT, QEE = (qp.aa + qm.aa) * invT2;
+        cplx QEP = -(qp.ac1 + qm.ac2) * invT, QEPb = -(qp.ac2 + qm.ac1) * invT;
+        cplx EM = M / E, EMM = (1.0 - EM * EM) / E;
+        o->SM += C * (QE * EM);
+        o->SP += C * (qp.c1 + qm.c2);
+        o->SPb += C * (qp.c2 + qm.c1);
+        o->SMM += C * (QEE * EM * EM + QE * EMM);
+        o->SMP += C * (QEP * EM);
+        o->SMPb += C * (QEPb * EM);
+        o->SPP += C * (qp.c1c1 + qm.c2c2);
+        o->SPPb += C * (qp.c1c2 + qm.c1c2);
+        o->SPbPb += C * (qp.c2c2 + qm.c1c1);
+        o->SMmu += C * (-(qp.aa - qm.aa) * invT2) * EM;
+        o->SMT += C * ((qp.a + qm.a + qp.aa * a + qm.aa * b) * invT2) * EM;
+        o->SPmu += C * (qp.ac1 - qm.ac2) * invT;
+        o->SPbmu += C * (qp.ac2 - qm.ac1) * invT;
+        o->SPT += C * (-(qp.ac1 * a + qm.ac2 * b) * invT);
+        o->SPbT += C * (-(qp.ac2 * a + qm.ac1 * b) * invT);
+    }
+}
+
+/* dI/dM and d2I/dM2 of the vacuum term (vacuum_I) */
+static void vacuum_I_c(cplx M, cplx* dI, cplx* d2I)
+{
+    double L = P.Lambda;
+    double sgn = (creal(M) < 0) ? -1.0 : 1.0;
+    cplx Mt = sgn * M + 1e-12;
+    cplx s = csqrt(L * L + Mt * Mt);
+    cplx lg = clog((L + s) / Mt);
+    *dI = sgn * Mt / (4.0 * PI_D * PI_D) * (L * s - Mt * Mt * lg);
+    *d2I = (2.0 * L * Mt * Mt / s + L * s - 3.0 * Mt * Mt * lg) / (4.0 * PI_D * PI_D);
+}
+
+/* H = d2 Omega/dx2 and F_T, F_mu = d2 Omega/dx dT, d2 Omega/dx dmu at complex
+ * (x, T, mu): the formulas of eval_point, polyakov_U, polyakov_U_T and
+ * oracle_thermo_derivs. */
+static void hess_theta_c(const nodes_t* nd, const cplx* x, cplx T, cplx mu, double xi, cplx H[5][5], cplx FT[5],
+                         cplx Fmu[5])
+{
+    double G = P.G, K = P.K;
+    cplx pu = x[0], pd = x[1], ps = x[2], Phi = x[3], Phib = x[4];
+    cplx M[3] = {P.m_ud0 - 4.0 * G * pu + 2.0 * K * pd * ps, P.m_ud0 - 4.0 * G * pd + 2.0 * K * pu * ps,
+                 P.m_s0 - 4.0 * G * ps + 2.0 * K * pu * pd};
+    cplx dM[3][3] = {{-4.0 * G, 2.0 * K * ps, 2.0 * K * pd},
+                     {2.0 * K * ps, -4.0 * G, 2.0 * K * pu},
+                     {2.0 * K * pd, 2.0 * K * pu, -4.0 * G}};
+    cplx d2chi[3][3] = {{4.0 * G, -4.0 * K * ps, -4.0 * K * pd},
+                        {-4.0 * K * ps, 4.0 * G, -4.0 * K * pu},
+                        {-4.0 * K * pd, -4.0 * K * pu, 4.0 * G}};
+    flav_c f[3];
+    cplx gM[3], hM[3];
+    for (int i = 0; i < 3; i++) {
+        flavour_sums_c(nd, M[i], mu, T, Phi, Phib, xi, &f[i]);
+        cplx dI, d2I;
+        vacuum_I_c(M[i], &dI, &d2I);
+        gM[i] = -2.0 * P.N_color * dI - 2.0 * T * f[i].SM;
+        hM[i] = -2.0 * P.N_color * d2I - 2.0 * T * f[i].SMM;
+    }
+    for (int a = 0; a < 3; a++) {
+        for (int b = 0; b < 3; b++) {
+            cplx h = d2chi[a][b];
+            for (int i = 0; i < 3; i++) {
+                double d2 = (a != b && a != i && b != i) ? 2.0 * K : 0.0;   /* d2M_i / dphi_a dphi_b */
+                h += hM[i] * dM[i][a] * dM[i][b] + gM[i] * d2;
+            }
+            H[a][b] = h;
+        }
+        cplx hP = 0.0, hPb = 0.0;
+        for (int i = 0; i < 3; i++) {
+            hP += -2.0 * T * f[i].SMP * dM[i][a];
+            hPb += -2.0 * T * f[i].SMPb * dM[i][a];
+        }
+        H[a][3] = H[3][a] = hP;
+        H[a][4] = H[4][a] = hPb;
+    }
+    /* Polyakov potential: second derivatives in the loops and mixed ones with T */
+    cplx Tr = P.T0 / T;
+    cplx Ta = P.a0 + P.a1 * Tr + P.a2 * Tr * Tr, Tb = P.b3 * Tr * Tr * Tr;
+    cplx val = 1.0 - 6.0 * Phib * Phi + 4.0 * (Phib * Phib * Phib + Phi * Phi * Phi) - 3.0 * (Phib * Phi) * (Phib * Phi);
+    int floored = (creal(val) <= 0.0) || (creal(val) < POLYAKOV_EPS);
+    cplx lP = 0, lPb = 0, lPP = 0, lPPb = 0, lPbPb = 0;
+    if (!floored) {
+        lP = (-6.0 * Phib + 12.0 * Phi * Phi - 6.0 * Phi * Phib * Phib) / val;
+        lPb = (-6.0 * Phi + 12.0 * Phib * Phib - 6.0 * Phi * Phi * Phib) / val;
+        lPP = (24.0 * Phi - 6.0 * Phib * Phib) / val - lP * lP;
+        lPPb = (-6.0 - 12.0 * Phi * Phib) / val - lP * lPb;
+        lPbPb = (24.0 * Phib - 6.0 * Phi * Phi) / val - lPb * lPb;
+    }
+    cplx T4 = T * T * T * T;
+    cplx SPP = 0, SPPb = 0, SPbPb = 0;
+    for (int i = 0; i < 3; i++) {
+        SPP += f[i].SPP;
+        SPPb += f[i].SPPb;
+        SPbPb += f[i].SPbPb;
+    }
+    H[3][3] = T4 * (Tb * lPP) - 2.0 * T * SPP;
+    H[3][4] = H[4][3] = T4 * (-0.5 * Ta + Tb * lPPb) - 2.0 * T * SPPb;
+    H[4][4] = T4 * (Tb * lPbPb) - 2.0 * T * SPbPb;
+    double T0 = P.T0;
+    cplx A1 = 4.0 * P.a0 * T * T * T + 3.0 * P.a1 * T0 * T * T + 2.0 * P.a2 * T0 * T0 * T;
+    double B1 = P.b3 * T0 * T0 * T0;
+    for (int a = 0; a < 5; a++) FT[a] = Fmu[a] = 0.0;
+    for (int i = 0; i < 3; i++) {
+        cplx LMmu = -2.0 * T * f[i].SMmu, LMT = -2.0 * f[i].SM - 2.0 * T * f[i].SMT;
+        for (int a = 0; a < 3; a++) {
+            Fmu[a] += dM[i][a] * LMmu;
+            FT[a] += dM[i][a] * LMT;
+        }
+        Fmu[3] += -2.0 * T * f[i].SPmu;
+        Fmu[4] += -2.0 * T * f[i].SPbmu;
+        FT[3] += -2.0 * f[i].SP - 2.0 * T * f[i].SPT;
+        FT[4] += -2.0 * f[i].SPb - 2.0 * T * f[i].SPbT;
+    }
+    FT[3] += -0.5 * Phib * A1 + B1 * lP;
+    FT[4] += -0.5 * Phi * A1 + B1 * lPb;
+}
+
+/* out[PNJL_IMPLICIT2_FIELDS]: x[5], dx_dT[5], dx_dmu[5], d2x_dT2[5],
+ * d2x_dTdmu[5], d2x_dmu2[5], masses[3], dM_dT[3], dM_dmu[3], d2M_dT2[3],
+ * d2M_dTdmu[3], d2M_dmu2[3] (masses with the bare 5.5 and 140.0 MeV of
+ * ThermoDerivatives.jl:111-121).  Returns 1 if dF/dx is singular (all NaN). */
+int oracle_implicit_derivs2(const double* x, double T, double mu, double xi, int p_num, int t_num, double* out)
+{
+    ensure_params();
+    const nodes_t* nd = cache_nodes(p_num, t_num);
+    double d1[36];
+    if (oracle_thermo_derivs(x, T, mu, xi, p_num, t_num, d1)) {
+        for (int k = 0; k < PNJL_IMPLICIT2_FIELDS; k++) out[k] = NAN;
+        return 1;
+    }
+    const double* xT = d1 + 24;
+    const double* xmu = d1 + 29;
+    const double h = 1e-20;
+    /* D_w of H and F_theta along w_T = (x_T, 1, 0) and w_mu = (x_mu, 0, 1) */
+    double Hr[25], dH[2][5][5], dFT[2][5], dFmu[2][5];
+    for (int dir = 0; dir < 2; dir++) {
+        const double* w = dir == 0 ? xT : xmu;
+        cplx xc[5], H[5][5], FT[5], Fmu[5];
+        for (int a = 0; a < 5; a++) xc[a] = x[a] + I * (h * w[a]);
+        cplx Tc = T + (dir == 0 ? I * h : 0.0), muc = mu + (dir == 1 ? I * h : 0.0);
+        hess_theta_c(nd, xc, Tc, muc, xi, H, FT, Fmu);
+        for (int a = 0; a < 5; a++) {
+            for (int b = 0; b < 5; b++) {
+                dH[dir][a][b] = cimag(H[a][b]) / h;
+                if (dir == 0) Hr[a * 5 + b] = creal(H[a][b]);
+            }
+            dFT[dir][a] = cimag(FT[a]) / h;
+            dFmu[dir][a] = cimag(Fmu[a]) / h;
+        }
+    }
+    /* x_TT (w_T, a = T), x_Tmu (w_mu, a = T), x_mumu (w_mu, a = mu) */
+    double x2[3][5];
+    for (int k = 0; k < 3; k++) {
+        const int dir = (k == 0) ? 0 : 1;
+        const double* xa = (k == 2) ? xmu : xT;
+        const double* Fa = (k == 2) ? dFmu[dir] : dFT[dir];
+        double rhs[5];
+        for (int a = 0; a < 5; a++) {
+            double r = Fa[a];
+            for (int b = 0; b < 5; b++) r += dH[dir][a][b] * xa[b];
+            rhs[a] = -r;
+        }
+        if (lu_solve(5, Hr, rhs, x2[k])) {
+            for (int q = 0; q < PNJL_IMPLICIT2_FIELDS; q++) out[q] = NAN;
+            return 1;
+        }
+    }
+    for (int a = 0; a < 5; a++) {
+        out[a] = x[a];
+        out[5 + a] = xT[a];
+        out[10 + a] = xmu[a];
+        out[15 + a] = x2[0][a];
+        out[20 + a] = x2[1][a];
+        out[25 + a] = x2[2][a];
+    }
+    /* masses: M_i = m_i0 - 4 G phi_i + 2 K phi_j phi_k (j, k the other two) */
+    double Mr[3];
+    deriv_masses(x, Mr);
+    const double G = P.G, K = P.K;
+    for (int i = 0; i < 3; i++) {
+        const int j = (i == 0) ? 1 : 0, k = (i == 2) ? 1 : 2;
+        double m1[2], m2[3];
+        for (int d = 0; d < 2; d++) {
+            const double* v = d == 0 ? xT : xmu;
+            m1[d] = -4.0 * G * v[i] + 2.0 * K * (v[j] * x[k] + x[j] * v[k]);
+        }
+        for (int q = 0; q < 3; q++) {
+            const double* va = (q == 2) ? xmu : xT;   /* TT: (T, T), Tmu: (T, mu), mumu: (mu, mu) */
+            const double* vb = (q == 0) ? xT : xmu;
+            m2[q] = -4.0 * G * x2[q][i] + 2.0 * K * (x2[q][j] * x[k] + x[j] * x2[q][k] + va[j] * vb[k] + vb[j] * va[k]);
+        }
+        out[30 + i] = Mr[i];
+        out[33 + i] = m1[0];
+        out[36 + i] = m1[1];
+        out[39 + i] = m2[0];
+        out[42 + i] = m2[1];
+        out[45 + i] = m2[2];
+    }
+    return 0;
+}

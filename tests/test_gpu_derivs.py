@@ -113,8 +113,16 @@ def test_host_api_is_consistent(gpu_ctx):
     assert TD.dP_dT(T, mu, order=2, p_num=48, t_num=12) == b["ds_dT"][0]
     m = TD.mass_derivatives(T, mu, p_num=48, t_num=12)
     assert np.array_equal(m["dM_dT"], d["dM_dT"])
-    with pytest.raises(NotImplementedError):
-        TD.mass_derivatives(T, mu, order=2)
+    m2 = TD.mass_derivatives(T, mu, order=2, p_num=48, t_num=12)
+    assert np.allclose(m2["dM_dT"], d["dM_dT"], rtol=1e-10) and np.allclose(m2["masses"], d["masses"], rtol=1e-14)
+    assert set(m2) == {"masses", "dM_dT", "dM_dmu", "d2M_dT2", "d2M_dTdmu", "d2M_dmu2"}
+    s2 = TD.solve_with_derivatives(T, mu, order=2, p_num=48, t_num=12)
+    assert set(s2) == {"x", "dx_dT", "dx_dmu", "d2x_dT2", "d2x_dmu2", "d2x_dTdmu"}
+    assert set(TD.solve_with_derivatives(T, mu, p_num=48, t_num=12)) == {"x", "dx_dT", "dx_dmu"}
+    with pytest.raises(ValueError):
+        TD.mass_derivatives(T, mu, order=3)
+    with pytest.raises(ValueError):
+        TD.solve_with_derivatives(T, mu, order=0)
     with pytest.raises(NotImplementedError):
         TD.dP_dT(T, mu, order=3)
 
@@ -129,3 +137,32 @@ def test_derivatives_match_reference_output(gpu_ctx):
         d = TD.thermo_derivatives(float(row["T_MeV"]) / HB, float(row["mu_MeV"]) / HB, xi=0.0, p_num=24, t_num=8)
         assert d["converged"]
         check_derivative_row(d, row)
+
+
+@pytest.mark.parametrize("pn,tn", [(24, 8), (64, 4)])
+def test_second_order_derivatives_match_oracle(gpu_ctx, oracle, pn, tn):
+    """pnjl_implicit_derivatives2_batch (dual numbers over the closed-form
+    Hessian and mixed terms) against the oracle's complex-step restatement on
+    the same roots: x and the masses with their first and second (T, mu)
+    derivatives (solve_with_derivatives order 2, ImplicitSolver.jl:660-701;
+    mass_derivatives order 2, ThermoDerivatives.jl:153-172).  1e-9 relative
+    to max(|oracle|, 1e-3 of the block's largest value); roots next to a
+    spinodal (|dx/dtheta| > 1e3: the second derivatives carry the Hessian's
+    condition twice) at 1e-6.  Three-flavour states (phi_d != phi_u) included."""
+    T, mu, xi, x = _roots(oracle, 200, 5, pn, tn)
+    x2 = x[:30].copy()
+    x2[:, 1] *= 1.01
+    T, mu, xi, x = np.r_[T, T[:30]], np.r_[mu, mu[:30]], np.r_[xi, xi[:30]], np.r_[x, x2]
+    g = gpu_ctx.implicit_derivatives2_batch(x, T, mu, xi, pn, tn)
+    o = [oracle.implicit_derivs2(x[i], T[i], mu[i], xi[i], pn, tn) for i in range(len(T))]
+    stiff = np.array([max(np.abs(oi["dx_dT"]).max(), np.abs(oi["dx_dmu"]).max()) > 1e3 for oi in o])
+    worst = {}
+    for k in g:
+        ov = np.array([oi[k] for oi in o])
+        sc = np.maximum(np.abs(ov), 1e-3 * np.abs(ov).max(axis=1, keepdims=True))
+        err = np.max(np.abs(g[k] - ov) / sc, axis=1)
+        tol = np.where(stiff, 1e-6, 1e-9)
+        worst[k] = float(err[~stiff].max()) if (~stiff).any() else 0.0
+        assert np.all(err <= tol), (k, np.flatnonzero(err > tol)[:5], err.max())
+    print(f"[deriv2 p{pn} t{tn}] {len(T)} roots ({int(stiff.sum())} next to a spinodal); worst relative "
+          f"difference per block: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()), flush=True)
