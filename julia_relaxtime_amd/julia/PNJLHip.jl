@@ -12,6 +12,7 @@ Replaces:
   PNJL.solve(::FixedMu, T_fm, mu_fm; ...)          src/pnjl/solver/ImplicitSolver.jl:211-328
   polarization_aniso(channel, k0, k_norm, ...)      src/relaxtime/PolarizationAniso.jl:22-24
   ThermoDerivatives (mass / thermo / bulk-viscosity derivatives)  ThermoDerivatives.jl:132-466
+  solve_with_derivatives(T_fm, mu_fm; order)       src/pnjl/solver/ImplicitSolver.jl:660-701
   calculate_propagator_by_channel(process, channel, ...)  src/relaxtime/TotalPropagator.jl:894-963
   solve_meson_mass(meson, quark_params, thermo_params; ...)  src/relaxtime/MesonMass.jl:241-276
 """
@@ -262,6 +263,64 @@ function thermo_derivatives_batch(T_fm::AbstractVector{Float64}, mu_fm::Abstract
         check(ctx, rc, "pnjl_thermo_derivatives_batch")
     end
     return out
+end
+
+"""
+    implicit_derivatives2_batch(T_fm, mu_fm, x; xi, p_num=64, t_num=8)
+
+x, its first and second (T, mu) derivatives and those of the masses at the
+roots `x` (5 x n): a 48 x n matrix, rows as documented for
+`pnjl_implicit_derivatives2_batch` (x, dx_dT, dx_dmu, d2x_dT2, d2x_dTdmu,
+d2x_dmu2, masses, dM_dT, dM_dmu, d2M_dT2, d2M_dTdmu, d2M_dmu2).
+"""
+function implicit_derivatives2_batch(T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{Float64},
+                                     x::Matrix{Float64}; xi::AbstractVector{Float64}=zeros(length(T_fm)),
+                                     p_num::Integer=64, t_num::Integer=8, ctx::Context=default_context())
+    n = length(T_fm)
+    out = zeros(48, n)
+    GC.@preserve T_fm mu_fm xi x out begin
+        rc = ccall((:pnjl_implicit_derivatives2_batch, LIB), Cint,
+                   (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Cint,
+                    Ptr{Float64}, Cint, Ptr{Cvoid}),
+                   ctx.h, n, T_fm, mu_fm, xi, x, p_num, t_num, out, 0, C_NULL)
+        check(ctx, rc, "pnjl_implicit_derivatives2_batch")
+    end
+    return out
+end
+
+function _derivs2(T_fm, mu_fm, xi, p_num, t_num, ctx)
+    r = solve_batch([Float64(T_fm)], [Float64(mu_fm)]; xi=[Float64(xi)], p_num=p_num, t_num=t_num, ctx=ctx)
+    return implicit_derivatives2_batch([Float64(T_fm)], [Float64(mu_fm)], r.solution; xi=[Float64(xi)],
+                                       p_num=p_num, t_num=t_num, ctx=ctx)[:, 1]
+end
+
+"""
+    solve_with_derivatives(T_fm, mu_fm; order=1, xi=0.0, p_num=64, t_num=8)
+
+Same arguments and NamedTuple as the reference (ImplicitSolver.jl:660-701):
+(x, dx_dT, dx_dμ) and, at order 2, d2x_dT2, d2x_dμ2, d2x_dTdμ.
+"""
+function solve_with_derivatives(T_fm::Real, mu_fm::Real; order::Int=1, xi::Real=0.0, p_num::Int=64,
+                                t_num::Int=8, ctx::Context=default_context())
+    order in (1, 2) || error("order must be 1 or 2, got $order")
+    d = _derivs2(T_fm, mu_fm, xi, p_num, t_num, ctx)
+    order == 1 && return (x=d[1:5], dx_dT=d[6:10], dx_dμ=d[11:15])
+    return (x=d[1:5], dx_dT=d[6:10], dx_dμ=d[11:15], d2x_dT2=d[16:20], d2x_dμ2=d[26:30], d2x_dTdμ=d[21:25])
+end
+
+"""
+    mass_derivatives(T_fm, mu_fm; order=1, xi=0.0, p_num=64, t_num=8)
+
+Same arguments and NamedTuple as the reference (ThermoDerivatives.jl:132-179):
+(masses, dM_dT, dM_dmu) and, at order 2, d2M_dT2, d2M_dTdmu, d2M_dmu2.
+"""
+function mass_derivatives(T_fm::Real, mu_fm::Real; order::Int=1, xi::Real=0.0, p_num::Int=64, t_num::Int=8,
+                          ctx::Context=default_context())
+    order in (1, 2) || error("order must be 1 or 2, got $order")
+    d = _derivs2(T_fm, mu_fm, xi, p_num, t_num, ctx)
+    order == 1 && return (masses=d[31:33], dM_dT=d[34:36], dM_dmu=d[37:39])
+    return (masses=d[31:33], dM_dT=d[34:36], dM_dmu=d[37:39], d2M_dT2=d[40:42], d2M_dTdmu=d[43:45],
+            d2M_dmu2=d[46:48])
 end
 
 # scattering processes in SCATTERING_MESON_MAP's order (PNJL_PROC_* ids 0..16)
