@@ -313,17 +313,37 @@ struct AdamArgs {
     float* loss_out;      // optional: loss of this step
 };
 
-__global__ void adam_kernel(AdamArgs A)
+// ADAM_PARAMS parameters per block; the THREADS / ADAM_PARAMS threads of a
+// parameter each sum every ADAM_SLICES-th block partial (independent loads in
+// flight), then the slices add in LDS in a fixed order
+constexpr int ADAM_PARAMS = 64, ADAM_SLICES = THREADS / ADAM_PARAMS;
+
+__global__ void __launch_bounds__(THREADS) adam_kernel(AdamArgs A)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && A.loss_out) {
+    __shared__ float red[THREADS];
+    const int lane = threadIdx.x % ADAM_PARAMS, slice = threadIdx.x / ADAM_PARAMS;
+    const int p = blockIdx.x * ADAM_PARAMS + lane;
+    if (A.loss_out && blockIdx.x == 0) {
         float l = 0.f;
-        for (int b = 0; b < A.nblocks; b++) l += A.loss_partial[b];
-        *A.loss_out = l;
+        for (int b = threadIdx.x; b < A.nblocks; b += THREADS) l += A.loss_partial[b];
+        red[threadIdx.x] = l;
+        __syncthreads();
+        for (int w = THREADS / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *A.loss_out = red[0];
+        __syncthreads();
     }
-    if (p >= NPARAM) return;
     float g = 0.f;
-    for (int b = 0; b < A.nblocks; b++) g += A.partial[(int64_t)b * NPARAM + p];
+    if (p < NPARAM) {
+#pragma unroll 8
+        for (int b = slice; b < A.nblocks; b += ADAM_SLICES) g += A.partial[(int64_t)b * NPARAM + p];
+    }
+    red[threadIdx.x] = g;
+    __syncthreads();
+    if (slice != 0 || p >= NPARAM) return;
+    g = (red[lane] + red[lane + ADAM_PARAMS]) + (red[lane + 2 * ADAM_PARAMS] + red[lane + 3 * ADAM_PARAMS]);
     if (A.grad_out) A.grad_out[p] = g;
     float w = A.P[p];
     if (A.t > 0) {
@@ -477,7 +497,8 @@ int refresh_bf16(pnjl_surrogate* s, hipStream_t st)
     a.nblocks = 0;
     a.t = 0;
     a.w = s->w;
-    hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::NPARAM + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::NPARAM + mlp::ADAM_PARAMS - 1) / mlp::ADAM_PARAMS),
+                       dim3(mlp::THREADS), 0, st, a);
     SCHK(s, hipGetLastError());
     return PNJL_OK;
 }
@@ -539,7 +560,8 @@ int step(pnjl_surrogate* s, int64_t batch, int explicit_rows, uint64_t seed, flo
     a.w = s->w;
     a.loss_partial = s->loss_partial;
     a.loss_out = loss_dev;
-    hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::NPARAM + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::NPARAM + mlp::ADAM_PARAMS - 1) / mlp::ADAM_PARAMS),
+                       dim3(mlp::THREADS), 0, st, a);
     SCHK(s, hipGetLastError());
     return PNJL_OK;
 }
