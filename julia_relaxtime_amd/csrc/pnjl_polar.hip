@@ -1254,86 +1254,237 @@ __device__ __forceinline__ double mass_guess(const pnjl_mass_query& a)
     }
 }
 
+// Trust-region state of a pole search between the launches of
+// pnjl_meson_mass_batch (every field it carries from one iteration to the next).
+struct MassState {
+    double x[2], r[2], J[2][2], d[2], delta;
+    int it, conv, stopped, pad;
+};
+
 // solve_meson_mass  MesonMass.jl:241-276 with NLsolve's trust region (dogleg,
 // autoscale, factor 1) and a central-difference Jacobian; one thread per query.
 // out[i*6] = (mass, gamma, residual_norm (inf), converged, iterations, exception)
-// stride 64 puts one query per wavefront, for counting FLOPs.
-// 3 waves per SIMD (168 VGPRs, some spilled): the launch is throughput-bound
-// on its 1000-iteration queries, and the extra waves hide the quadratures'
-// latency (1.5x over 256 VGPRs at 1 wave).
+//
+// The queries run in phases of growing iteration budgets (16, 16, 32, ...,
+// then unlimited): phase 0 starts every query; a query that is not done when
+// its phase's budget is spent saves its state and joins the list of the next
+// phase.  The searches differ by orders of magnitude in length (most stop in
+// a few dozen iterations, some run NLsolve's full 1000), so one launch would
+// hold every wavefront for its longest query; the phases keep the long ones
+// packed together.  A query's arithmetic is the same as in one uninterrupted
+// loop.  stride 64 puts one query per wavefront, for counting FLOPs.
+// 3 waves per SIMD (168 VGPRs, some spilled): the extra waves hide the
+// quadratures' latency (1.5x over 256 VGPRs at 1 wave).
 __global__ void __launch_bounds__(256, 3) mass_kernel(Phys P, const double* tab, int64_t n,
                                                    const pnjl_mass_query* qs, const double* kc, double G, double K,
-                                                   int Nc, double* out, int stride)
+                                                   int Nc, double* out, int stride, MassState* states,
+                                                   const int32_t* in_list, const unsigned* in_count,
+                                                   int32_t* out_list, unsigned* out_count, int budget)
 {
     __shared__ double lds[kTabLen];
     load_tabs(lds, tab);
     const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ts % stride) return;
-    const int64_t i = ts / stride;
-    if (i >= n) return;
+    const int64_t slot = ts / stride;
+    const bool first = (in_list == nullptr);
+    if (first ? slot >= n : slot >= (int64_t)*in_count) return;
+    const int64_t i = first ? slot : (int64_t)in_list[slot];
     const Tabs T = make_tabs(lds);
     const pnjl_mass_query a = qs[i];
     double* o = out + i * 6;
-    if (a.meson < 0 || a.meson >= PNJL_MESON_ID_COUNT || a.max_iter < 0) {
-        for (int j = 0; j < 6; j++) o[j] = __builtin_nan("");
-        return;
-    }
     const meson::KCoef k = kc ? load_kc(kc + i * 12)
                               : meson::couplings(G, K, meson::G_from_A(a.A_u, a.m_u, Nc),
                                                  meson::G_from_A(a.A_s, a.m_s, Nc));
-    double x[2] = {isnan(a.mass0) ? mass_guess(a) : a.mass0, a.gamma0};
-    double r[2], J[2][2];
-    MassB0 B;
-    mass_b0(P, T, a, x[0], B);
-    mass_residual(P, a, k, B, x[0], x[1], r);
-    if (!isfinite(r[0]) || !isfinite(r[1])) {   // NLsolve's IsFiniteException at the start
-        o[0] = x[0], o[1] = x[1], o[2] = __builtin_nan(""), o[3] = 0.0, o[4] = 0.0, o[5] = 1.0;
+    double x[2], r[2], J[2][2], d[2], delta;
+    int it;
+    bool conv, stopped;
+    if (first) {
+        if (a.meson < 0 || a.meson >= PNJL_MESON_ID_COUNT || a.max_iter < 0) {
+            for (int j = 0; j < 6; j++) o[j] = __builtin_nan("");
+            return;
+        }
+        x[0] = isnan(a.mass0) ? mass_guess(a) : a.mass0;
+        x[1] = a.gamma0;
+        MassB0 B;
+        mass_b0(P, T, a, x[0], B);
+        mass_residual(P, a, k, B, x[0], x[1], r);
+        if (!isfinite(r[0]) || !isfinite(r[1])) {   // NLsolve's IsFiniteException at the start
+            o[0] = x[0], o[1] = x[1], o[2] = __builtin_nan(""), o[3] = 0.0, o[4] = 0.0, o[5] = 1.0;
+            return;
+        }
+        mass_jac(P, T, a, k, x, B, J);
+        it = 0;
+        conv = fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
+        stopped = isnan(x[0]) || isnan(x[1]) || isnan(r[0]) || isnan(r[1]);
+        d[0] = d[1] = delta = 0.0;
+        if (!conv) {
+            for (int j = 0; j < 2; j++) {
+                d[j] = sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]);
+                if (d[j] == 0.0) d[j] = 1.0;
+            }
+            delta = 1.0 * wnorm2(d, x);
+            if (delta == 0.0) delta = 1.0;
+        }
+    } else {
+        const MassState& S = states[i];
+        x[0] = S.x[0], x[1] = S.x[1], r[0] = S.r[0], r[1] = S.r[1];
+        J[0][0] = S.J[0][0], J[0][1] = S.J[0][1], J[1][0] = S.J[1][0], J[1][1] = S.J[1][1];
+        d[0] = S.d[0], d[1] = S.d[1], delta = S.delta;
+        it = S.it, conv = S.conv != 0, stopped = S.stopped != 0;
+    }
+    int spent = 0;
+    while (!stopped && !conv && it < a.max_iter && (budget < 0 || spent < budget)) {
+        it++;
+        spent++;
+        double p[2], fn[2];
+        dogleg2(p, r, d, J, delta);
+        const double xold[2] = {x[0], x[1]};
+        x[0] += p[0], x[1] += p[1];
+        MassB0 Bn;
+        mass_b0(P, T, a, x[0], Bn);
+        mass_residual(P, a, k, Bn, x[0], x[1], fn);
+        double rp[2];
+        for (int q = 0; q < 2; q++) rp[q] = J[q][0] * p[0] + J[q][1] * p[1] + r[q];
+        const double ssr = r[0] * r[0] + r[1] * r[1];
+        const double rho = (ssr - (fn[0] * fn[0] + fn[1] * fn[1])) / (ssr - (rp[0] * rp[0] + rp[1] * rp[1]));
+        if (rho > 1e-4) {
+            r[0] = fn[0], r[1] = fn[1];
+            mass_jac(P, T, a, k, x, Bn, J);
+            for (int j = 0; j < 2; j++) d[j] = fmax(0.1 * d[j], sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]));
+            const bool xc = fmax(fabs(x[0] - xold[0]), fabs(x[1] - xold[1])) < a.xtol;
+            conv = xc || fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
+        } else {
+            x[0] -= p[0], x[1] -= p[1];
+            conv = false;
+        }
+        if (rho < 0.1)
+            delta = delta / 2.0;
+        else if (rho >= 0.9)
+            delta = 2.0 * wnorm2(d, p);
+        else if (rho >= 0.5)
+            delta = fmax(delta, 2.0 * wnorm2(d, p));
+        stopped = isnan(x[0]) || isnan(x[1]) || isnan(fn[0]) || isnan(fn[1]);
+    }
+    if (!stopped && !conv && it < a.max_iter) {
+        // budget spent: save the state for the next phase
+        MassState& S = states[i];
+        S.x[0] = x[0], S.x[1] = x[1], S.r[0] = r[0], S.r[1] = r[1];
+        S.J[0][0] = J[0][0], S.J[0][1] = J[0][1], S.J[1][0] = J[1][0], S.J[1][1] = J[1][1];
+        S.d[0] = d[0], S.d[1] = d[1], S.delta = delta;
+        S.it = it, S.conv = 0, S.stopped = 0;
+        out_list[atomicAdd(out_count, 1u)] = (int32_t)i;
         return;
     }
-    mass_jac(P, T, a, k, x, B, J);
-    int it = 0;
-    bool conv = fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
-    bool stopped = isnan(x[0]) || isnan(x[1]) || isnan(r[0]) || isnan(r[1]);
-    if (!conv) {
-        double d[2];
-        for (int j = 0; j < 2; j++) {
-            d[j] = sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]);
-            if (d[j] == 0.0) d[j] = 1.0;
-        }
-        double delta = 1.0 * wnorm2(d, x);
-        if (delta == 0.0) delta = 1.0;
-        while (!stopped && !conv && it < a.max_iter) {
-            it++;
-            double p[2], fn[2];
-            dogleg2(p, r, d, J, delta);
-            const double xold[2] = {x[0], x[1]};
-            x[0] += p[0], x[1] += p[1];
-            MassB0 Bn;
-            mass_b0(P, T, a, x[0], Bn);
-            mass_residual(P, a, k, Bn, x[0], x[1], fn);
-            double rp[2];
-            for (int q = 0; q < 2; q++) rp[q] = J[q][0] * p[0] + J[q][1] * p[1] + r[q];
-            const double ssr = r[0] * r[0] + r[1] * r[1];
-            const double rho = (ssr - (fn[0] * fn[0] + fn[1] * fn[1])) / (ssr - (rp[0] * rp[0] + rp[1] * rp[1]));
-            if (rho > 1e-4) {
-                r[0] = fn[0], r[1] = fn[1];
-                mass_jac(P, T, a, k, x, Bn, J);
-                for (int j = 0; j < 2; j++) d[j] = fmax(0.1 * d[j], sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]));
-                const bool xc = fmax(fabs(x[0] - xold[0]), fabs(x[1] - xold[1])) < a.xtol;
-                conv = xc || fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
-            } else {
-                x[0] -= p[0], x[1] -= p[1];
-                conv = false;
-            }
-            if (rho < 0.1)
-                delta = delta / 2.0;
-            else if (rho >= 0.9)
-                delta = 2.0 * wnorm2(d, p);
-            else if (rho >= 0.5)
-                delta = fmax(delta, 2.0 * wnorm2(d, p));
-            stopped = isnan(x[0]) || isnan(x[1]) || isnan(fn[0]) || isnan(fn[1]);
-        }
+    o[0] = x[0], o[1] = x[1], o[2] = fmax(fabs(r[0]), fabs(r[1])), o[3] = conv ? 1.0 : 0.0, o[4] = (double)it;
+    o[5] = 0.0;
+}
+
+// The long tail of the pole searches (the last phases: a few percent of the
+// queries, each running hundreds of iterations) with four lanes per query:
+// every trust-region iteration needs B0 at the trial k0 and, when the step is
+// accepted, at k0 +- h for the Jacobian's k0 column; lanes 0, 1, 2 of the
+// query's group evaluate the three at once (lane 3 repeats lane 0) and the
+// group exchanges them, so an iteration costs one B0 evaluation of latency
+// instead of three.  Every lane then runs the same control logic on the same
+// values as the one-thread loop of mass_kernel.
+constexpr int kMassLanes = 4;
+
+__device__ __forceinline__ void shfl_b0(MassB0& dst, const MassB0& src, int lane)
+{
+    for (int j = 0; j < 2; j++) {
+        dst.r[j] = __shfl(src.r[j], lane, kMassLanes);
+        dst.i[j] = __shfl(src.i[j], lane, kMassLanes);
     }
+}
+
+__global__ void __launch_bounds__(256, 3) mass_tail_kernel(Phys P, const double* tab, const pnjl_mass_query* qs,
+                                                        const double* kc, double G, double K, int Nc, double* out,
+                                                        MassState* states, const int32_t* in_list,
+                                                        const unsigned* in_count, int32_t* out_list,
+                                                        unsigned* out_count, int budget)
+{
+    __shared__ double lds[kTabLen];
+    load_tabs(lds, tab);
+    const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t slot = ts / kMassLanes;
+    const int gl = (int)(ts % kMassLanes);
+    if (slot >= (int64_t)*in_count) return;   // whole groups leave together
+    const int64_t i = (int64_t)in_list[slot];
+    const Tabs T = make_tabs(lds);
+    const pnjl_mass_query a = qs[i];
+    const meson::KCoef k = kc ? load_kc(kc + i * 12)
+                              : meson::couplings(G, K, meson::G_from_A(a.A_u, a.m_u, Nc),
+                                                 meson::G_from_A(a.A_s, a.m_s, Nc));
+    const MassState& S = states[i];
+    double x[2] = {S.x[0], S.x[1]}, r[2] = {S.r[0], S.r[1]};
+    double J[2][2] = {{S.J[0][0], S.J[0][1]}, {S.J[1][0], S.J[1][1]}};
+    double d[2] = {S.d[0], S.d[1]}, delta = S.delta;
+    int it = S.it;
+    bool conv = S.conv != 0, stopped = S.stopped != 0;
+    int spent = 0;
+    while (!stopped && !conv && it < a.max_iter && (budget < 0 || spent < budget)) {
+        it++;
+        spent++;
+        double p[2], fn[2];
+        dogleg2(p, r, d, J, delta);
+        const double xold[2] = {x[0], x[1]};
+        x[0] += p[0], x[1] += p[1];
+        // B0 at the trial k0 (lanes 0, 3) and at k0 +- h (lanes 1, 2), h as mass_jac's k0 column
+        const double h = fmax(kFdStep * fabs(x[0]), kFdStep);
+        const double k0 = gl == 1 ? x[0] + h : (gl == 2 ? x[0] - h : x[0]);
+        MassB0 Bl, Bn, Bp, Bm;
+        mass_b0(P, T, a, k0, Bl);
+        shfl_b0(Bn, Bl, 0);   // lanes of the query's group (__shfl's lane is relative to its width)
+        shfl_b0(Bp, Bl, 1);
+        shfl_b0(Bm, Bl, 2);
+        mass_residual(P, a, k, Bn, x[0], x[1], fn);
+        double rp[2];
+        for (int q = 0; q < 2; q++) rp[q] = J[q][0] * p[0] + J[q][1] * p[1] + r[q];
+        const double ssr = r[0] * r[0] + r[1] * r[1];
+        const double rho = (ssr - (fn[0] * fn[0] + fn[1] * fn[1])) / (ssr - (rp[0] * rp[0] + rp[1] * rp[1]));
+        if (rho > 1e-4) {
+            r[0] = fn[0], r[1] = fn[1];
+            // mass_jac with its three B0 sets already at hand
+            for (int j = 0; j < 2; j++) {
+                const double hj = fmax(kFdStep * fabs(x[j]), kFdStep);
+                double xp[2] = {x[0], x[1]}, xm[2] = {x[0], x[1]}, fp[2], fm[2];
+                xp[j] = x[j] + hj;
+                xm[j] = x[j] - hj;
+                if (j == 0) {
+                    mass_residual(P, a, k, Bp, xp[0], xp[1], fp);
+                    mass_residual(P, a, k, Bm, xm[0], xm[1], fm);
+                } else {
+                    mass_residual(P, a, k, Bn, xp[0], xp[1], fp);
+                    mass_residual(P, a, k, Bn, xm[0], xm[1], fm);
+                }
+                for (int q = 0; q < 2; q++) J[q][j] = (fp[q] - fm[q]) / (2.0 * hj);
+            }
+            for (int j = 0; j < 2; j++) d[j] = fmax(0.1 * d[j], sqrt(J[0][j] * J[0][j] + J[1][j] * J[1][j]));
+            const bool xc = fmax(fabs(x[0] - xold[0]), fabs(x[1] - xold[1])) < a.xtol;
+            conv = xc || fmax(fabs(r[0]), fabs(r[1])) < a.ftol;
+        } else {
+            x[0] -= p[0], x[1] -= p[1];
+            conv = false;
+        }
+        if (rho < 0.1)
+            delta = delta / 2.0;
+        else if (rho >= 0.9)
+            delta = 2.0 * wnorm2(d, p);
+        else if (rho >= 0.5)
+            delta = fmax(delta, 2.0 * wnorm2(d, p));
+        stopped = isnan(x[0]) || isnan(x[1]) || isnan(fn[0]) || isnan(fn[1]);
+    }
+    if (gl != 0) return;
+    if (!stopped && !conv && it < a.max_iter) {
+        MassState& W = states[i];
+        W.x[0] = x[0], W.x[1] = x[1], W.r[0] = r[0], W.r[1] = r[1];
+        W.J[0][0] = J[0][0], W.J[0][1] = J[0][1], W.J[1][0] = J[1][0], W.J[1][1] = J[1][1];
+        W.d[0] = d[0], W.d[1] = d[1], W.delta = delta;
+        W.it = it, W.conv = 0, W.stopped = 0;
+        out_list[atomicAdd(out_count, 1u)] = (int32_t)i;
+        return;
+    }
+    double* o = out + i * 6;
     o[0] = x[0], o[1] = x[1], o[2] = fmax(fabs(r[0]), fabs(r[1])), o[3] = conv ? 1.0 : 0.0, o[4] = (double)it;
     o[5] = 0.0;
 }
@@ -1766,11 +1917,37 @@ int pnjl_meson_mass_batch(pnjl_ctx* ctx, int64_t n, const pnjl_mass_query* q, co
         dkc = (const double*)c;
         dout = (double*)b;
     }
+    // phases of growing iteration budgets; the last one runs every remaining
+    // query to its end (max_iter)
+    constexpr int kPhases = 8;
+    const int budgets[kPhases] = {16, 16, 32, 64, 128, 256, 512, -1};
+    void *vs = nullptr, *vl0 = nullptr, *vl1 = nullptr, *vc = nullptr;
+    if (int rc = stage_buf(ctx, 89, (size_t)n * sizeof(polar::MassState), &vs)) return rc;
+    if (int rc = stage_buf(ctx, 90, (size_t)n * sizeof(int32_t), &vl0)) return rc;
+    if (int rc = stage_buf(ctx, 91, (size_t)n * sizeof(int32_t), &vl1)) return rc;
+    if (int rc = stage_buf(ctx, 92, kPhases * sizeof(unsigned), &vc)) return rc;
+    unsigned* cnt = (unsigned*)vc;
+    int32_t* lists[2] = {(int32_t*)vl0, (int32_t*)vl1};
     HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(polar::zero_u32, dim3(1), dim3(64), 0, st, cnt, kPhases);
     const int stride = (flags & PNJL_FLAG_ONE_PER_WAVE) ? 64 : 1;
-    hipLaunchKernelGGL(polar::mass_kernel, dim3(blocks_for(n * stride)), dim3(256), 0, st, phys(ctx), tab, n, dq, dkc,
-                       ctx->params.G, ctx->params.K, ctx->params.N_color, dout, stride);
-    HIPCHK(ctx, hipGetLastError());
+    // the tail phases (from kTailPhase on: few queries, hundreds of
+    // iterations each) run four lanes per query; the one-query-per-wavefront
+    // FLOP count keeps one thread per query throughout
+    constexpr int kTailPhase = 4;
+    for (int ph = 0; ph < kPhases; ph++) {
+        const int32_t* in_list = ph == 0 ? nullptr : lists[(ph - 1) & 1];
+        const unsigned* in_count = ph == 0 ? nullptr : cnt + ph - 1;
+        if (ph >= kTailPhase && stride == 1)
+            hipLaunchKernelGGL(polar::mass_tail_kernel, dim3(blocks_for(n * polar::kMassLanes)), dim3(256), 0, st,
+                               phys(ctx), tab, dq, dkc, ctx->params.G, ctx->params.K, ctx->params.N_color, dout,
+                               (polar::MassState*)vs, in_list, in_count, lists[ph & 1], cnt + ph, budgets[ph]);
+        else
+            hipLaunchKernelGGL(polar::mass_kernel, dim3(blocks_for(n * stride)), dim3(256), 0, st, phys(ctx), tab, n,
+                               dq, dkc, ctx->params.G, ctx->params.K, ctx->params.N_color, dout, stride,
+                               (polar::MassState*)vs, in_list, in_count, lists[ph & 1], cnt + ph, budgets[ph]);
+        HIPCHK(ctx, hipGetLastError());
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
     if (!dev) HIPCHK(ctx, hipMemcpyAsync(out, dout, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost, st));
     if (!(flags & PNJL_FLAG_ASYNC) || !dev) {
