@@ -11,13 +11,33 @@ from test_oracle_tmu_scan import COLS, _oracle_solver, check_against_reference
 pytestmark = pytest.mark.gpu
 
 
+HB = 197.327
+
+
 def test_gpu_tmu_scan(gpu_ctx, oracle, tmp_path):
+    """The GPU scan against the stored scan (the same 47 rows on another root
+    as the oracle-driven scan) and against the oracle-driven scan row by row:
+    'converged' on every row; every row whose state differs beyond the 6
+    written decimals is a root from both states (a Newton run from the GPU's
+    and from the oracle's written state stays there), the count printed and
+    asserted."""
     from julia_relaxtime_amd import tmu_scan as TS
     g_out, o_out = str(tmp_path / "g.csv"), str(tmp_path / "o.csv")
     TS.run_tmu_scan(output_path=g_out)
     TS.run_tmu_scan(output_path=o_out, solver=_oracle_solver(oracle))
-    assert check_against_reference(oracle, g_out) > 0.9
+    check_against_reference(oracle, g_out)
     g, o = list(csv.DictReader(open(g_out))), list(csv.DictReader(open(o_out)))
-    same = [all(abs(float(a[k]) - float(b[k])) <= 1.5e-6 for k in COLS) for a, b in zip(g, o)]
-    assert np.mean(same) >= 0.99
-    assert all(a["converged"] == b["converged"] for a, b, s in zip(g, o, same) if s)
+    assert len(g) == len(o)
+    assert all(a["converged"] == b["converged"] for a, b in zip(g, o))
+    diff = [i for i, (a, b) in enumerate(zip(g, o)) if any(abs(float(a[k]) - float(b[k])) > 1.5e-6 for k in COLS)]
+    print(f"[tmu_scan GPU vs oracle] {len(g) - len(diff)} of {len(g)} rows equal to 6 decimals, {len(diff)} on "
+          f"another root: {[(g[i]['T_MeV'], g[i]['mu_MeV']) for i in diff]}", flush=True)
+    if diff:
+        f = lambda rows, k: np.array([float(rows[i][k]) for i in diff])
+        T, mu = f(g, "T_MeV") / HB, f(g, "mu_MeV") / HB
+        for rows in (g, o):
+            x = np.stack([f(rows, k) for k in ("phi_u", "phi_d", "phi_s", "Phi1", "Phi2")], 1)
+            r = oracle.solve_batch(T, mu, 0.0, policy=oracle.SEED_EXPLICIT, seed=x, p_num=24, t_num=8, nthreads=8)
+            assert r["converged"].all()
+            assert np.all(np.max(np.abs(r["x"] - x), axis=1) < 5e-5)
+    assert len(diff) == 0
