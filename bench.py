@@ -40,6 +40,7 @@ Workloads (grids are linspace with endpoints; MeV -> fm^-1 with 197.327):
 """
 import argparse
 import ctypes as C
+import gc
 import json
 import os
 import sys
@@ -252,12 +253,12 @@ def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
 
 # HBM bytes per scan_kernel launch of the learned-seed scan (c4_surrogate):
 # FETCH_SIZE + WRITE_SIZE of its two launches per step, from two rocprofv3
-# --pmc runs of this bench over three timed steps (FETCH 0.1401 + 0.0119 GiB,
-# WRITE 0.3704 + 0.0548 GiB per step), divided by the two launches.  A 1 GiB
+# --pmc runs of this bench over three timed steps (FETCH 0.1411 + 0.0119 GiB,
+# WRITE 0.3706 + 0.0544 GiB per step), divided by the two launches.  A 1 GiB
 # device copy reads 0.5 GiB in FETCH_SIZE and exactly 1 GiB in WRITE_SIZE,
 # so FETCH is scaled by 2.  The algorithmic bytes are ~0.25 KB per point
 # (inputs, seeds, outputs); the rest is the per-point attempt records.
-C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (0.1401 + 0.0119) + 0.3704 + 0.0548) / 2 * 2 ** 30
+C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (0.1411 + 0.0119) + 0.3706 + 0.0544) / 2 * 2 ** 30
 
 # launch geometry of the learned seed's set-up solves (training targets, judge
 # mask): fixed, so that the trained network does not depend on what the
@@ -344,13 +345,14 @@ def run_propagators(args, w):
         step()
     torch.cuda.synchronize()
     kms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        ev[1].synchronize()
-        kms.append(ev[0].elapsed_time(ev[1]))
-    torch.cuda.synchronize()
-    ms_step = (time.perf_counter() - t0) / args.steps * 1e3
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            ev[1].synchronize()
+            kms.append(ev[0].elapsed_time(ev[1]))
+        torch.cuda.synchronize()
+        ms_step = (time.perf_counter() - t0) / args.steps * 1e3
     o = out.cpu().numpy()
     kavg = float(np.mean(kms))
     flops = None if PROP_FLOPS_PER_QUERY is None else PROP_FLOPS_PER_QUERY * n
@@ -374,12 +376,12 @@ def run_propagators(args, w):
 
 MASS_METRIC = "meson pole masses solved/s (solve_meson_mass, 8 mesons x 65,536 (T, mu) points)"
 # FP64 FLOPs per pole-mass query of this batch, counted once on 8,192 seeded
-# queries run one per wavefront (python3 bench.py --flop-probe mass):
-# 821,068 per query (38.9 trust-region iterations on average; 36 % of the
-# queries stop at the start on a non-finite residual).  The kernel's time is
-# set by its longest serial chains (queries running the full 1000
-# iterations), not by FP64 throughput.
-MASS_FLOPS_PER_QUERY = 821068.0
+# queries run one per wavefront (python3 bench.py --flop-probe mass), summed
+# over the phase launches: 822,180 per query (38.7 trust-region iterations on
+# average; 36 % of the queries stop at the start on a non-finite residual).
+# The kernel's time is set by its longest serial chains (queries running the
+# full 1000 iterations), not by FP64 throughput.
+MASS_FLOPS_PER_QUERY = 822180.0
 
 
 def _mass_worker(args):
@@ -744,17 +746,34 @@ def judge_everywhere(ctx, w, opts, model, Td, mud, xid, seeds, cfg, stream, args
     return dict(judge_everywhere_ms_per_step=ms, judge_everywhere_points_per_s=solved / (ms * 1e-3))
 
 
+class no_gc:
+    """Python's cyclic garbage collector off inside a timed region (as
+    timeit does): a collection that lands between two launches of a step
+    leaves the GPU idle for tens of milliseconds."""
+
+    def __enter__(self):
+        gc.collect()
+        self.was = gc.isenabled()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        if self.was:
+            gc.enable()
+
+
 def timed_steps(step, steps, warmup):
     import torch
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     scan_ms = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        scan_ms.append(step())
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps * 1e3, scan_ms
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            scan_ms.append(step())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    return dt / steps * 1e3, scan_ms
 
 
 # ---------------------------------------------------------------------------
@@ -934,16 +953,24 @@ def main():
         cfg = fixed or ctx.autotune(T, mu, xi, policy=policy, seed=model.infer(T, mu, xi),
                                     p_num=w["p_num"], t_num=w["t_num"], opts=opts)[0]
 
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        gpu_ms = []
+
         def step():
             ctx.set_launch_cfg(**cfg)
-            model.infer(Td, mud, xid, out=state["seeds"], stream=stream)
+            ev[0].record(stream)
+            model.infer(Td, mud, xid, out=state["seeds"], stream=stream, sync=False)
             state["out"] = ctx.solve_batch(Td, mud, xid, policy=policy, seed=state["seeds"],
                                            p_num=w["p_num"], t_num=w["t_num"], opts=opts, out=state.get("out"),
                                            stream=stream, sync=True)
+            ev[1].record(stream)
+            ev[1].synchronize()
+            gpu_ms.append(ev[0].elapsed_time(ev[1]))
             state["stage_ms"] = ctx.last_scan_ms()
             return float(np.sum(state["stage_ms"]))
 
         ms_step, scan_ms = timed_steps(step, args.steps, args.warmup)
+        gpu_ms = gpu_ms[args.warmup:]
         Q = {k: v.cpu().numpy() for k, v in state["out"].items()}
         flops = scan_flops(state["out"])
         solved = int(np.sum((Q["converged"] == 1) & (Q["residual_norm"] < 1e-10)))
@@ -961,6 +988,8 @@ def main():
             node_passes=int(Q["passes"].sum()), plain_node_passes=int(P["passes"].sum()),
             plain_scan_kernel_ms=float(np.mean(scan_plain)), plain_launch=plain_cfg,
             scan_stage_ms=[round(float(v), 3) for v in state["stage_ms"]],
+            gpu_ms_per_step=float(np.mean(gpu_ms)), gpu_ms_steps=[round(float(v), 2) for v in gpu_ms],
+            scan_ms_steps=[round(float(v), 2) for v in scan_ms],
             plain_fp64_tflops=flops_plain / (np.mean(scan_plain) * 1e-3) / 1e12)
         if w["policy"] == "MULTIROOT3":
             eo = (Q["omega"] - P["omega"]) / np.abs(P["omega"])

@@ -142,9 +142,10 @@ class Surrogate:
         self._check(rc, "pnjl_surrogate_loss_grad")
         return loss.value, g
 
-    def infer(self, T, mu, xi, out=None, stream=None):
+    def infer(self, T, mu, xi, out=None, stream=None, sync=True):
         """Seeds (n,5) = (phi_u, phi_u, phi_s, Phi, Phibar).  numpy in -> numpy
-        out; torch.cuda tensors in -> a torch tensor in HBM."""
+        out; torch.cuda tensors in -> a torch tensor in HBM (sync=False: only
+        enqueued on the stream, e.g. ahead of a solve on the same stream)."""
         if type(T).__module__.startswith("torch"):
             import torch
             n = T.numel()
@@ -152,7 +153,8 @@ class Surrogate:
                 out = torch.empty((n, 5), dtype=torch.float64, device=T.device)
             st = stream if stream is not None else torch.cuda.current_stream(T.device)
             rc = self._L.pnjl_surrogate_infer(self.h, n, T.data_ptr(), mu.data_ptr(), xi.data_ptr(), out.data_ptr(),
-                                              _lib.FLAG_DEVICE_PTRS, C.c_void_p(st.cuda_stream))
+                                              _lib.FLAG_DEVICE_PTRS | (0 if sync else _lib.FLAG_ASYNC),
+                                              C.c_void_p(st.cuda_stream))
             self._check(rc, "pnjl_surrogate_infer")
             return out
         T, mu, xi = _d(T).ravel(), _d(mu).ravel(), _d(xi).ravel()
