@@ -33,8 +33,12 @@
 // Storage: the group's hot state -- the reduced node sums of the last pass,
 // the trust-region state, the attempt's seed, the previous chain solution
 // and x_old -- lives in LDS next to the node tables (STATE_STRIDE doubles
-// per group), and every function is inlined, so no call frames or spills
-// go to memory.  Candidate records (x, Omega, thermodynamics, flags), which
+// per group).  The node pass and the Newton step are inlined into the
+// kernel; the cold paths (an item's start and end, candidate records, the
+// candidate choice, the trust-region step) are out-of-line functions that
+// take the group's storage and read the launch arguments from an LDS copy
+// (cold_args), so the kernel keeps 2 waves per SIMD without spilling vector
+// registers.  Candidate records (x, Omega, thermodynamics, flags), which
 // are written once per attempt end or accepted trust-region step, live in a
 // small per-group area in HBM.
 #pragma once
