@@ -133,39 +133,97 @@ __device__ __forceinline__ double p_int(double E, double m)
     return sqrt(E * E - mp * mp);
 }
 
-// integrate_hybrid_interval  IntervalQuadratureStrategies.jl:194-310
-template <class F>
-__device__ double rule_power(const F& f, double a, double b, const double* t, const double* tp, const double* W, int n)
+// Lane-split quadrature (L > 1, the pole search's long tail): the L
+// consecutive lanes of a group each evaluate the nodes i = gl, gl + L, ... of
+// an N-node rule and leave the terms in the group's LDS row, then every lane
+// adds all N terms in the order i = 0 .. N-1 -- the one-thread loop's sum, bit
+// for bit, on every lane.  A skipped (non-finite) node contributes +0.0 there,
+// which leaves any sum the loop can hold unchanged: the sums start at +0.0
+// and so never hold -0.0.  Callers keep the group's control flow uniform
+// (same arguments on every lane of the group); the group's lanes are in one
+// wavefront, whose LDS accesses complete in program order.  L = 1 is the
+// one-thread loop itself.
+constexpr int kQRow = 33;              // terms of the longest rule
+constexpr int kQRows = 256 / 4;        // rows per 256-thread block at 4 lanes per group
+__shared__ double qterms[kQRows][kQRow];
+
+template <int L>
+__device__ __forceinline__ int qlane()
 {
-    double len = b - a, total = 0.0;
-    for (int i = 0; i < n; i++) {
-        double x = a + len * t[i];
-        double w = W[i] * len * tp[i];
-        double v = f(x);
-        if (isfinite(v)) total += w * v;
-    }
-    return total;
+    return L == 1 ? 0 : (int)(threadIdx.x & (L - 1));
 }
 
-template <class F>
-__device__ double hybrid(const F& f, const Tabs& q, double a, double b, int pos, int n)
+template <int L, int N, class Term>
+__device__ __forceinline__ double ordered_sum(double acc, const Term& term)
 {
-    if (!(isfinite(a) && isfinite(b)) || b <= a) return 0.0;
-    if (pos == BOTH) {
-        double half = (b - a) / 2, center = (a + b) / 2, total = 0.0;
-        for (int i = 0; i < 33; i++) {
-            double x = center + half * q.de_phi[i];
-            double w = half * q.de_phip[i];
+    static_assert(L > 1 && (L & (L - 1)) == 0 && 256 / L <= kQRows && N <= kQRow, "lane-split rule shape");
+    double* row = qterms[threadIdx.x / L];
+    const int gl = qlane<L>();
+#pragma unroll 1
+    for (int i = gl; i < N; i += L) row[i] = term(i);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int i = 0; i < N; i++) acc += row[i];   // the loads issue together, the adds keep their order
+    // the row is written again by the group's next rule only after every lane has read it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return acc;
+}
+
+// integrate_hybrid_interval  IntervalQuadratureStrategies.jl:194-310
+template <int L, int N, class F>
+__device__ double rule_power(const F& f, double a, double b, const double* t, const double* tp, const double* W)
+{
+    double len = b - a, total = 0.0;
+    if constexpr (L > 1) {
+        return ordered_sum<L, N>(total, [&](int i) {
+            double x = a + len * t[i];
+            double w = W[i] * len * tp[i];
+            double v = f(x);
+            return isfinite(v) ? w * v : 0.0;
+        });
+    } else {
+        for (int i = 0; i < N; i++) {
+            double x = a + len * t[i];
+            double w = W[i] * len * tp[i];
             double v = f(x);
             if (isfinite(v)) total += w * v;
         }
         return total;
     }
+}
+
+template <int L, class F>
+__device__ double hybrid(const F& f, const Tabs& q, double a, double b, int pos, int n)
+{
+    if (!(isfinite(a) && isfinite(b)) || b <= a) return 0.0;
+    if (pos == BOTH) {
+        double half = (b - a) / 2, center = (a + b) / 2, total = 0.0;
+        if constexpr (L > 1) {
+            return ordered_sum<L, 33>(total, [&](int i) {
+                double x = center + half * q.de_phi[i];
+                double w = half * q.de_phip[i];
+                double v = f(x);
+                return isfinite(v) ? w * v : 0.0;
+            });
+        } else {
+            for (int i = 0; i < 33; i++) {
+                double x = center + half * q.de_phi[i];
+                double w = half * q.de_phip[i];
+                double v = f(x);
+                if (isfinite(v)) total += w * v;
+            }
+            return total;
+        }
+    }
     if (pos == RIGHT)
-        return n == 16 ? rule_power(f, a, b, q.pr_t16, q.pr_tp16, q.w16, 16)
-                       : rule_power(f, a, b, q.pr_t32, q.pr_tp32, q.w32, 32);
-    return n == 16 ? rule_power(f, a, b, q.pl_t16, q.pl_tp16, q.w16, 16)
-                   : rule_power(f, a, b, q.pl_t32, q.pl_tp32, q.w32, 32);
+        return n == 16 ? rule_power<L, 16>(f, a, b, q.pr_t16, q.pr_tp16, q.w16)
+                       : rule_power<L, 32>(f, a, b, q.pr_t32, q.pr_tp32, q.w32);
+    return n == 16 ? rule_power<L, 16>(f, a, b, q.pl_t16, q.pl_tp16, q.w16)
+                   : rule_power<L, 32>(f, a, b, q.pl_t32, q.pl_tp32, q.w32);
 }
 
 // ---- B~0 at k = 0  OneLoopIntegrals.jl:117-224 ------------------------------
@@ -212,6 +270,7 @@ struct B0Regular {
     }
 };
 
+template <int L>
 __device__ void tilde_B0_k0(const Phys& P, const Tabs& q, int sg, double lam, double m_in, double mp_in, double mu,
                             double T, double Phi, double Phib, double& re, double& im)
 {
@@ -227,7 +286,7 @@ __device__ void tilde_B0_k0(const Phys& P, const Tabs& q, int sg, double lam, do
         has = (E0 > Emin - tol) && (E0 < Emax + tol);
     }
     if (!has) {
-        re = hybrid(f, q, Emin, Emax, NONE, 16) * 2.0;
+        re = hybrid<L>(f, q, Emin, Emax, NONE, 16) * 2.0;
         im = 0.0 / lam;
         return;
     }
@@ -239,11 +298,11 @@ __device__ void tilde_B0_k0(const Phys& P, const Tabs& q, int sg, double lam, do
         double gap = fmax(kPvGapRel * (Emax - Emin), 64 * jeps(E0));
         double left_end = fmax(Emin, E0 - gap), right_start = fmin(Emax, E0 + gap);
         pv = 0.0;
-        if (left_end > Emin) pv += hybrid(f, q, Emin, left_end, NONE, 16);
-        if (right_start < Emax) pv += hybrid(f, q, right_start, Emax, NONE, 16);
+        if (left_end > Emin) pv += hybrid<L>(f, q, Emin, left_end, NONE, 16);
+        if (right_start < Emax) pv += hybrid<L>(f, q, right_start, Emax, NONE, 16);
     } else {
         const B0Regular g{num, E0, f0, Emin, Emax};
-        double regular = hybrid(g, q, Emin, Emax, NONE, 32);
+        double regular = hybrid<L>(g, q, Emin, Emax, NONE, 32);
         double logterm = f0 * (log(fabs(Emax - E0)) - log(fabs(Emin - E0)));
         pv = (regular + logterm) / lam;
     }
@@ -339,10 +398,10 @@ __device__ int sing_kpos(double lam, double k, double m, double mp, double Emin,
 }
 
 // integrate_piecewise_hybrid  :335-421
-template <class F>
+template <int L, class F>
 __device__ double piecewise(const F& f, const Tabs& q, double Emin, double Emax, int niv, const double iv[2][2])
 {
-    if (niv == 0) return hybrid(f, q, Emin, Emax, NONE, 16);
+    if (niv == 0) return hybrid<L>(f, q, Emin, Emax, NONE, 16);
     double E1 = __builtin_nan(""), E2 = __builtin_nan("");
     if (niv == 1) {
         E1 = iv[0][0];
@@ -357,9 +416,9 @@ __device__ double piecewise(const F& f, const Tabs& q, double Emin, double Emax,
     }
     if (isfinite(E1) && isfinite(E2)) {
         double total = 0.0;
-        if (E1 > Emin) total += hybrid(f, q, Emin, E1, RIGHT, 32);
-        if (E2 > E1) total += hybrid(f, q, E1, E2, BOTH, 32);
-        if (Emax > E2) total += hybrid(f, q, E2, Emax, LEFT, 16);
+        if (E1 > Emin) total += hybrid<L>(f, q, Emin, E1, RIGHT, 32);
+        if (E2 > E1) total += hybrid<L>(f, q, E1, E2, BOTH, 32);
+        if (Emax > E2) total += hybrid<L>(f, q, E2, Emax, LEFT, 16);
         return total;
     }
     // general path: sorted unique breakpoints (insertion sort of <= 6 values)
@@ -382,17 +441,18 @@ __device__ double piecewise(const F& f, const Tabs& q, double Emin, double Emax,
         if (u == 0 || pts[i] != pts[u - 1]) pts[u++] = pts[i];
     const int nseg = u - 1;
     if (nseg <= 0) return 0.0;
-    if (nseg == 1) return hybrid(f, q, pts[0], pts[1], NONE, 16);
+    if (nseg == 1) return hybrid<L>(f, q, pts[0], pts[1], NONE, 16);
     double total = 0.0;
     for (int i = 1; i <= nseg; i++) {
         double a = pts[i - 1], b = pts[i];
         if (!(isfinite(a) && isfinite(b)) || b <= a) continue;
         int pos = (i == 1) ? RIGHT : (i == nseg ? LEFT : BOTH);
-        total += hybrid(f, q, a, b, pos, i == nseg ? 16 : 32);
+        total += hybrid<L>(f, q, a, b, pos, i == nseg ? 16 : 32);
     }
     return total;
 }
 
+template <int L>
 __device__ void tilde_B0_kpos(const Phys& P, const Tabs& q, int sg, double lam, double k, double m_in, double mp_in,
                               double mu, double T, double Phi, double Phib, double& re, double& im)
 {
@@ -401,7 +461,7 @@ __device__ void tilde_B0_kpos(const Phys& P, const Tabs& q, int sg, double lam, 
     double iv[2][2];
     const int niv = sing_kpos(lam, k, m, mp, Emin, Emax, iv);
     const B0Kpos f{sg, lam, k, m, mp, mu, T, Phi, Phib};
-    const double real = piecewise(f, q, Emin, Emax, niv, iv);
+    const double real = piecewise<L>(f, q, Emin, Emax, niv, iv);
     double imag = 0.0;
     if (niv > 0) {
         for (int i = 0; i < niv; i++) imag += dist_int_b0(sg, iv[i][0], iv[i][1], mu, T, Phi, Phib);
@@ -411,16 +471,18 @@ __device__ void tilde_B0_kpos(const Phys& P, const Tabs& q, int sg, double lam, 
     im = imag / k;
 }
 
+template <int L>
 __device__ void tilde_B0(const Phys& P, const Tabs& q, int sg, double lam, double k, double m, double mp, double mu,
                          double T, double Phi, double Phib, double& re, double& im)
 {
-    if (fabs(k) < kEpsK) tilde_B0_k0(P, q, sg, lam, m, mp, mu, T, Phi, Phib, re, im);
-    else tilde_B0_kpos(P, q, sg, lam, k, m, mp, mu, T, Phi, Phib, re, im);
+    if (fabs(k) < kEpsK) tilde_B0_k0<L>(P, q, sg, lam, m, mp, mu, T, Phi, Phib, re, im);
+    else tilde_B0_kpos<L>(P, q, sg, lam, k, m, mp, mu, T, Phi, Phib, re, im);
 }
 
 // B0  OneLoopIntegrals.jl:489-503: B~+(-l; m1, m2, mu1) - B~-(l; m1, m2, mu1)
 // + B~+(l; m2, m1, mu2) - B~-(-l; m2, m1, mu2), summed in this order.  One
 // loop over the four terms keeps a single copy of tilde_B0 in the kernel.
+template <int L = 1>
 __device__ void B0(const Phys& P, const Tabs& q, double lam, double k, double m1, double mu1, double m2, double mu2,
                    double T, double Phi, double Phib, double& re, double& im)
 {
@@ -431,7 +493,7 @@ __device__ void B0(const Phys& P, const Tabs& q, double lam, double k, double m1
         const bool swap = t >= 2;
         const double l = (t == 0 || t == 3) ? -lam : lam;
         double r, i;
-        tilde_B0(P, q, (t & 1) ? MINUS : PLUS, l, k, swap ? m2 : m1, swap ? m1 : m2, swap ? mu2 : mu1, T, Phi, Phib,
+        tilde_B0<L>(P, q, (t & 1) ? MINUS : PLUS, l, k, swap ? m2 : m1, swap ? m1 : m2, swap ? mu2 : mu1, T, Phi, Phib,
                  r, i);
         if (t & 1) {
             re -= r;
@@ -529,6 +591,7 @@ __device__ int roots_AB(double lam, double k, double m, double mp, double Emin, 
 }
 
 // tilde_B0_correction_k_zero  :133-190
+template <int L>
 __device__ void tilde_Bc_k0(const Phys& P, const Tabs& q, int sg, double lam, double m, double mp, double mu, double T,
                             double Phi, double Phib, double xi, double& re, double& im)
 {
@@ -550,13 +613,17 @@ __device__ void tilde_Bc_k0(const Phys& P, const Tabs& q, int sg, double lam, do
     };
     double sum = 0.0;
     const double A0 = has ? numer(E0) : 0.0;
-    for (int i = 0; i < 32; i++) {
+    auto term = [&](int i) {
         double E = ((Emax - Emin) * q.x32[i] + (Emax + Emin)) / 2.0;
         double w = q.w32[i] * (Emax - Emin) / 2.0;
         double den = cE * E + dc;
         double v = numer(E) - A0;
-        sum += w * ((isfinite(den) && isfinite(v)) ? v / den : 0.0);
-    }
+        return w * ((isfinite(den) && isfinite(v)) ? v / den : 0.0);
+    };
+    if constexpr (L > 1)
+        sum = ordered_sum<L, 32>(sum, term);
+    else
+        for (int i = 0; i < 32; i++) sum += term(i);
     if (has) sum += A0 / cE * (log(fabs(cE * Emax + dc)) - log(fabs(cE * Emin + dc)));
     re = sum;
     // imag_integrand_k_zero  :98-112
@@ -570,6 +637,7 @@ __device__ void tilde_Bc_k0(const Phys& P, const Tabs& q, int sg, double lam, do
 }
 
 // tilde_B0_correction_k_positive  :307-375
+template <int L>
 __device__ void tilde_Bc_kpos(const Phys& P, const Tabs& q, int sg, double lam, double k, double m, double mp,
                               double mu, double T, double Phi, double Phib, double xi, double& re, double& im)
 {
@@ -605,7 +673,7 @@ __device__ void tilde_Bc_kpos(const Phys& P, const Tabs& q, int sg, double lam, 
         else if (i == niv - 1) pos = LEFT;
         else pos = BOTH;
         const int n = (nr == 0) ? 16 : (pos == LEFT ? 16 : 32);
-        real += hybrid(fr, q, iv[i][0], iv[i][1], pos, n);
+        real += hybrid<L>(fr, q, iv[i][0], iv[i][1], pos, n);
     }
     double imag = 0.0;
     if (nimag > 0) {
@@ -613,11 +681,20 @@ __device__ void tilde_Bc_kpos(const Phys& P, const Tabs& q, int sg, double lam, 
         for (int j = 0; j < nimag; j++) {
             const double E1 = ivi[j][0], E2 = ivi[j][1];
             const double half = (E2 - E1) / 2, center = (E1 + E2) / 2;
-            for (int i = 0; i < 16; i++) {
-                double x = center + half * q.x16[i];
-                double w = q.w16[i] * half;
-                double v = fi(x);
-                if (isfinite(v)) imag += w * v;
+            if constexpr (L > 1) {
+                imag = ordered_sum<L, 16>(imag, [&](int i) {
+                    double x = center + half * q.x16[i];
+                    double w = q.w16[i] * half;
+                    double v = fi(x);
+                    return isfinite(v) ? w * v : 0.0;
+                });
+            } else {
+                for (int i = 0; i < 16; i++) {
+                    double x = center + half * q.x16[i];
+                    double w = q.w16[i] * half;
+                    double v = fi(x);
+                    if (isfinite(v)) imag += w * v;
+                }
             }
         }
         imag *= jsign(lam);
@@ -626,16 +703,18 @@ __device__ void tilde_Bc_kpos(const Phys& P, const Tabs& q, int sg, double lam, 
     im = imag;
 }
 
+template <int L>
 __device__ void tilde_Bc(const Phys& P, const Tabs& q, int sg, double lam, double k, double m, double mp, double mu,
                          double T, double Phi, double Phib, double xi, double& re, double& im)
 {
-    if (k > kEpsK) tilde_Bc_kpos(P, q, sg, lam, k, m, mp, mu, T, Phi, Phib, xi, re, im);
-    else tilde_Bc_k0(P, q, sg, lam, m, mp, mu, T, Phi, Phib, xi, re, im);
+    if (k > kEpsK) tilde_Bc_kpos<L>(P, q, sg, lam, k, m, mp, mu, T, Phi, Phib, xi, re, im);
+    else tilde_Bc_k0<L>(P, q, sg, lam, m, mp, mu, T, Phi, Phib, xi, re, im);
 }
 
 // B0_correction  OneLoopIntegralsAniso.jl:396-413 (all four terms added):
 // dB~q(-l; m1, m2, mu1) + dB~qbar(l; m1, m2, mu1) + dB~q(l; m2, m1, mu2)
 // + dB~qbar(-l; m2, m1, mu2).
+template <int L = 1>
 __device__ void B0_corr(const Phys& P, const Tabs& q, double lam, double k, double m1, double m2, double mu1,
                         double mu2, double T, double Phi, double Phib, double xi, double& re, double& im)
 {
@@ -646,7 +725,7 @@ __device__ void B0_corr(const Phys& P, const Tabs& q, double lam, double k, doub
         const bool swap = t >= 2;
         const double l = (t == 0 || t == 3) ? -lam : lam;
         double r, i;
-        tilde_Bc(P, q, (t & 1) ? ANTIQUARK : QUARK, l, k, swap ? m2 : m1, swap ? m1 : m2, swap ? mu2 : mu1, T, Phi,
+        tilde_Bc<L>(P, q, (t & 1) ? ANTIQUARK : QUARK, l, k, swap ? m2 : m1, swap ? m1 : m2, swap ? mu2 : mu1, T, Phi,
                  Phib, xi, r, i);
         re += r;
         im += i;
@@ -655,6 +734,7 @@ __device__ void B0_corr(const Phys& P, const Tabs& q, double lam, double k, doub
 
 // The symmetrised B0 (+ B0_correction) of polarization_aniso
 //   PolarizationAniso.jl:29-54
+template <int L = 1>
 __device__ void b0_total(const Phys& P, const Tabs& q, double k0, double k, double m1, double m2, double mu1,
                          double mu2, double T, double Phi, double Phib, double xi, int num_s, double& br, double& bi)
 {
@@ -664,7 +744,7 @@ __device__ void b0_total(const Phys& P, const Tabs& q, double k0, double k, doub
 #pragma unroll 1
     for (int v = 0; v < nv; v++) {
         const double l = (v == 0 ? k0 : -k0) + mu1 - mu2;
-        B0(P, q, l, k, m1, mu1, m2, mu2, T, Phi, Phib, vr[v], vi[v]);
+        B0<L>(P, q, l, k, m1, mu1, m2, mu2, T, Phi, Phib, vr[v], vi[v]);
     }
     br = nv == 2 ? 0.5 * (vr[0] + vr[1]) : vr[0];
     bi = nv == 2 ? 0.5 * (vi[0] + vi[1]) : vi[0];
@@ -672,7 +752,7 @@ __device__ void b0_total(const Phys& P, const Tabs& q, double k0, double k, doub
 #pragma unroll 1
         for (int v = 0; v < nv; v++) {
             const double l = (v == 0 ? k0 : -k0) + mu1 - mu2;
-            B0_corr(P, q, l, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, vr[v], vi[v]);
+            B0_corr<L>(P, q, l, k, m1, m2, mu1, mu2, T, Phi, Phib, xi, vr[v], vi[v]);
         }
         br += nv == 2 ? 0.5 * (vr[0] + vr[1]) : vr[0];
         bi += nv == 2 ? 0.5 * (vi[0] + vi[1]) : vi[0];
@@ -1124,6 +1204,7 @@ __device__ __forceinline__ int mass_pairs(int m, int* pairs)
 
 // B0 (symmetrised for u sbar, + correction) of the meson's pairs at k0
 //   MesonMass.jl:34-60, :222-231 (MesonMass passes num_s 0 for s sbar, equivalent to 2)
+template <int L = 1>
 __device__ __noinline__ void mass_b0(const Phys& P, const Tabs& T, const pnjl_mass_query& a, double k0, MassB0& B)
 {
     int pairs[2];
@@ -1132,7 +1213,7 @@ __device__ __noinline__ void mass_b0(const Phys& P, const Tabs& T, const pnjl_ma
     for (int j = 0; j < np; j++) {
         double m1, m2, mu1, mu2, A12;
         mass_pair(a, pairs[j], m1, m2, mu1, mu2, A12);
-        b0_total(P, T, k0, a.k, m1, m2, mu1, mu2, a.T, a.Phi, a.Phib, a.xi, pairs[j] == 1 ? 1 : 0, B.r[j], B.i[j]);
+        b0_total<L>(P, T, k0, a.k, m1, m2, mu1, mu2, a.T, a.Phi, a.Phib, a.xi, pairs[j] == 1 ? 1 : 0, B.r[j], B.i[j]);
     }
 }
 
@@ -1379,15 +1460,22 @@ __global__ void __launch_bounds__(256, 3) mass_kernel(Phys P, const double* tab,
     o[5] = 0.0;
 }
 
-// The long tail of the pole searches (the last phases: a few percent of the
-// queries, each running hundreds of iterations) with four lanes per query:
-// every trust-region iteration needs B0 at the trial k0 and, when the step is
-// accepted, at k0 +- h for the Jacobian's k0 column; lanes 0, 1, 2 of the
-// query's group evaluate the three at once (lane 3 repeats lane 0) and the
-// group exchanges them, so an iteration costs one B0 evaluation of latency
-// instead of three.  Every lane then runs the same control logic on the same
-// values as the one-thread loop of mass_kernel.
-constexpr int kMassLanes = 4;
+// The long tail of the pole searches (the phases after the first two: a few
+// percent of the queries, most of them running NLsolve's full 1000
+// iterations) with 16 lanes per query, four sub-groups of four: every
+// trust-region iteration needs B0 at the trial k0 and, when the step is
+// accepted, at k0 +- h for the Jacobian's k0 column; sub-groups 0, 1, 2
+// evaluate the three at once (sub-group 3 repeats sub-group 0), each with its
+// quadrature rules split over its four lanes and summed in the one-thread
+// order (ordered_sum), and the group exchanges them.  So an iteration costs a
+// quarter of one B0 evaluation of latency instead of three evaluations, the
+// phase runs 16x more lanes to hide it, and every lane runs the same control
+// logic on the same values, bit for bit, as the one-thread loop of
+// mass_kernel.  (Splitting a mixed meson's two quark pairs over sub-groups
+// as well was slower: the pairs' masses differ, so their quadrature branches
+// diverge within the wavefront.)
+constexpr int kMassLanes = 16;
+constexpr int kMassQLanes = 4;   // quadrature lanes per B0 evaluation
 
 __device__ __forceinline__ void shfl_b0(MassB0& dst, const MassB0& src, int lane)
 {
@@ -1403,11 +1491,14 @@ __global__ void __launch_bounds__(256, 3) mass_tail_kernel(Phys P, const double*
                                                         const unsigned* in_count, int32_t* out_list,
                                                         unsigned* out_count, int budget)
 {
+    // blocks past the list leave before the table load (uniform per block)
+    if ((int64_t)blockIdx.x * (blockDim.x / kMassLanes) >= (int64_t)*in_count) return;
     __shared__ double lds[kTabLen];
     load_tabs(lds, tab);
     const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t slot = ts / kMassLanes;
     const int gl = (int)(ts % kMassLanes);
+    const int sub = gl / kMassQLanes;
     if (slot >= (int64_t)*in_count) return;   // whole groups leave together
     const int64_t i = (int64_t)in_list[slot];
     const Tabs T = make_tabs(lds);
@@ -1429,14 +1520,14 @@ __global__ void __launch_bounds__(256, 3) mass_tail_kernel(Phys P, const double*
         dogleg2(p, r, d, J, delta);
         const double xold[2] = {x[0], x[1]};
         x[0] += p[0], x[1] += p[1];
-        // B0 at the trial k0 (lanes 0, 3) and at k0 +- h (lanes 1, 2), h as mass_jac's k0 column
+        // B0 at the trial k0 (sub-groups 0, 3) and at k0 +- h (sub-groups 1, 2), h as mass_jac's k0 column
         const double h = fmax(kFdStep * fabs(x[0]), kFdStep);
-        const double k0 = gl == 1 ? x[0] + h : (gl == 2 ? x[0] - h : x[0]);
+        const double k0 = sub == 1 ? x[0] + h : (sub == 2 ? x[0] - h : x[0]);
         MassB0 Bl, Bn, Bp, Bm;
-        mass_b0(P, T, a, k0, Bl);
+        mass_b0<kMassQLanes>(P, T, a, k0, Bl);
         shfl_b0(Bn, Bl, 0);   // lanes of the query's group (__shfl's lane is relative to its width)
-        shfl_b0(Bp, Bl, 1);
-        shfl_b0(Bm, Bl, 2);
+        shfl_b0(Bp, Bl, kMassQLanes);
+        shfl_b0(Bm, Bl, 2 * kMassQLanes);
         mass_residual(P, a, k, Bn, x[0], x[1], fn);
         double rp[2];
         for (int q = 0; q < 2; q++) rp[q] = J[q][0] * p[0] + J[q][1] * p[1] + r[q];
@@ -1932,9 +2023,9 @@ int pnjl_meson_mass_batch(pnjl_ctx* ctx, int64_t n, const pnjl_mass_query* q, co
     hipLaunchKernelGGL(polar::zero_u32, dim3(1), dim3(64), 0, st, cnt, kPhases);
     const int stride = (flags & PNJL_FLAG_ONE_PER_WAVE) ? 64 : 1;
     // the tail phases (from kTailPhase on: few queries, hundreds of
-    // iterations each) run four lanes per query; the one-query-per-wavefront
-    // FLOP count keeps one thread per query throughout
-    constexpr int kTailPhase = 4;
+    // iterations each) run kMassLanes lanes per query; the
+    // one-query-per-wavefront FLOP count keeps one thread per query throughout
+    constexpr int kTailPhase = 2;
     for (int ph = 0; ph < kPhases; ph++) {
         const int32_t* in_list = ph == 0 ? nullptr : lists[(ph - 1) & 1];
         const unsigned* in_count = ph == 0 ? nullptr : cnt + ph - 1;
