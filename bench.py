@@ -260,38 +260,12 @@ def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
 # (inputs, seeds, outputs); the rest is the per-point attempt records.
 C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (0.1411 + 0.0119) + 0.3706 + 0.0544) / 2 * 2 ** 30
 
-# launch geometry of the learned seed's set-up solves (training targets, judge
-# mask): fixed, so that the trained network does not depend on what the
-# autotuner picked for the timed scans (lanes per point changes the node
-# sums' order, hence the last bits of the targets, hence the training run)
-SETUP_LAUNCH = dict(lanes_per_point=8, chunk=1, waves_per_cu=16, tail_lanes_per_point=0)
-
-
 def train_learned_seed(ctx, w, opts):
-    saved = ctx.launch_cfg()
-    ctx.set_launch_cfg(**SETUP_LAUNCH)
-    try:
-        return _train_learned_seed(ctx, w, opts)
-    finally:
-        ctx.set_launch_cfg(**saved)
-
-
-def _train_learned_seed(ctx, w, opts):
-    from julia_relaxtime_amd.surrogate import Surrogate, training_points
-    t0 = time.perf_counter()
-    T, mu, xi = training_points(200_000, seed=4)
-    tr = ctx.solve_batch(T, mu, xi, policy=0, p_num=w["p_num"], t_num=w["t_num"], opts=opts)
-    ok = tr["converged"] == 1
-    s = Surrogate(0, init_seed=4)
-    s.fit_output_scaling(tr["x"][ok])
-    loss = s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=20000, batch=4096, lr=1e-3, seed=4)
-    train_ms = s.last_ms
-    Tp, mup, xip = training_points(1_000_000, seed=5)
-    s.build_judge_mask(ctx, np.r_[T, Tp], np.r_[mu, mup], np.r_[xi, xip], p_num=w["p_num"], t_num=w["t_num"],
-                       opts=opts)
-    info = dict(train_points=int(ok.sum()), adam_steps=20000, batch=4096, lr=1e-3, train_kernel_ms=train_ms,
-                final_loss=float(np.mean(loss[-200:])), judge_mask_fraction=float(s.judge_mask.mean()),
-                setup_s=time.perf_counter() - t0)
+    """The learned seed's set-up (surrogate.learned_seed): training roots,
+    Adam, judge mask; `info` goes into the line's config."""
+    from julia_relaxtime_amd.surrogate import learned_seed
+    s, info = learned_seed(ctx, p_num=w["p_num"], t_num=w["t_num"], opts=opts)
+    info.pop("loss")
     return s, info
 
 

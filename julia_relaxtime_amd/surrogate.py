@@ -177,12 +177,17 @@ class Surrogate:
         return self
 
     def build_judge_mask(self, ctx, T, mu, xi, plain_roots=None, p_num=64, t_num=4, opts=None, bins=(128, 128, 16),
-                         dilate=1, multi_root=False, refine=8, refine_seed=6, neighbourhood="full"):
+                         dilate=1, multi_root=False, refine=8, refine_seed=6, neighbourhood="full",
+                         multi_max_iter=12):
         """Ambiguity mask of the learned seed's judge, learned on probe points:
         a bin is set where the learned-seed root, the plain (DefaultSeed) root
         and the MultiSeed (argmin-Omega) root are not all the same root, or a
         solve failed; with `multi_root`, also where the learned, HADRON and
-        QUARK seeds reach distinct roots.
+        QUARK seeds reach distinct roots.  The MultiSeed probe runs each
+        candidate for at most `multi_max_iter` iterations (Newton, then the
+        trust region): it only has to find the competing roots, and its
+        candidates that fail would otherwise run the whole budget.  The probes
+        run on the context's GPU; only the verdicts come back.
 
         Two passes: the given probes (the training set, optionally with more
         random points), then -- when `refine` > 0 -- `refine` further random
@@ -193,7 +198,7 @@ class Surrogate:
         returns (mask, ambiguous probes of the first pass)."""
         T, mu, xi = _d(T), _d(mu), _d(xi)
         nb = np.asarray(bins)
-        amb = self._ambiguous(ctx, T, mu, xi, plain_roots, p_num, t_num, opts, multi_root)
+        amb = self._ambiguous(ctx, T, mu, xi, plain_roots, p_num, t_num, opts, multi_root, multi_max_iter)
         mask = np.zeros(tuple(nb), dtype=np.uint8)
         b = self._bins(T, mu, xi, nb)
         mask[b[amb, 0], b[amb, 1], b[amb, 2]] = 1
@@ -204,7 +209,8 @@ class Surrogate:
             pick = np.repeat(np.arange(cells.shape[0]), refine)
             u = (cells[pick] + rng.random((pick.size, 3))) / nb
             P = self.in_lo + u * (self.in_hi - self.in_lo)
-            amb2 = self._ambiguous(ctx, P[:, 0], P[:, 1], P[:, 2], None, p_num, t_num, opts, multi_root)
+            amb2 = self._ambiguous(ctx, P[:, 0], P[:, 1], P[:, 2], None, p_num, t_num, opts, multi_root,
+                                   multi_max_iter)
             b2 = self._bins(P[:, 0], P[:, 1], P[:, 2], nb)
             mask[b2[amb2, 0], b2[amb2, 1], b2[amb2, 2]] = 1
         mask = _dilate(mask, dilate, full=(neighbourhood == "full"))
@@ -216,46 +222,58 @@ class Surrogate:
         u = (np.stack([T, mu, xi], 1) - self.in_lo) / (self.in_hi - self.in_lo)
         return np.clip((u * nb).astype(int), 0, nb - 1)
 
-    def _ambiguous(self, ctx, T, mu, xi, plain_roots, p_num, t_num, opts, multi_root):
+    def _ambiguous(self, ctx, T, mu, xi, plain_roots, p_num, t_num, opts, multi_root, multi_max_iter=12):
         """Probes at which the learned-seed, DefaultSeed and MultiSeed solves do
         not all reach the same root (DualBranchScan's sameness test and Omega
-        to 1e-10), or one of them failed."""
-        from ._lib import SEED_DEFAULT, SEED_MULTI, SEED_SURROGATE
+        to 1e-10), or one of them failed: a numpy bool per probe.  The probes
+        and their solves stay in HBM."""
+        import torch
+        from ._lib import PT_FIRST_ORDER, SEED_DEFAULT, SEED_MULTI, SEED_MULTIROOT3, SEED_SURROGATE
+        o = opts if opts is not None else _lib.default_opts()
+        mo = type(o).from_buffer_copy(o)
+        if multi_max_iter:
+            mo.max_iter = min(int(o.max_iter), int(multi_max_iter))
+        dev = torch.device("cuda", ctx.device)
+        Td, mud, xid = (torch.tensor(_d(a), device=dev) for a in (T, mu, xi))
         saved = getattr(self, "judge_mask", None)
         ctx.set_judge_mask(None, None, None)
         try:
-            if plain_roots is None:
-                pr = ctx.solve_batch(T, mu, xi, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=opts)
-            else:
-                pr = plain_roots
-            seeds = self.infer(T, mu, xi)
-            q = ctx.solve_batch(T, mu, xi, policy=SEED_SURROGATE, seed=seeds, p_num=p_num, t_num=t_num, opts=opts)
-            m = ctx.solve_batch(T, mu, xi, policy=SEED_MULTI, p_num=p_num, t_num=t_num, opts=opts)
+            seeds = self.infer(Td, mud, xid)
+            q = ctx.solve_batch(Td, mud, xid, policy=SEED_SURROGATE, seed=seeds, p_num=p_num, t_num=t_num, opts=o)
+            m = ctx.solve_batch(Td, mud, xid, policy=SEED_MULTI, p_num=p_num, t_num=t_num, opts=mo)
             amb = (q["converged"] != 1) | (m["converged"] != 1) | ~same_root(q, m)
-            if isinstance(pr, dict):
+            if plain_roots is None:
+                pr = ctx.solve_batch(Td, mud, xid, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=o)
+                amb |= (pr["converged"] != 1) | ~same_root(pr, m)
+            elif isinstance(plain_roots, dict):
+                pr = {k: torch.as_tensor(plain_roots[k], device=dev) for k in ("x", "masses", "omega", "converged")}
                 amb |= (pr["converged"] != 1) | ~same_root(pr, m)
             else:
-                amb |= np.max(np.abs(_d(pr) - m["x"]) / np.maximum(1.0, np.abs(m["x"])), axis=1) > 1e-8
+                px = torch.as_tensor(_d(plain_roots), device=dev)
+                amb |= ((px - m["x"]).abs() / torch.clamp(m["x"].abs(), min=1.0)).amax(dim=1) > 1e-8
             if multi_root:
                 # also every probe where the HADRON and QUARK seeds reach distinct roots
-                from ._lib import PT_FIRST_ORDER, SEED_MULTIROOT3
-                r3 = ctx.solve_batch(T, mu, xi, policy=SEED_MULTIROOT3, seed=seeds, p_num=p_num, t_num=t_num,
-                                     opts=opts)
+                r3 = ctx.solve_batch(Td, mud, xid, policy=SEED_MULTIROOT3, seed=seeds, p_num=p_num, t_num=t_num,
+                                     opts=o)
                 amb |= (r3["flags"] & PT_FIRST_ORDER) != 0
+            return amb.cpu().numpy()
         finally:
             if saved is not None:
                 ctx.set_judge_mask(saved, self.in_lo, self.in_hi)
-        return amb
 
 
 def same_root(a, b, omega_rtol=1e-10):
-    """Per point: do solve results a and b (dicts with x, masses, omega) hold
-    the same root?  DualBranchScan's sameness test (|d phi_u| < 0.01, |d Phi|
-    < 0.01, |d M_u| < 0.01 fm^-1; DualBranchScan.jl:409-419) and Omega equal
-    to omega_rtol -- two Newton runs that stop within the solver's
-    tolerances of one root can differ in x by more than 1e-9 where the
-    Hessian is nearly singular (next to the critical end point), never in
-    Omega, which is stationary there."""
+    """Per point: do solve results a and b (dicts with x, masses, omega, as
+    numpy arrays or both as torch tensors) hold the same root?
+    DualBranchScan's sameness test (|d phi_u| < 0.01, |d Phi| < 0.01, |d M_u|
+    < 0.01 fm^-1; DualBranchScan.jl:409-419) and Omega equal to omega_rtol --
+    two Newton runs that stop within the solver's tolerances of one root can
+    differ in x by more than 1e-9 where the Hessian is nearly singular (next
+    to the critical end point), never in Omega, which is stationary there."""
+    if type(a["x"]).__module__.startswith("torch"):
+        return (((a["x"][:, 0] - b["x"][:, 0]).abs() < 0.01) & ((a["x"][:, 3] - b["x"][:, 3]).abs() < 0.01)
+                & ((a["masses"][:, 0] - b["masses"][:, 0]).abs() < 0.01)
+                & ((a["omega"] - b["omega"]).abs() <= omega_rtol * b["omega"].abs()))
     ax, bx = np.asarray(a["x"]), np.asarray(b["x"])
     am, bm = np.asarray(a["masses"]), np.asarray(b["masses"])
     ao, bo = np.asarray(a["omega"]), np.asarray(b["omega"])
@@ -278,6 +296,44 @@ def _dilate(mask, n, full=True):
                     g |= p[1 + d0:1 + d0 + nb[0], 1 + d1:1 + d1 + nb[1], 1 + d2:1 + d2 + nb[2]]
         m = g
     return m.astype(np.uint8)
+
+
+# launch geometry of the set-up solves (training targets, judge-mask probes):
+# fixed, so that the trained network does not depend on the geometry a caller
+# picked for its scans (lanes per point sets the order of the node sums, hence
+# the last bits of the targets, hence the training run)
+SETUP_LAUNCH = dict(lanes_per_point=8, chunk=1, waves_per_cu=16, tail_lanes_per_point=0)
+
+
+def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, steps=8000, batch=8192, lr=2e-3,
+                 n_probe=1_000_000, multi_max_iter=12):
+    """The learned seed as the scans use it, built on the context's GPU: plain
+    (DefaultSeed) roots at n_train points of the training box
+    (training_points, default_rng(4)), Adam on them (steps x batch, lr), and
+    the judge mask from those points plus n_probe random probes
+    (default_rng(5)).  Returns (Surrogate, info) with the set-up's figures;
+    the context's launch geometry is restored afterwards."""
+    import time
+    t0 = time.perf_counter()
+    saved = ctx.launch_cfg()
+    ctx.set_launch_cfg(**SETUP_LAUNCH)
+    try:
+        T, mu, xi = training_points(n_train, seed=4)
+        tr = ctx.solve_batch(T, mu, xi, policy=_lib.SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=opts)
+        ok = tr["converged"] == 1
+        s = Surrogate(ctx.device, init_seed=4)
+        s.fit_output_scaling(tr["x"][ok])
+        loss = s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=steps, batch=batch, lr=lr, seed=4)
+        train_ms = s.last_ms
+        Tp, mup, xip = training_points(n_probe, seed=5)
+        s.build_judge_mask(ctx, np.r_[T, Tp], np.r_[mu, mup], np.r_[xi, xip], p_num=p_num, t_num=t_num, opts=opts,
+                           multi_max_iter=multi_max_iter)
+    finally:
+        ctx.set_launch_cfg(**saved)
+    info = dict(train_points=int(ok.sum()), adam_steps=steps, batch=batch, lr=lr, train_kernel_ms=train_ms,
+                final_loss=float(np.mean(loss[-200:])), judge_probes=n_train + n_probe,
+                judge_mask_fraction=float(s.judge_mask.mean()), setup_s=time.perf_counter() - t0, loss=loss)
+    return s, info
 
 
 def training_points(n=200_000, seed=4):

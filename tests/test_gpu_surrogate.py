@@ -146,22 +146,18 @@ def test_learned_seed_policies_match_oracle(gpu_ctx, oracle, policy, judge):
 
 @pytest.fixture(scope="module")
 def c4_model(gpu_ctx):
-    """The learned seed as the scan uses it: 200 000 plain roots at
-    U([20,300] MeV x [0,450] MeV x [-0.4,0.4]) from default_rng(4), Adam lr
-    1e-3, batch 4096, 20 000 steps; judge mask from those roots plus 10^6
-    further random probes (default_rng(5))."""
+    """The learned seed as the scan and the bench build it
+    (surrogate.learned_seed): 200 000 plain roots at U([20,300] MeV x
+    [0,450] MeV x [-0.4,0.4]) from default_rng(4), Adam lr 2e-3, batch 8192,
+    8 000 steps; judge mask from those roots plus 10^6 further random probes
+    (default_rng(5))."""
     from julia_relaxtime_amd import _lib
-    from julia_relaxtime_amd.surrogate import Surrogate, training_points
+    from julia_relaxtime_amd.surrogate import learned_seed
     opts = _lib.default_opts(ftol=1e-10, max_iter=40)
-    T, mu, xi = training_points(200_000, seed=4)
-    tr = gpu_ctx.solve_batch(T, mu, xi, policy=0, p_num=64, t_num=4, opts=opts)
-    ok = tr["converged"] == 1
-    s = Surrogate(0, init_seed=4)
-    s.fit_output_scaling(tr["x"][ok])
-    loss = s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=20000, batch=4096, lr=1e-3, seed=4)
-    Tp, mup, xip = training_points(1_000_000, seed=5)
-    s.build_judge_mask(gpu_ctx, np.r_[T, Tp], np.r_[mu, mup], np.r_[xi, xip], opts=opts)
-    yield s, loss
+    s, info = learned_seed(gpu_ctx, p_num=64, t_num=4, opts=opts)
+    print(f"[c4 learned seed] set-up {info['setup_s']:.2f} s, mask fraction {info['judge_mask_fraction']:.4f}",
+          flush=True)
+    yield s, info["loss"]
     gpu_ctx.set_judge_mask(None, None, None)
 
 
