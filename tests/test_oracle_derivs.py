@@ -58,10 +58,10 @@ def test_derivatives_match_finite_differences(oracle, state):
 
 
 def test_bulk_viscosity_coefficients_match_reference_output(oracle):
-    """bulk_viscosity_xi0.0.csv rows whose root DefaultSeed reproduces (T = 150
-    MeV; the T = 160 MeV rows sit on a negative-mass root that the reference's
-    run reached and a DefaultSeed solve does not).  The stored masses use the
-    bare masses 5.5 and 140.0 MeV of ThermoDerivatives.jl:111-121."""
+    """bulk_viscosity_xi0.0.csv rows at T = 150 MeV (the T = 160 MeV rows hold
+    no root of the gap equations: test_reference_T160_rows_are_not_roots).
+    The stored masses use the bare masses 5.5 and 140.0 MeV of
+    ThermoDerivatives.jl:111-121."""
     rows = [r for r in golden_rows("bulk_viscosity_xi0.0.csv") if float(r["T_MeV"]) == 150.0]
     assert len(rows) == 2
     for row in rows:
@@ -110,10 +110,85 @@ def check_derivative_row(d, row):
 
 def test_derivatives_match_reference_output(oracle):
     """derivatives_xi0.0.csv rows at T = 150 MeV (p24/t8); its T = 160 MeV rows
-    sit on the same negative-mass root as the bulk-viscosity file's."""
+    hold no root of the gap equations (test_reference_T160_rows_are_not_roots)."""
     rows = [r for r in golden_rows("derivatives_xi0.0.csv") if float(r["T_MeV"]) == 150.0]
     assert len(rows) == 2
     for row in rows:
         T, mu = float(row["T_MeV"]) / HB, float(row["mu_MeV"]) / HB
         r = _solve(oracle, T, mu, 0.0)
         check_derivative_row(oracle.thermo_derivs(r["x"][0], T, mu, 0.0, 24, 8), row)
+
+
+def _condensates_from_masses(Mu, Ms):
+    """(phi_u = phi_d, phi_s) with M_u = m_u0 - 4 G phi_u + 2 K phi_u phi_s and
+    M_s = m_s0 - 4 G phi_s + 2 K phi_u^2 (fm^-1), the bare masses 5.5 and
+    140.0 MeV with which ThermoDerivatives.jl:111-121 writes the masses: every
+    solution found from a grid of starts."""
+    from scipy.optimize import fsolve
+    L = 602.3 / HB
+    G, K, m0, ms0 = 1.835 / L ** 2, 12.36 / L ** 5, 5.5 / HB, 140.0 / HB
+    f = lambda v: [m0 - 4 * G * v[0] + 2 * K * v[0] * v[1] - Mu, ms0 - 4 * G * v[1] + 2 * K * v[0] ** 2 - Ms]
+    sols = {}
+    for a in np.linspace(-12, 12, 13):
+        for b in np.linspace(-12, 12, 13):
+            v, _, ier, _ = fsolve(f, [a, b], full_output=True)
+            if ier == 1 and max(abs(np.array(f(v)))) < 1e-12:
+                sols[(round(v[0], 6), round(v[1], 6))] = v
+    return list(sols.values())
+
+
+def _best_gradient(oracle, pu, ps, T, mu, n=25):
+    """min over (Phi, Phib) in [0, 1.2]^2 (Phi = Phib at mu = 0) of
+    max |dOmega/dx| at (pu, pu, ps, Phi, Phib): a grid, then a bounded local
+    minimisation of |dOmega/dx|^2 from the three best grid points"""
+    from scipy.optimize import minimize
+    g2 = lambda P: oracle.omega_derivs(np.array([pu, pu, ps, P[0], P[-1]]), np.full(3, mu), T, 0.0, 24, 8)[1]
+    pts = [(a, b) for a in np.linspace(0.0, 1.2, n) for b in (np.linspace(0.0, 1.2, n) if mu > 0 else [a])]
+    vals = sorted((float(np.max(np.abs(g2(P)))), P) for P in pts)
+    best = vals[0][0]
+    dim = 2 if mu > 0 else 1
+    for _, P in vals[:3]:
+        r = minimize(lambda q: float(np.sum(g2(q) ** 2)), np.array(P[:dim]), method="L-BFGS-B",
+                     bounds=[(0.0, 1.2)] * dim, options={"ftol": 1e-30, "gtol": 1e-14, "maxiter": 500})
+        best = min(best, float(np.max(np.abs(g2(r.x)))))
+    return best
+
+
+@pytest.mark.parametrize("name", ["derivatives_xi0.0.csv", "bulk_viscosity_xi0.0.csv"])
+def test_reference_T160_rows_are_not_roots(oracle, name):
+    """The T = 160 MeV rows of the reference's derivative outputs (M_u = -636.6
+    MeV, M_s = 367.5 MeV, reported converged) hold no root of the gap
+    equations: the condensates that reproduce their stored masses leave
+    max |dOmega/dx| >= 1 for every Polyakov loop (Phi, Phib) in [0, 1.2]^2,
+    and a seed sweep over phi_u in [-9, 9], phi_s in [-6, 6] finds only the
+    M_u = 358.9 and 650.3 MeV roots at (160 MeV, 0).  The same inversion on
+    the T = 150 MeV rows lands on a root (max |dOmega/dx| below the 6-decimal
+    rounding of the stored masses) with the stored pressure -- so the
+    reference's run wrote a state its solver flagged converged without
+    solving the equations there, and those rows cannot be reproduced by any
+    root (the rows of T = 150 MeV are checked in full above)."""
+    for row in golden_rows(name):
+        Tm, mm = float(row["T_MeV"]), float(row["mu_MeV"])
+        T, mu = Tm / HB, mm / HB
+        sols = _condensates_from_masses(float(row["M_u_MeV"]) / HB, float(row["M_s_MeV"]) / HB)
+        assert sols
+        g = min(_best_gradient(oracle, v[0], v[1], T, mu) for v in sols)
+        print(f"[{name} T={Tm} mu={mm}] {len(sols)} condensate pairs from the stored masses, "
+              f"min max|dOmega/dx| over Phi, Phib: {g:.3e}", flush=True)
+        if Tm == 160.0:
+            assert g > 1.0
+        else:
+            assert Tm == 150.0
+            assert g < 1e-5          # the stored masses' 6 decimals
+    # T = 150, mu = 0: the physical pair refined in Phi reaches the stored root and pressure
+    row = [r for r in golden_rows("derivatives_xi0.0.csv") if float(r["T_MeV"]) == 150 and float(r["mu_MeV"]) == 0][0]
+    from scipy.optimize import minimize_scalar
+    T = 150.0 / HB
+    pu, ps = [v for v in _condensates_from_masses(float(row["M_u_MeV"]) / HB, float(row["M_s_MeV"]) / HB)
+              if v[0] < 0 and v[1] < 0][0]
+    gn = lambda P: float(np.max(np.abs(oracle.omega_derivs(np.array([pu, pu, ps, P, P]), np.zeros(3), T, 0.0,
+                                                             24, 8)[1])))
+    r = minimize_scalar(gn, bounds=(0.0, 0.2), method="bounded", options={"xatol": 1e-12})
+    assert r.fun < 1e-6
+    th = oracle.thermo(np.array([pu, pu, ps, r.x, r.x]), T, 0.0, 0.0, 24, 8)
+    assert abs(th["pressure"] - float(row["pressure_fm4"])) < 1e-5
