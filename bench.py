@@ -12,12 +12,16 @@ Workloads (grids are linspace with endpoints; MeV -> fm^-1 with 197.327):
                   iterations 40.  One step = MLP inference of the seeds
                   (BF16 MFMA) + the learned-seed scan (8-step cutoff,
                   DefaultSeed fallback, reference judge in the learned
-                  ambiguity mask).  Set-up, untimed and reported: 200,000
-                  plain roots at U([20,300] x [0,450] MeV x [-0.4,0.4])
-                  (default_rng(4)), Adam lr 1e-3 batch 4096 20,000 steps on
-                  the GPU, judge mask from those and 10^6 more probes
-                  (default_rng(5)).  The plain DefaultSeed scan of the same
-                  grid is timed with the same K/W and reported beside it.
+                  ambiguity mask).  Set-up (surrogate.learned_seed),
+                  untimed and reported: 200,000 plain roots at
+                  U([20,300] x [0,450] MeV x [-0.4,0.4]) (default_rng(4)),
+                  Adam lr 2e-3 batch 8192 8,000 steps on the GPU, judge mask
+                  from those and 10^6 more probes (default_rng(5)).  The plain
+                  DefaultSeed scan of the same grid is timed with the same K/W
+                  and reported beside it.  The default run also times every
+                  other workload below (child processes, 3 steps after 1
+                  warm-up, no CPU leg) into config.other_workloads
+                  (--no-others skips them).
   c5_multiroot    512 x 512 x 5: T 10-250 MeV, mu 200-450 MeV, xi -0.2..0.2
                   step 0.1 = 1,310,720 points, p = 64, t = 4, iterations 40.
                   One step = MLP inference + MULTIROOT3 (learned seed, HADRON
@@ -847,6 +851,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c4_surrogate", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-others", action="store_true",
+                    help="default workload only: skip timing the other workloads (child processes)")
     ap.add_argument("--launch", default=None,
                     help="lanes,chunk,waves_per_cu[,tail_lanes] for both scans instead of autotuning "
                          "(profiling runs)")
@@ -1019,7 +1025,42 @@ def main():
         line["config"]["copy_probe_bytes"] = int(src.numel() * 4)
     if not args.no_cpu:
         line["cpu_baseline"] = dict(cpu_baseline(w, T, mu, xi), **host_cpu())
+    if args.workload == "c4_surrogate" and not args.profile and not args.no_others:
+        line["config"]["other_workloads"] = other_workloads()
     print(json.dumps(line), flush=True)
+
+
+# the rest of the workloads, timed in the default run too (each a child
+# process: 3 timed steps after 1 warm-up, no CPU leg); their figures go into
+# the default line's config, so that one run records all of them.  The scan
+# workloads take the launch geometry the autotuner picks for them (its search
+# over the 1.3 M-point C5 grid alone takes about a minute).
+OTHER_WORKLOADS = {"c5_multiroot": "8,1,16,0", "c3_polarization": None, "propagators": None, "meson_masses": None,
+                   "dual_branch": None, "maxwell_line": None, "tmu_scan": None, "c1_continuity": "32,4,16,0"}
+
+
+def other_workloads(timeout_s=150):
+    import subprocess
+    out = {}
+    for name, launch in OTHER_WORKLOADS.items():
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", "3", "--warmup", "1",
+               "--no-cpu"] + (["--launch", launch] if launch else [])
+        t0 = time.perf_counter()
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s)
+            rows = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+            if p.returncode != 0 or not rows:
+                out[name] = dict(error=f"exit {p.returncode}", stderr_tail=p.stderr[-300:])
+                continue
+            d = json.loads(rows[-1])
+            r = d.get("roofline") or {}
+            out[name] = dict(metric=d["metric"], value=d["value"], unit=d["unit"], ms_per_step=d["ms_per_step"],
+                             steps=d["steps"], roofline_frac=r.get("frac"), kernel=r.get("kernel"),
+                             kernel_ms_per_step=r.get("kernel_ms_per_step"), wall_s=time.perf_counter() - t0)
+        except subprocess.TimeoutExpired:
+            out[name] = dict(error=f"timeout after {timeout_s} s")
+        print(f"bench: {name} {out[name].get('value', out[name].get('error'))}", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":
