@@ -15,8 +15,11 @@ Replaces:
   solve_with_derivatives(T_fm, mu_fm; order)       src/pnjl/solver/ImplicitSolver.jl:660-701
   calculate_propagator_by_channel(process, channel, ...)  src/relaxtime/TotalPropagator.jl:894-963
   solve_meson_mass(meson, quark_params, thermo_params; ...)  src/relaxtime/MesonMass.jl:241-276
+and adds the learned seed (no counterpart in the reference): Surrogate, learned_seed, SurrogateSeed.
 """
 module PNJLHip
+
+using Random
 
 const LIB = get(ENV, "PNJL_HIP_LIB", joinpath(@__DIR__, "..", "libpnjl_hip.so"))
 
@@ -202,6 +205,13 @@ function solve(mode, T_fm::Real, mu_fm::Real; xi::Real=0.0, seed_strategy=nothin
     fallback_method === :trust_region || throw(ArgumentError("the HIP backend falls back to :trust_region"))
     physicality_check === nothing || nameof(physicality_check) === :_default_is_physical_solution ||
         throw(ArgumentError("the HIP backend applies the default physicality check"))
+    seed_strategy isa SurrogateSeed && return solve_learned(mode, [Float64(T_fm)], [Float64(mu_fm)], seed_strategy;
+                                                            xi=[Float64(xi)], p_num=p_num, t_num=t_num,
+                                                            iterations=iterations, make_result=make_result,
+                                                            ctx=ctx, trust_region_fallback=trust_region_fallback,
+                                                            auto_multiseed_fallback=auto_multiseed_fallback,
+                                                            residual_norm_max=Float64(residual_norm_max),
+                                                            xtol=Float64(xtol), ftol=Float64(ftol))[1]
     policy, x0 = policy_for(seed_strategy, T_fm, mu_fm, mode, get_seed)
     r = solve_batch([Float64(T_fm)], [Float64(mu_fm)]; xi=[Float64(xi)], policy=policy,
                     seeds=x0 === nothing ? nothing : reshape(x0, 5, 1), p_num=p_num, t_num=t_num,
@@ -231,6 +241,9 @@ function solve_many(mode, T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{F
                     t_num::Integer=8, iterations::Integer=1000, make_result=SolverResult,
                     ctx::Context=default_context(), kwargs...)
     nameof(typeof(mode)) === :FixedMu || throw(ArgumentError("the HIP backend solves FixedMu() only"))
+    seed_strategy isa SurrogateSeed && return solve_learned(mode, T_fm, mu_fm, seed_strategy; xi=xi, p_num=p_num,
+                                                            t_num=t_num, iterations=iterations,
+                                                            make_result=make_result, ctx=ctx, kwargs...)
     policy, x0 = policy_for(seed_strategy, T_fm[1], mu_fm[1], mode, nothing)
     x0 === nothing || throw(ArgumentError("solve_many: point-independent strategies only"))
     r = solve_batch(T_fm, mu_fm; xi=xi, policy=policy, p_num=p_num, t_num=t_num, iterations=iterations,
@@ -431,6 +444,270 @@ function polarization_aniso(channel::Symbol, k0, k_norm, m1, m2, mu1, mu2, T, Ph
                ctx.h, 1, q, re, im, 0, C_NULL)
     check(ctx, rc, "pnjl_polarization_batch")
     return re[], im[]
+end
+
+# ---------------------------------------------------------------------------
+# Learned seed (no counterpart in the reference): the MLP seed model of
+# pnjl_surrogate_*, its judge mask, and SurrogateSeed, the seed strategy that
+# runs PNJL_SEED_SURROGATE / PNJL_SEED_MULTIROOT3.  The reference seeds stay
+# the fallback and the judge (SeedStrategies.jl:204-284).  Same recipe as the
+# Python host's surrogate.learned_seed (training box, Adam steps, judge-mask
+# probes); the random points come from Julia's RNG, so the trained network
+# is not the Python one's bit for bit.
+# ---------------------------------------------------------------------------
+const JUDGE_REFERENCE, JUDGE_OMEGA = 0, 1
+const PT_SURR_FALLBACK, PT_SURR_OMEGA, PT_FIRST_ORDER, PT_SURR_MASKED = 0x10, 0x20, 0x40, 0x100
+const DOMAIN_LO = [20.0 / 197.327, 0.0, -0.4]                    # T, mu (fm^-1), xi of the training box
+const DOMAIN_HI = [300.0 / 197.327, 450.0 / 197.327, 0.4]
+
+struct LaunchCfg
+    lanes_per_point::Cint
+    chunk::Cint
+    waves_per_cu::Cint
+    tail_lanes_per_point::Cint
+end
+# the set-up solves' fixed geometry (the trained network must not depend on the scans' geometry)
+const SETUP_LAUNCH = LaunchCfg(8, 1, 16, 0)
+
+function launch_cfg(ctx::Context=default_context())
+    c = Ref(LaunchCfg(0, 0, 0, 0))
+    check(ctx, ccall((:pnjl_get_launch_cfg, LIB), Cint, (Ptr{Cvoid}, Ref{LaunchCfg}), ctx.h, c),
+          "pnjl_get_launch_cfg")
+    return c[]
+end
+
+set_launch_cfg!(cfg::LaunchCfg, ctx::Context=default_context()) =
+    check(ctx, ccall((:pnjl_set_launch_cfg, LIB), Cint, (Ptr{Cvoid}, Ref{LaunchCfg}), ctx.h, cfg),
+          "pnjl_set_launch_cfg")
+
+mutable struct Surrogate
+    h::Ptr{Cvoid}
+    in_lo::Vector{Float64}
+    in_hi::Vector{Float64}
+    out_mean::Vector{Float64}
+    out_std::Vector{Float64}
+    judge_mask::Union{Nothing,Array{UInt8,3}}     # [nxi, nmu, nT]: the C layout (T slowest) in column-major
+end
+
+function scheck(s::Surrogate, rc::Cint, what)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:pnjl_surrogate_last_error, LIB), Cstring, (Ptr{Cvoid},), s.h))
+    error("$what failed with status $rc: $msg")
+end
+
+"""A GPU-resident MLP seed model (3 -> 128 -> 128 -> 128 -> 4, BF16 MFMA, FP32 Adam)."""
+function Surrogate(device::Integer=0; init_seed::Integer=4)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pnjl_surrogate_create, LIB), Cint, (Cint, UInt64, Ref{Ptr{Cvoid}}), device, init_seed, h)
+    rc == 0 || error("pnjl_surrogate_create failed with status $rc")
+    s = Surrogate(h[], copy(DOMAIN_LO), copy(DOMAIN_HI), zeros(4), ones(4), nothing)
+    finalizer(x -> ccall((:pnjl_surrogate_destroy, LIB), Cvoid, (Ptr{Cvoid},), x.h), s)
+    set_scaling!(s)
+    return s
+end
+
+set_scaling!(s::Surrogate) =
+    scheck(s, ccall((:pnjl_surrogate_set_scaling, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                    s.h, s.in_lo, s.in_hi, s.out_mean, s.out_std), "pnjl_surrogate_set_scaling")
+
+"""Output normalisation from training roots (5 x n): mean and std of (phi_u, phi_s, Phi, Phibar)."""
+function fit_output_scaling!(s::Surrogate, roots::AbstractMatrix{Float64})
+    y = roots[[1, 3, 4, 5], :]
+    n = size(y, 2)
+    s.out_mean = vec(sum(y; dims=2)) ./ n
+    s.out_std = max.(sqrt.(vec(sum((y .- s.out_mean) .^ 2; dims=2)) ./ n), 1e-6)
+    set_scaling!(s)
+end
+
+param_count() = Int(ccall((:pnjl_surrogate_param_count, LIB), Cint, ()))
+
+function params(s::Surrogate)
+    p = zeros(Float32, param_count())
+    scheck(s, ccall((:pnjl_surrogate_get_params, LIB), Cint, (Ptr{Cvoid}, Ptr{Float32}), s.h, p),
+           "pnjl_surrogate_get_params")
+    return p
+end
+
+function params!(s::Surrogate, p::Vector{Float32})
+    length(p) == param_count() || throw(ArgumentError("expected $(param_count()) parameters"))
+    scheck(s, ccall((:pnjl_surrogate_set_params, LIB), Cint, (Ptr{Cvoid}, Ptr{Float32}), s.h, p),
+           "pnjl_surrogate_set_params")
+end
+
+"""Adam on the GPU over (T, mu, xi) -> roots (5 x n); returns the per-step loss."""
+function train!(s::Surrogate, T_fm::Vector{Float64}, mu_fm::Vector{Float64}, xi::Vector{Float64},
+                roots::Matrix{Float64}; steps::Integer=8000, batch::Integer=8192, lr::Real=2e-3, seed::Integer=4)
+    loss = zeros(Float32, max(steps, 1))
+    scheck(s, ccall((:pnjl_surrogate_train, LIB), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Cint,
+                     Float32, UInt64, Ptr{Float32}, Cint, Ptr{Cvoid}),
+                    s.h, length(T_fm), T_fm, mu_fm, xi, roots, steps, batch, Float32(lr), seed, loss, 0, C_NULL),
+           "pnjl_surrogate_train")
+    return loss[1:steps]
+end
+
+"""Seeds (5 x n: phi_u, phi_u, phi_s, Phi, Phibar) at (T, mu, xi)."""
+function infer(s::Surrogate, T_fm::Vector{Float64}, mu_fm::Vector{Float64}, xi::Vector{Float64})
+    seeds = zeros(5, length(T_fm))
+    scheck(s, ccall((:pnjl_surrogate_infer, LIB), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+                    s.h, length(T_fm), T_fm, mu_fm, xi, seeds, 0, C_NULL), "pnjl_surrogate_infer")
+    return seeds
+end
+
+"""Installs the model's judge mask in the context (nothing clears it)."""
+function set_judge_mask!(ctx::Context, s::Union{Nothing,Surrogate})
+    if s === nothing || s.judge_mask === nothing
+        rc = ccall((:pnjl_set_judge_mask, LIB), Cint,
+                   (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{UInt8}),
+                   ctx.h, 0, 0, 0, C_NULL, C_NULL, C_NULL)
+    else
+        nxi, nmu, nT = size(s.judge_mask)
+        rc = ccall((:pnjl_set_judge_mask, LIB), Cint,
+                   (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{UInt8}),
+                   ctx.h, nT, nmu, nxi, s.in_lo, s.in_hi, s.judge_mask)
+    end
+    check(ctx, rc, "pnjl_set_judge_mask")
+end
+
+# DualBranchScan's sameness test (DualBranchScan.jl:409-419) and Omega to 1e-10
+same_root(a, b, i) = abs(a.solution[1, i] - b.solution[1, i]) < 0.01 && abs(a.solution[4, i] - b.solution[4, i]) < 0.01 &&
+    abs(a.masses[1, i] - b.masses[1, i]) < 0.01 && abs(a.omega[i] - b.omega[i]) <= 1e-10 * abs(b.omega[i])
+
+# probes at which the learned-seed, DefaultSeed and MultiSeed roots are not all one root, or a solve failed
+function ambiguous(s::Surrogate, ctx, T, mu, xi; p_num, t_num, iterations, ftol, multi_iterations)
+    set_judge_mask!(ctx, nothing)
+    kw = (xi=xi, p_num=p_num, t_num=t_num, ftol=ftol, ctx=ctx)
+    q = solve_batch(T, mu; policy=SEED_SURROGATE, seeds=infer(s, T, mu, xi), iterations=iterations, kw...)
+    m = solve_batch(T, mu; policy=SEED_MULTI, iterations=min(iterations, multi_iterations), kw...)
+    pr = solve_batch(T, mu; policy=SEED_DEFAULT, iterations=iterations, kw...)
+    return [!(q.converged[i] && m.converged[i] && pr.converged[i] && same_root(q, m, i) && same_root(pr, m, i))
+            for i in eachindex(T)]
+end
+
+bin_of(s::Surrogate, T, mu, xi, nb) = ntuple(d -> clamp(floor(Int, (((T, mu, xi)[d] - s.in_lo[d]) /
+                                                                   (s.in_hi[d] - s.in_lo[d])) * nb[d]) + 1,
+                                                      1, nb[d]), 3)
+
+# n steps of binary dilation over the 26 neighbours; mask indexed [T, mu, xi]
+function dilate(mask::BitArray{3}, n::Integer)
+    for _ in 1:n
+        g = copy(mask)
+        for I in findall(mask), d in CartesianIndices((-1:1, -1:1, -1:1))
+            J = I + d
+            checkbounds(Bool, g, J) && (g[J] = true)
+        end
+        mask = g
+    end
+    return mask
+end
+
+"""
+    build_judge_mask!(s, ctx, T, mu, xi; bins=(128, 128, 16), refine=8, dilate_bins=1, multi_iterations=12, ...)
+
+The judge's ambiguity mask, learned on probe points: a bin is set where the
+learned-seed, plain (DefaultSeed) and MultiSeed (argmin-Omega) roots are not
+all one root, or a solve failed; a second pass puts `refine` random probes in
+every bin within two bins of a set one; the set bins are dilated over their
+26 neighbours.  Stored in `s.judge_mask` and installed in `ctx`.
+"""
+function build_judge_mask!(s::Surrogate, ctx::Context, T, mu, xi; bins=(128, 128, 16), refine::Integer=8,
+                           dilate_bins::Integer=1, multi_iterations::Integer=12, p_num::Integer=64,
+                           t_num::Integer=4, iterations::Integer=40, ftol::Real=1e-10, rng=Random.Xoshiro(6))
+    kw = (p_num=p_num, t_num=t_num, iterations=iterations, ftol=Float64(ftol), multi_iterations=multi_iterations)
+    mask = falses(bins...)
+    for i in findall(ambiguous(s, ctx, T, mu, xi; kw...))
+        mask[bin_of(s, T[i], mu[i], xi[i], bins)...] = true
+    end
+    if refine > 0 && any(mask)
+        cells = findall(dilate(mask, 2))
+        P = Matrix{Float64}(undef, length(cells) * refine, 3)
+        k = 0
+        for c in cells, _ in 1:refine
+            k += 1
+            for d in 1:3
+                P[k, d] = s.in_lo[d] + ((c[d] - 1) + rand(rng)) / bins[d] * (s.in_hi[d] - s.in_lo[d])
+            end
+        end
+        Tp, mup, xip = P[:, 1], P[:, 2], P[:, 3]
+        for i in findall(ambiguous(s, ctx, Tp, mup, xip; kw...))
+            mask[bin_of(s, Tp[i], mup[i], xip[i], bins)...] = true
+        end
+    end
+    mask = dilate(mask, dilate_bins)
+    s.judge_mask = permutedims(UInt8.(mask), (3, 2, 1))     # the C layout: [T][mu][xi], xi fastest
+    set_judge_mask!(ctx, s)
+    return s
+end
+
+training_points(n, rng) = ([DOMAIN_LO[d] + rand(rng) * (DOMAIN_HI[d] - DOMAIN_LO[d]) for _ in 1:n] for d in 1:3)
+
+"""
+    learned_seed(ctx; device=0, p_num=64, t_num=4, n_train=200_000, steps=8000, batch=8192, lr=2e-3,
+                 n_probe=1_000_000) -> Surrogate
+
+The learned seed as the scans use it: plain (DefaultSeed) roots at n_train
+points of the training box, Adam on them, and the judge mask from those
+points plus n_probe random probes; installed in `ctx` (`device` is the
+context's GPU).
+"""
+function learned_seed(ctx::Context=default_context(); device::Integer=0, p_num::Integer=64, t_num::Integer=4,
+                      iterations::Integer=40, ftol::Real=1e-10, n_train::Integer=200_000, steps::Integer=8000,
+                      batch::Integer=8192, lr::Real=2e-3, n_probe::Integer=1_000_000, rng=Random.Xoshiro(4))
+    saved = launch_cfg(ctx)
+    set_launch_cfg!(SETUP_LAUNCH, ctx)
+    try
+        T, mu, xi = training_points(n_train, rng)
+        tr = solve_batch(T, mu; xi=xi, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, iterations=iterations,
+                         ftol=Float64(ftol), ctx=ctx)
+        ok = findall(tr.converged)
+        s = Surrogate(device)
+        fit_output_scaling!(s, tr.solution[:, ok])
+        train!(s, T[ok], mu[ok], xi[ok], tr.solution[:, ok]; steps=steps, batch=batch, lr=lr)
+        Tp, mup, xip = training_points(n_probe, rng)
+        build_judge_mask!(s, ctx, vcat(T, Tp), vcat(mu, mup), vcat(xi, xip); p_num=p_num, t_num=t_num,
+                          iterations=iterations, ftol=ftol)
+        return s
+    finally
+        set_launch_cfg!(saved, ctx)
+    end
+end
+
+"""
+    SurrogateSeed(model; max_steps=8, judge=:reference)
+
+Seed strategy of the learned seed for `solve` and `solve_many`
+(`seed_strategy=SurrogateSeed(model)`): Newton from the model's
+seed for at most `max_steps` steps, the DefaultSeed policy where that fails;
+inside the model's judge mask `judge=:reference` solves the point with
+DefaultSeed unseen, `:omega` keeps the minimum-Omega root among the learned
+root and the MultiSeed candidates; `judge=:multiroot3` keeps the minimum-Omega
+root of the learned, HADRON and QUARK seeds at every point.
+"""
+struct SurrogateSeed
+    model::Surrogate
+    max_steps::Int
+    judge::Symbol
+end
+SurrogateSeed(model::Surrogate; max_steps::Integer=8, judge::Symbol=:reference) =
+    SurrogateSeed(model, max_steps, judge)
+
+function solve_learned(mode, T_fm::AbstractVector{Float64}, mu_fm::AbstractVector{Float64}, s::SurrogateSeed;
+                       xi::AbstractVector{Float64}=zeros(length(T_fm)), p_num::Integer=64, t_num::Integer=8,
+                       iterations::Integer=1000, make_result=SolverResult, ctx::Context=default_context(),
+                       kwargs...)
+    nameof(typeof(mode)) === :FixedMu || throw(ArgumentError("the HIP backend solves FixedMu() only"))
+    s.judge in (:reference, :omega, :multiroot3) || throw(ArgumentError("judge: :reference, :omega or :multiroot3"))
+    set_judge_mask!(ctx, s.model)
+    seeds = infer(s.model, Vector{Float64}(T_fm), Vector{Float64}(mu_fm), Vector{Float64}(xi))
+    r = solve_batch(T_fm, mu_fm; xi=xi, policy=s.judge === :multiroot3 ? SEED_MULTIROOT3 : SEED_SURROGATE,
+                    seeds=seeds, p_num=p_num, t_num=t_num, iterations=iterations, surrogate_max_steps=s.max_steps,
+                    surrogate_judge=s.judge === :omega ? JUDGE_OMEGA : JUDGE_REFERENCE, ctx=ctx, kwargs...)
+    return [make_result(mode, r.converged[i], r.solution[:, i], Tuple(r.solution[:, i]),
+                        (mu_fm[i], mu_fm[i], mu_fm[i]), r.omega[i], r.pressure[i], r.rho_norm[i], r.entropy[i],
+                        r.energy[i], Tuple(r.masses[:, i]), Int(r.iterations[i]), r.residual_norm[i], xi[i])
+            for i in eachindex(T_fm)]
 end
 
 end # module PNJLHip

@@ -17,9 +17,10 @@ JL = os.path.join(ROOT, "julia_relaxtime_amd", "julia", "PNJLHip.jl")
 
 # (Julia struct, C struct)
 STRUCTS = [("SolveOpts", "pnjl_solve_opts"), ("SolveOut", "pnjl_solve_out"), ("PolQuery", "pnjl_pol_query"),
-           ("PropQuery", "pnjl_prop_query"), ("MassQuery", "pnjl_mass_query")]
-C_SCALAR = {"double": "Float64", "int": "Cint", "int32_t": "Cint", "int64_t": "Int64"}
-SIZE = {"Float64": 8, "Cint": 4, "Int64": 8}
+           ("PropQuery", "pnjl_prop_query"), ("MassQuery", "pnjl_mass_query"), ("LaunchCfg", "pnjl_launch_cfg")]
+C_SCALAR = {"double": "Float64", "int": "Cint", "int32_t": "Cint", "int64_t": "Int64", "uint64_t": "UInt64",
+            "float": "Float32"}
+SIZE = {"Float64": 8, "Cint": 4, "Int64": 8, "UInt64": 8, "Float32": 4}
 
 
 def _strip_c_comments(s):
@@ -130,7 +131,8 @@ def c_prototypes():
 
 
 JL_STRUCT_OF = {"pnjl_solve_opts": "SolveOpts", "pnjl_solve_out": "SolveOut", "pnjl_pol_query": "PolQuery",
-                "pnjl_prop_query": "PropQuery", "pnjl_mass_query": "MassQuery", "pnjl_model_params": None}
+                "pnjl_prop_query": "PropQuery", "pnjl_mass_query": "MassQuery", "pnjl_launch_cfg": "LaunchCfg",
+                "pnjl_model_params": None}
 
 
 def _split_top(s):
