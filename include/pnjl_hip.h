@@ -217,7 +217,7 @@ int pnjl_solve_fixedmu_batch(pnjl_ctx* ctx, int64_t n,
  * every array is a device pointer (flags must hold PNJL_FLAG_DEVICE_PTRS)
  * and n <= 8192 points per call (larger n: PNJL_ERR_ARG).  The node table,
  * work-queue counters, candidate records and per-point lists live in one
- * fixed scratch array of ~41 MB per GPU, held from the library's load until
+ * fixed scratch array of ~26 MB per GPU, held from the library's load until
  * the process ends; its address is looked up once per device by
  * pnjl_ctx_create.  Every call on a GPU shares that GPU's array, so calls
  * that overlap in time on one GPU race (serialise them on one stream).  A

@@ -26,8 +26,7 @@ struct pnjl_ctx {
     std::string err;
     pnjl_launch_cfg cfg{4, 4, 8, 0};
     std::map<std::pair<int, int>, double*> tabs;
-    double* scratch = nullptr;          // per-group candidate records of the scan kernel
-    size_t scratch_bytes = 0;
+    size_t lds_block = 64 * 1024;       // LDS budget of one scan-kernel block (bytes)
     unsigned long long* queue = nullptr;
     double* bnd = nullptr;
     int nb = 0;

@@ -166,17 +166,6 @@ int get_tab(pnjl_ctx* ctx, int pn, int tn, double** out)
     return PNJL_OK;
 }
 
-int ensure_scratch(pnjl_ctx* ctx, size_t bytes)
-{
-    if (ctx->scratch_bytes >= bytes) return PNJL_OK;
-    if (ctx->scratch) HIPCHK(ctx, hipFree(ctx->scratch));
-    ctx->scratch = nullptr;
-    ctx->scratch_bytes = 0;
-    HIPCHK(ctx, hipMalloc(&ctx->scratch, bytes));
-    ctx->scratch_bytes = bytes;
-    return PNJL_OK;
-}
-
 // staging slot k of at least `bytes`
 int stage_buf(pnjl_ctx* ctx, size_t k, size_t bytes, void** out)
 {
@@ -213,20 +202,21 @@ bool valid_lpp(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16 |
 
 // Block geometry of the scan kernel for lanes per point `lpp` and node table
 // (pn, tn): blocks of whole wavefronts holding at most 64 groups, with the
-// node table and every group's LDS state within 64 KB; a lane count whose
-// groups do not fit next to a large table is raised to the next power of
-// two (a different summation order, equal to rounding).
+// node table and every group's LDS state (hot state and candidate records)
+// within the block's budget lds_block (80 KB on a 160 KB-LDS CU); a lane
+// count whose groups do not fit next to a large table is raised to the next
+// power of two (a different summation order, equal to rounding).
 struct ScanGeom {
     int lpp, threads, groups_per_block;
     size_t lds;
 };
 
-ScanGeom scan_geom(int lpp, int pn, int tn)
+ScanGeom scan_geom(int lpp, int pn, int tn, size_t lds_block)
 {
     const size_t tab = (size_t)pnjl::table_len(pn, tn) * sizeof(double);
     const size_t group = (size_t)pnjl::STATE_STRIDE * sizeof(double);
-    // 64 KB per block, less the block's copy of the launch arguments (static LDS)
-    const size_t limit = 64 * 1024 - (sizeof(pnjl::ScanArgs) + 255) / 256 * 256;
+    // the block's LDS budget, less its copy of the launch arguments (static LDS)
+    const size_t limit = lds_block - (sizeof(pnjl::ScanArgs) + 255) / 256 * 256;
     for (int l = lpp; l <= 64; l *= 2) {
         for (int waves = 4; waves >= 1; waves--) {
             const int threads = 64 * waves;
@@ -237,6 +227,20 @@ ScanGeom scan_geom(int lpp, int pn, int tn)
         }
     }
     return ScanGeom{64, 64, 1, tab + group};
+}
+
+// Lets every scan_kernel instantiation take `bytes` of dynamic LDS.
+bool allow_scan_lds(int bytes)
+{
+    using namespace pnjl;
+    const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+    return hipFuncSetAttribute((const void*)scan_kernel<1>, a, bytes) == hipSuccess &&
+           hipFuncSetAttribute((const void*)scan_kernel<2>, a, bytes) == hipSuccess &&
+           hipFuncSetAttribute((const void*)scan_kernel<4>, a, bytes) == hipSuccess &&
+           hipFuncSetAttribute((const void*)scan_kernel<8>, a, bytes) == hipSuccess &&
+           hipFuncSetAttribute((const void*)scan_kernel<16>, a, bytes) == hipSuccess &&
+           hipFuncSetAttribute((const void*)scan_kernel<32>, a, bytes) == hipSuccess &&
+           hipFuncSetAttribute((const void*)scan_kernel<64>, a, bytes) == hipSuccess;
 }
 
 void launch_scan(pnjl_ctx* ctx, const pnjl::ScanArgs& A, const ScanGeom& G, int blocks, hipStream_t st)
@@ -263,13 +267,11 @@ void launch_scan(pnjl_ctx* ctx, const pnjl::ScanArgs& A, const ScanGeom& G, int 
 // outputs.  Calls of the no-alloc entry point on one GPU share it: calls that
 // overlap in time on one GPU race.
 constexpr int64_t kNaPoints = 8192;    // points per call
-constexpr int64_t kNaGroups = 16384;   // scan-kernel groups (candidate-record areas)
 constexpr size_t kNaTabLen = 12544;    // doubles >= table_len(pn, tn) for pn * tn <= 4096, pn <= 4096
 constexpr size_t na_round(size_t d) { return (d + 31) / 32 * 32; }   // 256-byte boundaries
 constexpr size_t kNaQueue = 0;                                                      // kQueueWords u64
 constexpr size_t kNaTab = na_round(kNaQueue + kQueueWords);
-constexpr size_t kNaRec = na_round(kNaTab + kNaTabLen);
-constexpr size_t kNaRecS = na_round(kNaRec + (size_t)kNaGroups * pnjl::REC_AREA);
+constexpr size_t kNaRecS = na_round(kNaTab + kNaTabLen);
 constexpr size_t kNaRecP = na_round(kNaRecS + (size_t)kNaPoints * pnjl::REC_SIZE);
 constexpr size_t kNaRecM = na_round(kNaRecP + (size_t)kNaPoints * pnjl::REC_SIZE);
 constexpr size_t kNaRecJ = na_round(kNaRecM + (size_t)kNaPoints * 6 * pnjl::REC_SIZE);
@@ -333,10 +335,10 @@ int check_policy(int p)
 }
 
 // core of pnjl_solve_fixedmu_batch with device pointers.  fixed: the node
-// table, queue words, candidate records and per-point lists live in the
-// device's no-alloc scratch (n <= kNaPoints, the grid capped at kNaGroups
-// groups) instead of buffers the context allocates; the lanes per point, and
-// with them every point's result, are the same either way.
+// table, queue words, the items' candidate records and the per-point lists
+// live in the device's no-alloc scratch (n <= kNaPoints) instead of buffers
+// the context allocates; the launch, and with it every point's result, is the
+// same either way.
 int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const double* xi, const double* seed,
               int policy, int pn, int tn, const pnjl_solve_opts* o, const pnjl_solve_out* out,
               const pnjl_launch_cfg& cfg, hipStream_t stream, bool fixed = false)
@@ -394,7 +396,7 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     A.bnd = ctx->bnd;
     A.nb = ctx->nb;
     A.max_trips = 9LL * (2LL * o->max_iter + 4) + 16;
-    const ScanGeom G = scan_geom(cfg.lanes_per_point, pn, tn);
+    const ScanGeom G = scan_geom(cfg.lanes_per_point, pn, tn, ctx->lds_block);
     const int waves_per_block = G.threads / 64;
     int blocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / waves_per_block);
     // never more groups than work items: a small batch gets a small grid
@@ -403,24 +405,10 @@ int solve_dev(pnjl_ctx* ctx, int64_t n, const double* T, const double* mu, const
     if ((int64_t)blocks * G.groups_per_block > need_groups)
         blocks = (int)std::max<int64_t>(1, (need_groups + G.groups_per_block - 1) / G.groups_per_block);
     // the later pipeline stages (few, long items) run the full persistent grid
-    // at their own lanes per point: records for the largest grid
-    const ScanGeom G2 = scan_geom(cfg.tail_lanes_per_point > 0 ? cfg.tail_lanes_per_point : G.lpp, pn, tn);
-    int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / (G2.threads / 64));
-    if (fixed) {
-        blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, kNaGroups / G.groups_per_block));
-        fullblocks = (int)std::max<int64_t>(1, std::min<int64_t>(fullblocks, kNaGroups / G2.groups_per_block));
-    }
-    const int64_t groups = std::max<int64_t>((int64_t)blocks * G.groups_per_block,
-                                             (int64_t)fullblocks * G2.groups_per_block);
-    A.n_groups = groups;
-    if (fixed) {
-        if (groups > kNaGroups) return set_err(ctx, PNJL_ERR_ARG, "no-alloc solve: launch grid beyond the scratch");
-        A.records = na + kNaRec;
-    } else {
-        rc = ensure_scratch(ctx, (size_t)groups * pnjl::REC_AREA * sizeof(double));
-        if (rc) return rc;
-        A.records = ctx->scratch;
-    }
+    // at their own lanes per point
+    const ScanGeom G2 = scan_geom(cfg.tail_lanes_per_point > 0 ? cfg.tail_lanes_per_point : G.lpp, pn, tn,
+                                  ctx->lds_block);
+    const int fullblocks = std::max(1, ctx->num_cu * std::max(1, cfg.waves_per_cu) / (G2.threads / 64));
     unsigned long long* q = A.queue;
     if (chains) {
         hipLaunchKernelGGL(pnjl::zero_u64, dim3(1), dim3(64), 0, stream, q, kQueueWords);
@@ -607,8 +595,14 @@ int pnjl_ctx_create(const pnjl_model_params* params, int device, pnjl_ctx** out)
         return PNJL_ERR_HIP;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) {
         c->num_cu = prop.multiProcessorCount;
+        // two blocks of four wavefronts per CU (2 waves per SIMD) share its
+        // LDS: at most half of it per block, and within the per-block limit
+        const size_t half = prop.maxSharedMemoryPerMultiProcessor / 2;
+        const size_t want = std::min<size_t>(prop.sharedMemPerBlock, half);
+        if (want > c->lds_block && allow_scan_lds((int)want)) c->lds_block = want;
+    }
     bool ev_ok = true;
     for (int i = 0; i < 2 * kMaxTimedScans; i++) ev_ok = ev_ok && hipEventCreate(&c->ev_scan[i]) == hipSuccess;
     if (!ev_ok || hipMalloc(&c->queue, kQueueWords * sizeof(unsigned long long)) != hipSuccess ||
@@ -635,7 +629,6 @@ void pnjl_ctx_destroy(pnjl_ctx* c)
     for (void* b : c->stage)
         if (b) (void)hipFree(b);
     if (c->pinned) (void)hipHostFree(c->pinned);
-    if (c->scratch) (void)hipFree(c->scratch);
     if (c->queue) (void)hipFree(c->queue);
     if (c->bnd) (void)hipFree(c->bnd);
     if (c->jmask) (void)hipFree(c->jmask);

@@ -38,9 +38,11 @@
 // candidate choice, the trust-region step) are out-of-line functions that
 // take the group's storage and read the launch arguments from an LDS copy
 // (cold_args), so the kernel keeps 2 waves per SIMD without spilling vector
-// registers.  Candidate records (x, Omega, thermodynamics, flags), which
-// are written once per attempt end or accepted trust-region step, live in a
-// small per-group area in HBM.
+// registers.  The group's candidate records (x, Omega, thermodynamics,
+// flags) sit in LDS behind its hot state: an attempt's end reads back the
+// record it has just written (the convergence gate, the candidate choice,
+// the copy to the item's output slot), and with the records in HBM each of
+// those reads was a round trip the whole wavefront waited for.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -94,11 +96,9 @@ struct ScanArgs {
     double xtol, ftol, resmax, phi_tol;
     int max_iter, tr_fb, auto_ms, surr_cap;
     int judge_mode;                         // PNJL_JUDGE_* (SURROGATE inside the mask)
-    // queue counters (Q_* words) and the per-group record area in HBM
+    // queue counters (Q_* words)
     unsigned long long* queue;
     int chunk;
-    double* records;                        // REC_AREA doubles per group
-    int64_t n_groups;
     // phase-boundary tables: nb sets of {xi, T_CEP, n, pad, T[kMaxBndRows], mu[kMaxBndRows]}
     const double* bnd;
     int nb;
@@ -184,7 +184,7 @@ enum RecField {
     RF_XCONV = 26,
     REC_SIZE = 28
 };
-// the group's record area in HBM (offsets of its records)
+// the group's record area (offsets of its records within SB_REC)
 enum RecBlock {
     SB_PRIM = 0,                 // primary (Newton) candidate of the attempt
     SB_TRC = SB_PRIM + REC_SIZE, // trust-region candidate at the accepted x
@@ -201,9 +201,10 @@ enum StateField {
     SB_XOLD = SB_PREV + 5,       // x of the previous Newton / trust-region step
     SB_ST = SB_XOLD + 5,         // the group's solver state (struct St, 24 doubles)
     SB_FJ = SB_ST + 24,          // F[5] and packed J[15] of the last pass, for the trust-region step
+    SB_REC = SB_FJ + 20,         // the candidate records (REC_AREA doubles)
     // an odd number of doubles per group: the groups of a wavefront hit
     // distinct LDS banks when they access the same field
-    STATE_STRIDE = SB_FJ + 20 + ((SB_FJ + 20) % 2 == 0 ? 1 : 0)
+    STATE_STRIDE = SB_REC + REC_AREA + ((SB_REC + REC_AREA) % 2 == 0 ? 1 : 0)
 };
 enum TrField { TR_R = 0, TR_J = 5, TR_D = 20, TR_P = 25, TR_DELTA = 30 };
 
@@ -215,12 +216,11 @@ enum TrField { TR_R = 0, TR_J = 5, TR_D = 20, TR_P = 25, TR_DELTA = 30 };
 // field from there on every trip.
 __shared__ ScanArgs cold_args;
 
-// a group's storage: hot state in LDS (at), candidate records in HBM (rec)
+// a group's storage in LDS: hot state (at) and candidate records (rec)
 struct Scr {
     double* l;
-    double* g;
     __device__ __forceinline__ double& at(int f) const { return l[f]; }
-    __device__ __forceinline__ double& rec(int f) const { return g[f]; }
+    __device__ __forceinline__ double& rec(int f) const { return l[SB_REC + f]; }
 };
 
 // packed index of the symmetric 5x5 Jacobian (upper triangle, row-major)
@@ -859,7 +859,7 @@ struct St {
     bool plan_multi;          // this point runs MultiSeed as its plan
 };
 static_assert(sizeof(St) <= 24 * sizeof(double), "St must fit its LDS slot");
-static_assert(SB_ST + 24 <= STATE_STRIDE && STATE_STRIDE % 2 == 1, "state layout");
+static_assert(SB_ST + 24 <= SB_FJ && SB_REC + REC_AREA <= STATE_STRIDE && STATE_STRIDE % 2 == 1, "state layout");
 
 __host__ __device__ inline bool branch_policy(int p)
 {
@@ -1536,9 +1536,7 @@ __global__ void __launch_bounds__(256, 2) scan_kernel(ScanArgs A)
     __syncthreads();
 
     const int gl = threadIdx.x % LPP;
-    const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPP;
-    if (g >= A.n_groups) return;  // beyond the record area the host allocated
-    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE, A.records + g * REC_AREA};
+    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE};
     const bool chains = (A.plan == PLAN_CHAINS);
     St& st = *reinterpret_cast<St*>(s.l + SB_ST);
     st.phase = PH_IDLE;
@@ -1661,7 +1659,7 @@ __global__ void __launch_bounds__(256, 3) residual_kernel(ResArgs R)
     const int gl = threadIdx.x % LPP;
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPP;
     if (g >= R.n) return;
-    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE, nullptr};
+    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE};
     double x[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) x[i] = R.x[g * 5 + i];
