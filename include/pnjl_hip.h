@@ -346,8 +346,10 @@ int pnjl_last_scan_ms(pnjl_ctx* ctx, double* ms, int max);
 /* PNJL_SEED_SURROGATE / PNJL_SEED_MULTIROOT3, whose fallback and judge are  */
 /* the reference seeds of SeedStrategies.jl)                                 */
 /* MLP 3 -> 128 -> 128 -> 128 -> 4 (SiLU) on normalised (T, mu, xi) giving   */
-/* normalised (phi_u, phi_s, Phi, Phibar); BF16 weights/activations, FP32   */
-/* MFMA accumulation, FP32 master weights trained by Adam on the GPU.       */
+/* normalised (phi_u, phi_s, Phi, Phibar); training: BF16 weights and      */
+/* activations, FP32 MFMA accumulation, FP32 master weights trained by Adam */
+/* on the GPU; inference: the FP32 masters at FP32-class precision (split   */
+/* BF16 products on the matrix cores).                                      */
 /* Parameters are one packed FP32 vector, row-major:                        */
 /*   W1[128][3] b1[128] W2[128][128] b2[128] W3[128][128] b3[128]           */
 /*   W4[4][128] b4[4]   (34052 values)                                      */

@@ -8,8 +8,16 @@
 //   train_tile_kernel   one 32-sample tile per block (4 waves): forward, MSE
 //                       loss, backward; writes the block's partial gradient
 //   adam_kernel         sums the block partials and applies Adam; refreshes
-//                       the BF16 (and transposed BF16) weight copies
-//   infer_kernel        forward only; writes FP64 seeds (phi_u, phi_u, phi_s, Phi, Phibar)
+//                       the BF16 (and transposed BF16) weight copies and the
+//                       BF16 residuals W - bf16(W) of the hidden layers
+//   infer_kernel        forward only, at FP32-class precision: hidden-layer
+//                       products as split BF16 (x = hi + lo, three MFMAs:
+//                       hi*hi + hi*lo + lo*hi), first and last layers on the
+//                       FP32 masters; writes FP64 seeds (phi_u, phi_u, phi_s,
+//                       Phi, Phibar).  Training runs the plain BF16 forward;
+//                       the seeds come from the FP32 weights it maintains,
+//                       whose exact forward lands closer to the roots (fewer
+//                       Newton steps) than its BF16 rounding.
 //
 // MFMA lane layout (wave64, 16x16x32): lane l holds A[l%16][8(l/16)+i] and
 // B[8(l/16)+i][l%16] (i = 0..7) and receives D[4(l/16)+r][l%16] (r = 0..3).
@@ -55,6 +63,8 @@ struct WB {
     unsigned short* W3;
     unsigned short* W3T;
     unsigned short* W4;   // [OUT][H]
+    unsigned short* W2lo; // [H][H]  (o, i): bf16(W2 - bf16(W2))
+    unsigned short* W3lo;
 };
 
 struct TrainArgs {
@@ -364,10 +374,12 @@ __global__ void __launch_bounds__(THREADS) adam_kernel(AdamArgs A)
         int e = p - OFF_W2, o = e / H, i = e % H;
         A.w.W2[e] = wb;
         A.w.W2T[i * H + o] = wb;
+        A.w.W2lo[e] = f2bf(w - bf2f(wb));
     } else if (p >= OFF_W3 && p < OFF_B3) {
         int e = p - OFF_W3, o = e / H, i = e % H;
         A.w.W3[e] = wb;
         A.w.W3T[i * H + o] = wb;
+        A.w.W3lo[e] = f2bf(w - bf2f(wb));
     } else if (p >= OFF_W4 && p < OFF_B4) {
         A.w.W4[p - OFF_W4] = wb;
     }
@@ -383,6 +395,70 @@ struct InferArgs {
     double* seeds;        // [n][5]
 };
 
+// h (FP32) as hi + lo, both BF16: hi = bf16(h), lo = bf16(h - hi)
+__device__ __forceinline__ void split_store(float h, unsigned short* hi, unsigned short* lo, int e)
+{
+    const unsigned short b = f2bf(h);
+    hi[e] = b;
+    lo[e] = f2bf(h - bf2f(b));
+}
+
+// first layer on the FP32 masters, output split
+__device__ void fwd_first_split(Smem& S, const float* W1, const float* b1)
+{
+    for (int e = threadIdx.x; e < TILE * H; e += THREADS) {
+        const int s = e / H, o = e % H;
+        float z = b1[o];
+#pragma unroll
+        for (int k = 0; k < IN; k++) z = fmaf(W1[o * IN + k], S.xs[s][k], z);
+        split_store(z * sigmoidf_(z), S.h1, S.g1, e);
+    }
+}
+
+// hidden layer with split operands: z = in W^T + b as hi*hi + hi*lo + lo*hi
+// (the lo*lo term is below FP32 rounding of the sum), output split
+__device__ void fwd_hidden_split(const unsigned short* in_hi, const unsigned short* in_lo, const unsigned short* W,
+                                 const unsigned short* Wlo, const float* b, unsigned short* out_hi,
+                                 unsigned short* out_lo)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    for (int t = wave; t < (TILE / 16) * (H / 16); t += THREADS / 64) {
+        const int ts = t / (H / 16), to = t % (H / 16);
+        float4v acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kk = 0; kk < H; kk += 32) {
+            const int ia = (ts * 16 + r) * H + kk + 8 * q, iw = (to * 16 + r) * H + kk + 8 * q;
+            const bf16x8 ah = __builtin_bit_cast(bf16x8, ld8(in_hi + ia));
+            const bf16x8 al = __builtin_bit_cast(bf16x8, ld8(in_lo + ia));
+            const bf16x8 wh = __builtin_bit_cast(bf16x8, ld8(W + iw));
+            const bf16x8 wl = __builtin_bit_cast(bf16x8, ld8(Wlo + iw));
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, wh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, wh, acc, 0, 0, 0);
+        }
+        const int o = to * 16 + r;
+        const float bo = b[o];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int e = (ts * 16 + 4 * q + i) * H + o;
+            const float z = acc[i] + bo;
+            split_store(z * sigmoidf_(z), out_hi, out_lo, e);
+        }
+    }
+}
+
+// last layer on the FP32 masters over the split activations
+__device__ void fwd_last_split(Smem& S, const float* W4, const float* b4)
+{
+    for (int e = threadIdx.x; e < TILE * OUT; e += THREADS) {
+        const int s = e / OUT, j = e % OUT;
+        float z = b4[j];
+        for (int o = 0; o < H; o++)
+            z = fmaf(W4[j * H + o], bf2f(S.h3[s * H + o]) + bf2f(S.g3[s * H + o]), z);
+        S.y[s][j] = z;
+    }
+}
+
 __global__ void __launch_bounds__(THREADS) infer_kernel(InferArgs A)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -395,13 +471,14 @@ __global__ void __launch_bounds__(THREADS) infer_kernel(InferArgs A)
         for (int k = 0; k < IN; k++) S.xs[s][k] = ok ? (float)((in[k] - A.in_lo[k]) * A.in_scale[k] - 1.0) : 0.f;
     }
     __syncthreads();
-    fwd_first(S, A.w.W1, A.P + OFF_B1);
+    // split activations: hi in h_l, lo in g_l (unused by the forward)
+    fwd_first_split(S, A.P + OFF_W1, A.P + OFF_B1);
     __syncthreads();
-    fwd_hidden(S.h1, A.w.W2, A.P + OFF_B2, S.h2, nullptr);
+    fwd_hidden_split(S.h1, S.g1, A.w.W2, A.w.W2lo, A.P + OFF_B2, S.h2, S.g2);
     __syncthreads();
-    fwd_hidden(S.h2, A.w.W3, A.P + OFF_B3, S.h3, nullptr);
+    fwd_hidden_split(S.h2, S.g2, A.w.W3, A.w.W3lo, A.P + OFF_B3, S.h3, S.g3);
     __syncthreads();
-    fwd_last(S, A.w.W4, A.P + OFF_B4);
+    fwd_last_split(S, A.P + OFF_W4, A.P + OFF_B4);
     __syncthreads();
     for (int s = threadIdx.x; s < TILE; s += THREADS) {
         int64_t row = row0 + s;
@@ -616,7 +693,7 @@ int pnjl_surrogate_create(int device, uint64_t init_seed, pnjl_surrogate** out)
         return PNJL_ERR_HIP;
     }
     const size_t pb = mlp::NPARAM * sizeof(float);
-    const size_t nb = (size_t)(mlp::H * mlp::IN + 4 * mlp::H * mlp::H + mlp::OUT * mlp::H);
+    const size_t nb = (size_t)(mlp::H * mlp::IN + 6 * mlp::H * mlp::H + mlp::OUT * mlp::H);
     if (hipMalloc(&s->P, pb) != hipSuccess || hipMalloc(&s->m, pb) != hipSuccess || hipMalloc(&s->v, pb) != hipSuccess ||
         hipMalloc(&s->grad, pb) != hipSuccess || hipMalloc(&s->wbuf, nb * 2) != hipSuccess ||
         hipMalloc(&s->loss_dev, sizeof(float)) != hipSuccess || hipEventCreate(&s->e0) != hipSuccess ||
@@ -630,6 +707,8 @@ int pnjl_surrogate_create(int device, uint64_t init_seed, pnjl_surrogate** out)
     s->w.W3 = s->w.W2T + mlp::H * mlp::H;
     s->w.W3T = s->w.W3 + mlp::H * mlp::H;
     s->w.W4 = s->w.W3T + mlp::H * mlp::H;
+    s->w.W2lo = s->w.W4 + mlp::OUT * mlp::H;
+    s->w.W3lo = s->w.W2lo + mlp::H * mlp::H;
     // He-uniform initialisation of the weights, zero biases (splitmix64 stream)
     std::vector<float> h(mlp::NPARAM, 0.f);
     uint64_t st = init_seed;

@@ -1,5 +1,6 @@
-"""GPU checks of the learned seed: the BF16-MFMA training kernels against
-the FP64 oracle, training progress, and what the learned seed gains."""
+"""GPU checks of the learned seed: the BF16-MFMA training kernels and the
+FP32-class inference against the FP64 oracle, training progress, and what
+the learned seed gains."""
 import numpy as np
 import pytest
 
@@ -41,6 +42,33 @@ def test_gradient_matches_oracle(n):
         nb = np.linalg.norm(b)
         assert np.linalg.norm(a - b) <= 2e-2 * nb + 1e-7, (k, np.linalg.norm(a - b) / nb)
         assert np.linalg.norm(a - c) <= 6e-2 * np.linalg.norm(c) + 1e-7, k
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_inference_matches_exact_forward(seed):
+    """The seeds come from the FP32 master weights at FP32-class precision
+    (split-BF16 products): against the oracle's exact FP64 forward their
+    error stays far below what a BF16 forward of the same weights makes."""
+    from oracle import mlp_oracle as M
+    s = _model(seed)
+    s.train(*_rand_data(4096, seed=seed), steps=200, batch=1024, lr=1e-3, seed=seed)
+    T, mu, xi, roots = _rand_data(3000, seed=seed + 10)
+    T[:4], mu[:4], xi[:4] = s.in_lo[0], s.in_lo[1], s.in_lo[2]     # domain corners
+    T[4:8], mu[4:8], xi[4:8] = s.in_hi[0], s.in_hi[1], s.in_hi[2]
+    seeds = s.infer(T, mu, xi)
+    xn, _ = M.normalise(T, mu, xi, roots, s.in_lo, s.in_hi, s.out_mean, s.out_std)
+    p = s.params.astype(np.float64)
+    pick = [0, 0, 1, 2, 3]
+    def phys(y):
+        return (y * s.out_std + s.out_mean)[:, pick]
+    exact = phys(M.forward(p, xn, bf=False)[0])
+    bf = phys(M.forward(p, xn, bf=True)[0])
+    scale = np.asarray(s.out_std)[pick]
+    err = np.max(np.abs(seeds - exact) / scale)
+    err_bf = np.max(np.abs(bf - exact) / scale)
+    assert np.all(seeds[:, 0] == seeds[:, 1])
+    assert err < 2e-4, err
+    assert err < 0.05 * err_bf, (err, err_bf)
 
 
 def test_training_reduces_loss_and_seeds_land_near_roots(gpu_ctx):
