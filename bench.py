@@ -15,7 +15,8 @@ Workloads (grids are linspace with endpoints; MeV -> fm^-1 with 197.327):
                   ambiguity mask).  Set-up (surrogate.learned_seed),
                   untimed and reported: 200,000 plain roots at
                   U([20,300] x [0,450] MeV x [-0.4,0.4]) (default_rng(4)),
-                  Adam lr 2e-3 batch 8192 8,000 steps on the GPU, judge mask
+                  Adam batch 8192, 8,000 steps (lr 3e-3, 1.5e-3, 6e-4,
+                  2e-4 for 2,000 steps each) on the GPU, judge mask
                   from those and 10^6 more probes (default_rng(5)).  The plain
                   DefaultSeed scan of the same grid is timed with the same K/W
                   and reported beside it.  The default run also times every

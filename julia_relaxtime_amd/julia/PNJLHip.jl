@@ -580,7 +580,9 @@ function ambiguous(s::Surrogate, ctx, T, mu, xi; p_num, t_num, iterations, ftol,
     set_judge_mask!(ctx, nothing)
     kw = (xi=xi, p_num=p_num, t_num=t_num, ftol=ftol, ctx=ctx)
     q = solve_batch(T, mu; policy=SEED_SURROGATE, seeds=infer(s, T, mu, xi), iterations=iterations, kw...)
-    m = solve_batch(T, mu; policy=SEED_MULTI, iterations=min(iterations, multi_iterations), kw...)
+    # MultiSeed probe: Newton only, capped -- it only has to find the competing roots
+    m = solve_batch(T, mu; policy=SEED_MULTI, iterations=min(iterations, multi_iterations),
+                    trust_region_fallback=false, kw...)
     pr = solve_batch(T, mu; policy=SEED_DEFAULT, iterations=iterations, kw...)
     return [!(q.converged[i] && m.converged[i] && pr.converged[i] && same_root(q, m, i) && same_root(pr, m, i))
             for i in eachindex(T)]
@@ -643,18 +645,23 @@ end
 
 training_points(n, rng) = ([DOMAIN_LO[d] + rand(rng) * (DOMAIN_HI[d] - DOMAIN_LO[d]) for _ in 1:n] for d in 1:3)
 
+# Adam phases (steps, lr) of learned_seed: a step decay, so that the network
+# does not end on a late loss spike of a constant rate
+const LR_SCHEDULE = ((2000, 3e-3), (2000, 1.5e-3), (2000, 6e-4), (2000, 2e-4))
+
 """
-    learned_seed(ctx; device=0, p_num=64, t_num=4, n_train=200_000, steps=8000, batch=8192, lr=2e-3,
+    learned_seed(ctx; device=0, p_num=64, t_num=4, n_train=200_000, schedule=LR_SCHEDULE, batch=8192,
                  n_probe=1_000_000) -> Surrogate
 
 The learned seed as the scans use it: plain (DefaultSeed) roots at n_train
-points of the training box, Adam on them, and the judge mask from those
-points plus n_probe random probes; installed in `ctx` (`device` is the
+points of the training box, Adam on them (the (steps, lr) phases of
+`schedule`, batch draws seeded 4, 5, ... per phase), and the judge mask from
+those points plus n_probe random probes; installed in `ctx` (`device` is the
 context's GPU).
 """
 function learned_seed(ctx::Context=default_context(); device::Integer=0, p_num::Integer=64, t_num::Integer=4,
-                      iterations::Integer=40, ftol::Real=1e-10, n_train::Integer=200_000, steps::Integer=8000,
-                      batch::Integer=8192, lr::Real=2e-3, n_probe::Integer=1_000_000, rng=Random.Xoshiro(4))
+                      iterations::Integer=40, ftol::Real=1e-10, n_train::Integer=200_000, schedule=LR_SCHEDULE,
+                      batch::Integer=8192, n_probe::Integer=1_000_000, rng=Random.Xoshiro(4))
     saved = launch_cfg(ctx)
     set_launch_cfg!(SETUP_LAUNCH, ctx)
     try
@@ -664,7 +671,9 @@ function learned_seed(ctx::Context=default_context(); device::Integer=0, p_num::
         ok = findall(tr.converged)
         s = Surrogate(device)
         fit_output_scaling!(s, tr.solution[:, ok])
-        train!(s, T[ok], mu[ok], xi[ok], tr.solution[:, ok]; steps=steps, batch=batch, lr=lr)
+        for (i, (steps, lr)) in enumerate(schedule)
+            train!(s, T[ok], mu[ok], xi[ok], tr.solution[:, ok]; steps=steps, batch=batch, lr=lr, seed=3 + i)
+        end
         Tp, mup, xip = training_points(n_probe, rng)
         build_judge_mask!(s, ctx, vcat(T, Tp), vcat(mu, mup), vcat(xi, xip); p_num=p_num, t_num=t_num,
                           iterations=iterations, ftol=ftol)

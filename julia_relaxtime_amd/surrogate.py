@@ -184,10 +184,11 @@ class Surrogate:
         and the MultiSeed (argmin-Omega) root are not all the same root, or a
         solve failed; with `multi_root`, also where the learned, HADRON and
         QUARK seeds reach distinct roots.  The MultiSeed probe runs each
-        candidate for at most `multi_max_iter` iterations (Newton, then the
-        trust region): it only has to find the competing roots, and its
-        candidates that fail would otherwise run the whole budget.  The probes
-        run on the context's GPU; only the verdicts come back.
+        candidate as at most `multi_max_iter` Newton iterations, without the
+        trust-region fallback: it only has to find the competing roots, and
+        its candidates that fail would otherwise run the whole budget twice
+        over.  The probes run on the context's GPU; only the verdicts come
+        back.
 
         Two passes: the given probes (the training set, optionally with more
         random points), then -- when `refine` > 0 -- `refine` further random
@@ -200,8 +201,8 @@ class Surrogate:
         nb = np.asarray(bins)
         amb = self._ambiguous(ctx, T, mu, xi, plain_roots, p_num, t_num, opts, multi_root, multi_max_iter)
         mask = np.zeros(tuple(nb), dtype=np.uint8)
-        b = self._bins(T, mu, xi, nb)
-        mask[b[amb, 0], b[amb, 1], b[amb, 2]] = 1
+        b = self._bins(T[amb], mu[amb], xi[amb], nb)
+        mask[b[:, 0], b[:, 1], b[:, 2]] = 1
         if refine > 0 and mask.any():
             region = _dilate(mask, 2, full=True)
             cells = np.argwhere(region)
@@ -211,8 +212,8 @@ class Surrogate:
             P = self.in_lo + u * (self.in_hi - self.in_lo)
             amb2 = self._ambiguous(ctx, P[:, 0], P[:, 1], P[:, 2], None, p_num, t_num, opts, multi_root,
                                    multi_max_iter)
-            b2 = self._bins(P[:, 0], P[:, 1], P[:, 2], nb)
-            mask[b2[amb2, 0], b2[amb2, 1], b2[amb2, 2]] = 1
+            b2 = self._bins(P[amb2, 0], P[amb2, 1], P[amb2, 2], nb)
+            mask[b2[:, 0], b2[:, 1], b2[:, 2]] = 1
         mask = _dilate(mask, dilate, full=(neighbourhood == "full"))
         self.judge_mask = mask
         ctx.set_judge_mask(mask, self.in_lo, self.in_hi)
@@ -233,6 +234,7 @@ class Surrogate:
         mo = type(o).from_buffer_copy(o)
         if multi_max_iter:
             mo.max_iter = min(int(o.max_iter), int(multi_max_iter))
+            mo.trust_region_fallback = 0
         dev = torch.device("cuda", ctx.device)
         Td, mud, xid = (torch.tensor(_d(a), device=dev) for a in (T, mu, xi))
         saved = getattr(self, "judge_mask", None)
@@ -305,14 +307,21 @@ def _dilate(mask, n, full=True):
 SETUP_LAUNCH = dict(lanes_per_point=8, chunk=1, waves_per_cu=16, tail_lanes_per_point=0)
 
 
-def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, steps=8000, batch=8192, lr=2e-3,
+# Adam phases (steps, lr) of learned_seed: a step decay.  With a constant
+# rate the network that comes out is whatever state the last steps leave;
+# a late loss spike then makes seeds that double the scan's work.
+LR_SCHEDULE = ((2000, 3e-3), (2000, 1.5e-3), (2000, 6e-4), (2000, 2e-4))
+
+
+def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, schedule=LR_SCHEDULE, batch=8192,
                  n_probe=1_000_000, multi_max_iter=12):
     """The learned seed as the scans use it, built on the context's GPU: plain
     (DefaultSeed) roots at n_train points of the training box
-    (training_points, default_rng(4)), Adam on them (steps x batch, lr), and
-    the judge mask from those points plus n_probe random probes
-    (default_rng(5)).  Returns (Surrogate, info) with the set-up's figures;
-    the context's launch geometry is restored afterwards."""
+    (training_points, default_rng(4)), Adam on them (the (steps, lr) phases
+    of `schedule`, batch draws seeded 4, 5, ... per phase), and the judge
+    mask from those points plus n_probe random probes (default_rng(5)).
+    Returns (Surrogate, info) with the set-up's figures; the context's launch
+    geometry is restored afterwards."""
     import time
     t0 = time.perf_counter()
     saved = ctx.launch_cfg()
@@ -323,14 +332,18 @@ def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, steps=8000,
         ok = tr["converged"] == 1
         s = Surrogate(ctx.device, init_seed=4)
         s.fit_output_scaling(tr["x"][ok])
-        loss = s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=steps, batch=batch, lr=lr, seed=4)
-        train_ms = s.last_ms
+        loss, train_ms = [], 0.0
+        for i, (steps, lr) in enumerate(schedule):
+            loss.append(s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=steps, batch=batch, lr=lr, seed=4 + i))
+            train_ms += s.last_ms
+        loss = np.concatenate(loss)
         Tp, mup, xip = training_points(n_probe, seed=5)
         s.build_judge_mask(ctx, np.r_[T, Tp], np.r_[mu, mup], np.r_[xi, xip], p_num=p_num, t_num=t_num, opts=opts,
                            multi_max_iter=multi_max_iter)
     finally:
         ctx.set_launch_cfg(**saved)
-    info = dict(train_points=int(ok.sum()), adam_steps=steps, batch=batch, lr=lr, train_kernel_ms=train_ms,
+    info = dict(train_points=int(ok.sum()), adam_steps=int(sum(n for n, _ in schedule)), batch=batch,
+                lr=[list(ph) for ph in schedule], train_kernel_ms=train_ms,
                 final_loss=float(np.mean(loss[-200:])), judge_probes=n_train + n_probe,
                 judge_mask_fraction=float(s.judge_mask.mean()), setup_s=time.perf_counter() - t0, loss=loss)
     return s, info

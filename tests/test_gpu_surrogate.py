@@ -176,8 +176,8 @@ def test_learned_seed_policies_match_oracle(gpu_ctx, oracle, policy, judge):
 def c4_model(gpu_ctx):
     """The learned seed as the scan and the bench build it
     (surrogate.learned_seed): 200 000 plain roots at U([20,300] MeV x
-    [0,450] MeV x [-0.4,0.4]) from default_rng(4), Adam lr 2e-3, batch 8192,
-    8 000 steps; judge mask from those roots plus 10^6 further random probes
+    [0,450] MeV x [-0.4,0.4]) from default_rng(4), Adam batch 8192, 8 000
+    steps in four phases of lr 3e-3 .. 2e-4; judge mask from those roots plus 10^6 further random probes
     (default_rng(5))."""
     from julia_relaxtime_amd import _lib
     from julia_relaxtime_amd.surrogate import learned_seed
