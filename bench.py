@@ -258,12 +258,14 @@ def cpu_baseline(w, T, mu, xi, n_sample=20000, threads=None):
 
 # HBM bytes per scan_kernel launch of the learned-seed scan (c4_surrogate):
 # FETCH_SIZE + WRITE_SIZE of its two launches per step, from two rocprofv3
-# --pmc runs of this bench over three timed steps (FETCH 0.1411 + 0.0119 GiB,
-# WRITE 0.3706 + 0.0544 GiB per step), divided by the two launches.  A 1 GiB
-# device copy reads 0.5 GiB in FETCH_SIZE and exactly 1 GiB in WRITE_SIZE,
-# so FETCH is scaled by 2.  The algorithmic bytes are ~0.25 KB per point
-# (inputs, seeds, outputs); the rest is the per-point attempt records.
-C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (0.1411 + 0.0119) + 0.3706 + 0.0544) / 2 * 2 ** 30
+# --pmc runs of this bench over three timed steps (FETCH 0.1297 + 0.0071 GiB,
+# WRITE 0.2935 + 0.0386 GiB per step; profiles/scan_kernel_c4_surrogate_pmc.md),
+# divided by the two launches.  A 1 GiB device copy reads 0.5 GiB in
+# FETCH_SIZE and exactly 1 GiB in WRITE_SIZE, so FETCH is scaled by 2.  The
+# algorithmic bytes are ~0.25 KB per point (inputs, seeds, outputs); the rest
+# is the per-point attempt records.  Re-measure with the profile whenever the
+# scan kernel changes.
+C4_TRAFFIC_BYTES_PER_LAUNCH = (2.0 * (0.1297 + 0.0071) + 0.2935 + 0.0386) / 2 * 2 ** 30
 
 def train_learned_seed(ctx, w, opts):
     """The learned seed's set-up (surrogate.learned_seed): training roots,

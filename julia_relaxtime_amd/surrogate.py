@@ -227,7 +227,9 @@ class Surrogate:
         """Probes at which the learned-seed, DefaultSeed and MultiSeed solves do
         not all reach the same root (DualBranchScan's sameness test and Omega
         to 1e-10), or one of them failed: a numpy bool per probe.  The probes
-        and their solves stay in HBM."""
+        and their solves stay in HBM; the verdict is taken on the host from
+        plain copies of the four fields it reads (no torch compute kernels,
+        whose code a fresh process first has to load from disk: ~0.3 s)."""
         import torch
         from ._lib import PT_FIRST_ORDER, SEED_DEFAULT, SEED_MULTI, SEED_MULTIROOT3, SEED_SURROGATE
         o = opts if opts is not None else _lib.default_opts()
@@ -239,26 +241,29 @@ class Surrogate:
         Td, mud, xid = (torch.tensor(_d(a), device=dev) for a in (T, mu, xi))
         saved = getattr(self, "judge_mask", None)
         ctx.set_judge_mask(None, None, None)
+        def host(r, keys=("x", "masses", "omega", "converged")):
+            return {k: r[k].cpu().numpy() for k in keys}
+
         try:
             seeds = self.infer(Td, mud, xid)
-            q = ctx.solve_batch(Td, mud, xid, policy=SEED_SURROGATE, seed=seeds, p_num=p_num, t_num=t_num, opts=o)
-            m = ctx.solve_batch(Td, mud, xid, policy=SEED_MULTI, p_num=p_num, t_num=t_num, opts=mo)
+            q = host(ctx.solve_batch(Td, mud, xid, policy=SEED_SURROGATE, seed=seeds, p_num=p_num, t_num=t_num,
+                                     opts=o))
+            m = host(ctx.solve_batch(Td, mud, xid, policy=SEED_MULTI, p_num=p_num, t_num=t_num, opts=mo))
             amb = (q["converged"] != 1) | (m["converged"] != 1) | ~same_root(q, m)
             if plain_roots is None:
-                pr = ctx.solve_batch(Td, mud, xid, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=o)
+                pr = host(ctx.solve_batch(Td, mud, xid, policy=SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=o))
                 amb |= (pr["converged"] != 1) | ~same_root(pr, m)
             elif isinstance(plain_roots, dict):
-                pr = {k: torch.as_tensor(plain_roots[k], device=dev) for k in ("x", "masses", "omega", "converged")}
-                amb |= (pr["converged"] != 1) | ~same_root(pr, m)
+                amb |= (np.asarray(plain_roots["converged"]) != 1) | ~same_root(plain_roots, m)
             else:
-                px = torch.as_tensor(_d(plain_roots), device=dev)
-                amb |= ((px - m["x"]).abs() / torch.clamp(m["x"].abs(), min=1.0)).amax(dim=1) > 1e-8
+                px = _d(plain_roots).reshape(-1, 5)
+                amb |= (np.abs(px - m["x"]) / np.maximum(np.abs(m["x"]), 1.0)).max(axis=1) > 1e-8
             if multi_root:
                 # also every probe where the HADRON and QUARK seeds reach distinct roots
-                r3 = ctx.solve_batch(Td, mud, xid, policy=SEED_MULTIROOT3, seed=seeds, p_num=p_num, t_num=t_num,
-                                     opts=o)
+                r3 = host(ctx.solve_batch(Td, mud, xid, policy=SEED_MULTIROOT3, seed=seeds, p_num=p_num,
+                                          t_num=t_num, opts=o), ("flags",))
                 amb |= (r3["flags"] & PT_FIRST_ORDER) != 0
-            return amb.cpu().numpy()
+            return amb
         finally:
             if saved is not None:
                 ctx.set_judge_mask(saved, self.in_lo, self.in_hi)
@@ -330,6 +335,7 @@ def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, schedule=LR
         T, mu, xi = training_points(n_train, seed=4)
         tr = ctx.solve_batch(T, mu, xi, policy=_lib.SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=opts)
         ok = tr["converged"] == 1
+        t1 = time.perf_counter()
         s = Surrogate(ctx.device, init_seed=4)
         s.fit_output_scaling(tr["x"][ok])
         loss, train_ms = [], 0.0
@@ -337,6 +343,7 @@ def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, schedule=LR
             loss.append(s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=steps, batch=batch, lr=lr, seed=4 + i))
             train_ms += s.last_ms
         loss = np.concatenate(loss)
+        t2 = time.perf_counter()
         Tp, mup, xip = training_points(n_probe, seed=5)
         s.build_judge_mask(ctx, np.r_[T, Tp], np.r_[mu, mup], np.r_[xi, xip], p_num=p_num, t_num=t_num, opts=opts,
                            multi_max_iter=multi_max_iter)
@@ -345,7 +352,9 @@ def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, schedule=LR
     info = dict(train_points=int(ok.sum()), adam_steps=int(sum(n for n, _ in schedule)), batch=batch,
                 lr=[list(ph) for ph in schedule], train_kernel_ms=train_ms,
                 final_loss=float(np.mean(loss[-200:])), judge_probes=n_train + n_probe,
-                judge_mask_fraction=float(s.judge_mask.mean()), setup_s=time.perf_counter() - t0, loss=loss)
+                judge_mask_fraction=float(s.judge_mask.mean()), setup_s=time.perf_counter() - t0,
+                setup_phases_s=dict(training_roots=t1 - t0, training=t2 - t1, judge_mask=time.perf_counter() - t2),
+                loss=loss)
     return s, info
 
 
