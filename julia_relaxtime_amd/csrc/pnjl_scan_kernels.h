@@ -380,8 +380,34 @@ __device__ __forceinline__ int slot_of(int i, bool sym) { return i == 0 ? 0 : (i
 // ---------------------------------------------------------------------------
 // Omega, F and the packed symmetric Jacobian from the sums in scratch
 // ---------------------------------------------------------------------------
+// vacuum_I of the three masses.  With LPP >= 4 lanes per group, lane gl
+// evaluates mass gl % 3 and the group takes the three results from its lanes
+// 0, 1, 2 (the same arithmetic on the same inputs, so the same bits): one
+// evaluation's instructions per wavefront instead of three.  Every lane of
+// the group must be active.
+template <int LPP>
+__device__ __forceinline__ void vacuum_I3(const DevParams& P, const double M[3], int gl, double I[3], double dI[3],
+                                          double d2I[3])
+{
+    if constexpr (LPP >= 4) {
+        const int f = gl % 3;
+        double a, b, c;
+        vacuum_I(P, f == 0 ? M[0] : (f == 1 ? M[1] : M[2]), a, b, c);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            I[i] = __shfl(a, i, LPP);
+            dI[i] = __shfl(b, i, LPP);
+            d2I[i] = __shfl(c, i, LPP);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) vacuum_I(P, M[i], I[i], dI[i], d2I[i]);
+    }
+}
+
+template <int LPP>
 __device__ __forceinline__ void assemble_core(const DevParams& P, const Scr& s, const double* x, double T, bool sym, double& omega,
-                              double F[5], double Jp[15])
+                              double F[5], double Jp[15], int gl)
 {
     const double G = P.G, K = P.K;
     const double pu = x[0], pd = x[1], ps = x[2], Phi = x[3], Phib = x[4];
@@ -410,11 +436,12 @@ __device__ __forceinline__ void assemble_core(const DevParams& P, const Scr& s, 
     double gM[3], hM[3], bM1[3], bM2[3];
     double SP = 0, SPb = 0, SPP = 0, SPPb = 0, SPbPb = 0;
     const double m2T = -2.0 * T, m2Nc = -2.0 * P.Nc;
+    double vI[3], vdI[3], vd2I[3];
+    vacuum_I3<LPP>(P, M, gl, vI, vdI, vd2I);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const int b = SB_SUMS + 13 * slot_of(i, sym);
-        double I, dI, d2I;
-        vacuum_I(P, M[i], I, dI, d2I);
+        const double I = vI[i], dI = vdI[i], d2I = vd2I[i];
         double A1 = s.at(b + SF_A1);
         acc += m2Nc * I + m2T * s.at(b + SF_S);
         gM[i] = m2Nc * dI + m2T * (M[i] * A1);
@@ -1439,7 +1466,7 @@ template <int LPP>
 __device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St& st, int gl)
 {
     double omega, F[5], Jp[15];
-    assemble_core(A.P, s, st.x, st.T, st.sym, omega, F, Jp);
+    assemble_core<LPP>(A.P, s, st.x, st.T, st.sym, omega, F, Jp, gl);
     if (st.phase == PH_NEWTON) {
         bool fconv, xconv;
         if (st.fresh) {
@@ -1666,7 +1693,7 @@ __global__ void __launch_bounds__(256, 3) residual_kernel(ResArgs R)
     bool sym = (x[0] == x[1]);
     node_pass<LPP>(R.P, lds, R.pn, R.tn, s, x, R.T[g], R.mu[g], R.xi[g], sym, gl);
     double omega, F[5], Jp[15];
-    assemble_core(R.P, s, x, R.T[g], sym, omega, F, Jp);
+    assemble_core<LPP>(R.P, s, x, R.T[g], sym, omega, F, Jp, gl);
     if (gl == 0) {
 #pragma unroll
         for (int i = 0; i < 5; i++) {
