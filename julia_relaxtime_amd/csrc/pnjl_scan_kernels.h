@@ -333,16 +333,13 @@ __device__ __forceinline__ void node_loop(const double* lds, const double* pp, c
 }
 
 template <int LPP>
-__device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, double M, double T, double mu,
-                                             double xi, double Phi, double Phib, int gl, FlavSums& acc)
+__device__ __forceinline__ void flavour_sums(const double* lds, int pn, int tn, double M, double invT, double muT,
+                                             double K2mu, double xi, double Phi, double Phib, int gl, FlavSums& acc)
 {
     const double* pp = lds + kMathTab;
     const double* cc = pp + pn;
     const double* Cm = pp + pn + tn;
     const double* Cs = Cm + pn * tn;
-    const double invT = 1.0 / T;
-    const double muT = mu * invT;
-    const double K2mu = exp(-2.0 * muT);
     const double M2 = M * M;
     flav_zero(acc);
     // one loop per wave: a state that needs the checked loop (a negative
@@ -364,12 +361,16 @@ __device__ __forceinline__ void node_pass(const DevParams& P, const double* lds,
 {
     double M[3];
     mass_vec(P, x, M);
+    // 1/T, mu/T and exp(-2 mu/T): once per pass, not per flavour
+    const double invT = 1.0 / T;
+    const double muT = mu * invT;
+    const double K2mu = exp(-2.0 * muT);
     const int nf = sym ? 2 : 3;
 #pragma unroll 1
     for (int f = 0; f < nf; f++) {
         double Mf = (f == 0) ? M[0] : ((f == 1) ? M[2] : M[1]);
         FlavSums acc;
-        flavour_sums<LPP>(lds, pn, tn, Mf, T, mu, xi, x[3], x[4], gl, acc);
+        flavour_sums<LPP>(lds, pn, tn, Mf, invT, muT, K2mu, xi, x[3], x[4], gl, acc);
         store_flav(s, f, acc);
     }
 }
