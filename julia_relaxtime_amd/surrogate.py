@@ -327,6 +327,7 @@ def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, schedule=LR
     mask from those points plus n_probe random probes (default_rng(5)).
     Returns (Surrogate, info) with the set-up's figures; the context's launch
     geometry is restored afterwards."""
+    import threading
     import time
     t0 = time.perf_counter()
     saved = ctx.launch_cfg()
@@ -336,16 +337,29 @@ def learned_seed(ctx, p_num=64, t_num=4, opts=None, n_train=200_000, schedule=LR
         tr = ctx.solve_batch(T, mu, xi, policy=_lib.SEED_DEFAULT, p_num=p_num, t_num=t_num, opts=opts)
         ok = tr["converged"] == 1
         t1 = time.perf_counter()
+        # the probe points are drawn on the host while the GPU trains (the
+        # training call releases the interpreter lock)
+        probes = {}
+
+        def draw():
+            Tp, mup, xip = training_points(n_probe, seed=5)
+            probes.update(T=np.r_[T, Tp], mu=np.r_[mu, mup], xi=np.r_[xi, xip])
+
+        drawer = threading.Thread(target=draw)
+        drawer.start()
         s = Surrogate(ctx.device, init_seed=4)
         s.fit_output_scaling(tr["x"][ok])
         loss, train_ms = [], 0.0
-        for i, (steps, lr) in enumerate(schedule):
-            loss.append(s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=steps, batch=batch, lr=lr, seed=4 + i))
-            train_ms += s.last_ms
+        try:
+            for i, (steps, lr) in enumerate(schedule):
+                loss.append(s.train(T[ok], mu[ok], xi[ok], tr["x"][ok], steps=steps, batch=batch, lr=lr,
+                                    seed=4 + i))
+                train_ms += s.last_ms
+        finally:
+            drawer.join()
         loss = np.concatenate(loss)
         t2 = time.perf_counter()
-        Tp, mup, xip = training_points(n_probe, seed=5)
-        s.build_judge_mask(ctx, np.r_[T, Tp], np.r_[mu, mup], np.r_[xi, xip], p_num=p_num, t_num=t_num, opts=opts,
+        s.build_judge_mask(ctx, probes["T"], probes["mu"], probes["xi"], p_num=p_num, t_num=t_num, opts=opts,
                            multi_max_iter=multi_max_iter)
     finally:
         ctx.set_launch_cfg(**saved)
