@@ -885,6 +885,7 @@ struct St {
     bool have_best;
     bool have_prev;
     bool plan_multi;          // this point runs MultiSeed as its plan
+    bool judged;              // stage 1: the point is in the learned seed's reference-judge mask
 };
 static_assert(sizeof(St) <= 24 * sizeof(double), "St must fit its LDS slot");
 static_assert(SB_ST + 24 <= SB_FJ && SB_REC + REC_AREA <= STATE_STRIDE && STATE_STRIDE % 2 == 1, "state layout");
@@ -1154,7 +1155,17 @@ __device__ __forceinline__ bool pipeline_next(const ScanArgs& A, St& st, int gl,
     st.k = A.k_base + (int)(i - j * A.mult);
     st.idx = j;
     st.kind = A.kind;
-    st.pt = A.item_list ? (int64_t)A.item_list[j] : (A.order ? (int64_t)A.order[j] : j);
+    st.judged = false;
+    if (A.item_list) {
+        st.pt = (int64_t)A.item_list[j];
+    } else if (A.order) {
+        // the ordering pass marks the judged points by complementing their index
+        const int32_t v = A.order[j];
+        st.judged = v < 0;
+        st.pt = (int64_t)(v < 0 ? ~v : v);
+    } else {
+        st.pt = j;
+    }
     return true;
 }
 
@@ -1191,12 +1202,20 @@ __device__ __forceinline__ void begin_primary(const ScanArgs& A, const Scr& s, S
 __device__ __noinline__ void begin_item(Scr s, St& st, int gl)
 {
     const ScanArgs& A = cold_args;
-    reset_point(A, st);
     const int64_t p = st.pt;
     double x0[5];
+    if (st.kind == K_SURR) {
+        // the learned seed is read with the point's (T, mu, xi), in one round trip
+#pragma unroll
+        for (int k = 0; k < 5; k++) x0[k] = A.seed[p * 5 + k];
+    }
+    reset_point(A, st);
     switch (st.kind) {
     case K_SURR:
-        if (A.judge_mode == PNJL_JUDGE_REFERENCE && A.judge_mask && in_judge_region(A, st.T, st.mu, st.xi)) {
+        // judged: marked by the ordering pass (the same mask test on the same
+        // inputs), else tested here
+        if (A.order ? st.judged
+                    : (A.judge_mode == PNJL_JUDGE_REFERENCE && A.judge_mask && in_judge_region(A, st.T, st.mu, st.xi))) {
             // ambiguity region: the reference policy takes the point unseen
             if (gl == 0) {
                 double* r = A.recS + p * REC_SIZE;
@@ -1206,8 +1225,6 @@ __device__ __noinline__ void begin_item(Scr s, St& st, int gl)
             begin_primary(A, s, st);
             return;
         }
-#pragma unroll
-        for (int k = 0; k < 5; k++) x0[k] = A.seed[p * 5 + k];
         start_attempt(st, s, ATT_SURR, x0);
         return;
     case K_MULTI:
@@ -1656,7 +1673,8 @@ __global__ void __launch_bounds__(kOrderBlock) order_kernel(ScanArgs A, int32_t*
         unsigned long long pos = bbase[c] + rank;
         for (int w = 0; w < wave; w++) pos += wcount[w][c];
         for (int k = 0; k < c; k++) pos += counters[k];
-        order[pos] = (int32_t)p;
+        // classes 0 and 1 are the judged points: their index complemented
+        order[pos] = c < 2 ? ~(int32_t)p : (int32_t)p;
     }
 }
 
