@@ -216,11 +216,21 @@ enum TrField { TR_R = 0, TR_J = 5, TR_D = 20, TR_P = 25, TR_DELTA = 30 };
 // field from there on every trip.
 __shared__ ScanArgs cold_args;
 
-// a group's storage in LDS: hot state (at) and candidate records (rec)
+// The kernels' dynamic LDS: node tables, then the groups' storage.
+extern __shared__ double scan_lds[];
+
+struct St;
+
+// a group's storage in LDS: hot state (at), candidate records (rec) and the
+// solver state (st), addressed from the LDS array itself (an offset, not a
+// pointer), so that out-of-line functions access it as LDS rather than
+// through generic (flat) addresses
 struct Scr {
-    double* l;
-    __device__ __forceinline__ double& at(int f) const { return l[f]; }
-    __device__ __forceinline__ double& rec(int f) const { return l[SB_REC + f]; }
+    int off;
+    __device__ __forceinline__ double& at(int f) const { return scan_lds[off + f]; }
+    __device__ __forceinline__ double& rec(int f) const { return scan_lds[off + SB_REC + f]; }
+    __device__ __forceinline__ double* ptr(int f) const { return scan_lds + off + f; }
+    __device__ __forceinline__ St& st() const;
 };
 
 // packed index of the symmetric 5x5 Jacobian (upper triangle, row-major)
@@ -888,6 +898,7 @@ struct St {
     bool judged;              // stage 1: the point is in the learned seed's reference-judge mask
 };
 static_assert(sizeof(St) <= 24 * sizeof(double), "St must fit its LDS slot");
+__device__ __forceinline__ St& Scr::st() const { return *reinterpret_cast<St*>(scan_lds + off + SB_ST); }
 static_assert(SB_ST + 24 <= SB_FJ && SB_REC + REC_AREA <= STATE_STRIDE && STATE_STRIDE % 2 == 1, "state layout");
 
 __host__ __device__ inline bool branch_policy(int p)
@@ -993,8 +1004,9 @@ __device__ __forceinline__ void finish_point(const ScanArgs& A, const Scr& s, St
 }
 
 // Sets up point st.pt of a chain according to the policy (first attempt).
-__device__ __noinline__ void begin_chain_point(Scr s, St& st)
+__device__ __noinline__ void begin_chain_point(Scr s, St&)
 {
+    St& st = s.st();   // the caller's state, addressed as LDS
     const ScanArgs& A = cold_args;
     int64_t i = st.pt;
     st.T = A.T[i];
@@ -1051,8 +1063,9 @@ __device__ __noinline__ void begin_chain_point(Scr s, St& st)
 // A chain point's attempt ended with its result in record `blk`: the
 // automatic MultiSeed fallback runs its six candidates one after another.
 template <int LPP>
-__device__ __noinline__ void chain_attempt_done(Scr s, St& st, int blk, int gl, bool conv, bool exc)
+__device__ __noinline__ void chain_attempt_done(Scr s, St&, int blk, int gl, bool conv, bool exc)
 {
+    St& st = s.st();   // the caller's state, addressed as LDS
     const ScanArgs& A = cold_args;
     double x0[5];
     if (st.attempt == ATT_PRIMARY) {
@@ -1199,8 +1212,9 @@ __device__ __forceinline__ void begin_primary(const ScanArgs& A, const Scr& s, S
 }
 
 // Sets up the attempt of the item the group just took.
-__device__ __noinline__ void begin_item(Scr s, St& st, int gl)
+__device__ __noinline__ void begin_item(Scr s, St&, int gl)
 {
+    St& st = s.st();   // the caller's state, addressed as LDS
     const ScanArgs& A = cold_args;
     const int64_t p = st.pt;
     double x0[5];
@@ -1254,8 +1268,9 @@ __device__ __noinline__ void begin_item(Scr s, St& st, int gl)
 // the MultiSeed list, a judged root to the Omega judge's list, a failed
 // judge candidate (HADRON, hint-QUARK) to the second judge list; a failed
 // learned-seed attempt continues with the primary attempt in the same group.
-__device__ __noinline__ void item_done(Scr s, St& st, int blk, int gl, bool conv, bool exc)
+__device__ __noinline__ void item_done(Scr s, St&, int blk, int gl, bool conv, bool exc)
 {
+    St& st = s.st();   // the caller's state, addressed as LDS
     const ScanArgs& A = cold_args;
     const int64_t p = st.pt;
     const bool cont = (st.kind == K_SURR) && !conv;
@@ -1369,11 +1384,12 @@ __device__ __noinline__ int choose_candidate(Scr s)
 // eta 1e-4), after an evaluation at st.x.  Returns true if another pass is
 // needed; otherwise the TR candidate is final in SB_TRC.
 // F and the packed J of the pass are in the group's SB_FJ state.
-__device__ __noinline__ bool tr_step(Scr s, St& st, double omega)
+__device__ __noinline__ bool tr_step(Scr s, St&, double omega)
 {
+    St& st = s.st();   // the caller's state, addressed as LDS
     const ScanArgs& A = cold_args;
-    const double* F = s.l + SB_FJ;
-    const double* Jp = s.l + SB_FJ + 5;
+    const double* F = s.ptr(SB_FJ);
+    const double* Jp = s.ptr(SB_FJ + 5);
     double r[5], d[5], p[5], delta;
     bool converged = false, stopped;
     if (st.fresh) {
@@ -1491,7 +1507,7 @@ __device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St
             st.fresh = false;
             if (!all_finite5(F)) {
                 // NLsolve check_isfinite throws on the seed
-                put_exc_rec(s, SB_PRIM, s.l + SB_X0);
+                put_exc_rec(s, SB_PRIM, s.ptr(SB_X0));
                 attempt_done<LPP>(A, s, st, SB_PRIM, gl);
                 return;
             }
@@ -1574,16 +1590,16 @@ __host__ __device__ inline int table_len(int pn, int tn) { return kMathTab + pn 
 template <int LPP>
 __global__ void __launch_bounds__(256, 2) scan_kernel(ScanArgs A)
 {
-    extern __shared__ double lds[];
+    double* const lds = scan_lds;
     const int tab_len = table_len(A.pn, A.tn);
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) lds[i] = A.tab[i];
     if (threadIdx.x == 0) cold_args = A;
     __syncthreads();
 
     const int gl = threadIdx.x % LPP;
-    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE};
+    const Scr s{(int)(tab_len + (threadIdx.x / LPP) * STATE_STRIDE)};
     const bool chains = (A.plan == PLAN_CHAINS);
-    St& st = *reinterpret_cast<St*>(s.l + SB_ST);
+    St& st = s.st();
     st.phase = PH_IDLE;
     st.item = 0;
     st.item_end = 0;
@@ -1698,14 +1714,14 @@ struct ResArgs {
 template <int LPP>
 __global__ void __launch_bounds__(256, 3) residual_kernel(ResArgs R)
 {
-    extern __shared__ double lds[];
+    double* const lds = scan_lds;
     const int tab_len = table_len(R.pn, R.tn);
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) lds[i] = R.tab[i];
     __syncthreads();
     const int gl = threadIdx.x % LPP;
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPP;
     if (g >= R.n) return;
-    const Scr s{lds + tab_len + (threadIdx.x / LPP) * STATE_STRIDE};
+    const Scr s{(int)(tab_len + (threadIdx.x / LPP) * STATE_STRIDE)};
     double x[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) x[i] = R.x[g * 5 + i];
