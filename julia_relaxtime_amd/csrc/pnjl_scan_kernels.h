@@ -233,6 +233,16 @@ struct Scr {
     __device__ __forceinline__ St& st() const;
 };
 
+// p as a global-memory pointer.  Out-of-line functions read the launch
+// arguments from LDS (cold_args), so their pointers are generic and every
+// access would be a flat instruction, which also holds up the function's
+// LDS accesses (flat operations count against the LDS wait counter too).
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 // packed index of the symmetric 5x5 Jacobian (upper triangle, row-major)
 __host__ __device__ constexpr int sidx(int i, int j)
 {
@@ -1009,9 +1019,9 @@ __device__ __noinline__ void begin_chain_point(Scr s, St&)
     St& st = s.st();   // the caller's state, addressed as LDS
     const ScanArgs& A = cold_args;
     int64_t i = st.pt;
-    st.T = A.T[i];
-    st.mu = A.mu[i];
-    st.xi = A.xi[i];
+    st.T = gmem(A.T)[i];
+    st.mu = gmem(A.mu)[i];
+    st.xi = gmem(A.xi)[i];
     st.flags = 0;
     st.steps = 0;
     st.passes = 0;
@@ -1185,9 +1195,9 @@ __device__ __forceinline__ bool pipeline_next(const ScanArgs& A, St& st, int gl,
 __device__ __forceinline__ void reset_point(const ScanArgs& A, St& st)
 {
     const int64_t i = st.pt;
-    st.T = A.T[i];
-    st.mu = A.mu[i];
-    st.xi = A.xi[i];
+    st.T = gmem(A.T)[i];
+    st.mu = gmem(A.mu)[i];
+    st.xi = gmem(A.xi)[i];
     st.flags = 0;
     st.steps = 0;
     st.passes = 0;
@@ -1221,7 +1231,7 @@ __device__ __noinline__ void begin_item(Scr s, St&, int gl)
     if (st.kind == K_SURR) {
         // the learned seed is read with the point's (T, mu, xi), in one round trip
 #pragma unroll
-        for (int k = 0; k < 5; k++) x0[k] = A.seed[p * 5 + k];
+        for (int k = 0; k < 5; k++) x0[k] = gmem(A.seed)[p * 5 + k];
     }
     reset_point(A, st);
     switch (st.kind) {
@@ -1284,7 +1294,7 @@ __device__ __noinline__ void item_done(Scr s, St&, int blk, int gl, bool conv, b
         case K_MPOL: dst = A.recM + (p * 6 + st.k) * REC_SIZE; break;
         default: dst = A.recM + (p * 3 + st.k) * REC_SIZE; break;   // K_MR3
         }
-        for (int f = 0; f < REC_SIZE; f++) dst[f] = s.rec(blk + f);
+        for (int f = 0; f < REC_SIZE; f++) gmem(dst)[f] = s.rec(blk + f);
         if (A.passes) atomicAdd(&A.passes[p], st.passes);
         if (A.steps) atomicAdd(&A.steps[p], st.steps);
         if (st.kind == K_PRIMARY && !conv && A.auto_ms && !exc) {
