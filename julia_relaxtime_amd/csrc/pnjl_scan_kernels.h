@@ -1295,8 +1295,9 @@ __device__ __noinline__ void item_done(Scr s, St&, int blk, int gl, bool conv, b
         default: dst = A.recM + (p * 3 + st.k) * REC_SIZE; break;   // K_MR3
         }
         for (int f = 0; f < REC_SIZE; f++) gmem(dst)[f] = s.rec(blk + f);
-        if (A.passes) atomicAdd(&A.passes[p], st.passes);
-        if (A.steps) atomicAdd(&A.steps[p], st.steps);
+        // atomicAdd's relaxed, device-scope add on the global address
+        if (A.passes) __hip_atomic_fetch_add(gmem(A.passes) + p, st.passes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (A.steps) __hip_atomic_fetch_add(gmem(A.steps) + p, st.steps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (st.kind == K_PRIMARY && !conv && A.auto_ms && !exc) {
             const unsigned long long j = atomicAdd(A.queue + Q_MCOUNT, 1ULL);
             A.mlist[j] = (int32_t)p;
