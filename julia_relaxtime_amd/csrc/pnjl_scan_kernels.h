@@ -944,24 +944,24 @@ __device__ __forceinline__ void write_point(const ScanArgs& A, const Scr& s, con
     if (gl != 0) return;
     int64_t i = st.pt;
 #pragma unroll
-    for (int k = 0; k < 5; k++) A.x_out[i * 5 + k] = s.rec(blk + RF_X + k);
-    A.omega[i] = s.rec(blk + RF_OMEGA);
-    A.pressure[i] = s.rec(blk + RF_P);
-    A.rho_norm[i] = s.rec(blk + RF_RHO);
-    A.entropy[i] = s.rec(blk + RF_S);
-    A.energy[i] = s.rec(blk + RF_E);
+    for (int k = 0; k < 5; k++) gmem(A.x_out)[i * 5 + k] = s.rec(blk + RF_X + k);
+    gmem(A.omega)[i] = s.rec(blk + RF_OMEGA);
+    gmem(A.pressure)[i] = s.rec(blk + RF_P);
+    gmem(A.rho_norm)[i] = s.rec(blk + RF_RHO);
+    gmem(A.entropy)[i] = s.rec(blk + RF_S);
+    gmem(A.energy)[i] = s.rec(blk + RF_E);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        A.masses[i * 3 + k] = s.rec(blk + RF_M + k);
-        if (A.n_q) A.n_q[i * 3 + k] = s.rec(blk + RF_NQ + k);
-        if (A.n_qbar) A.n_qbar[i * 3 + k] = s.rec(blk + RF_NQB + k);
+        gmem(A.masses)[i * 3 + k] = s.rec(blk + RF_M + k);
+        if (A.n_q) gmem(A.n_q)[i * 3 + k] = s.rec(blk + RF_NQ + k);
+        if (A.n_qbar) gmem(A.n_qbar)[i * 3 + k] = s.rec(blk + RF_NQB + k);
     }
-    A.resnorm[i] = s.rec(blk + RF_RES);
-    A.iterations[i] = (int32_t)s.rec(blk + RF_IT);
-    A.converged[i] = (int32_t)s.rec(blk + RF_CONV);
-    if (A.flags) A.flags[i] = (int32_t)s.rec(blk + RF_FLAGS) | st.flags;
-    if (A.steps) A.steps[i] = st.steps;
-    if (A.passes) A.passes[i] = st.passes;
+    gmem(A.resnorm)[i] = s.rec(blk + RF_RES);
+    gmem(A.iterations)[i] = (int32_t)s.rec(blk + RF_IT);
+    gmem(A.converged)[i] = (int32_t)s.rec(blk + RF_CONV);
+    if (A.flags) gmem(A.flags)[i] = (int32_t)s.rec(blk + RF_FLAGS) | st.flags;
+    if (A.steps) gmem(A.steps)[i] = st.steps;
+    if (A.passes) gmem(A.passes)[i] = st.passes;
 }
 
 // point finished with record `blk`: write outputs, update the chain tracker
