@@ -219,7 +219,28 @@ __shared__ ScanArgs cold_args;
 // The kernels' dynamic LDS: node tables, then the groups' storage.
 extern __shared__ double scan_lds[];
 
-struct St;
+// The group's solver state.  It lives in the group's LDS block (every lane
+// of the group reads and writes the same values), which keeps it out of the
+// registers the node loop needs.
+struct St {
+    int64_t item, item_end;   // current chunk of items (chains: of chains)
+    int64_t pt, pt_end;       // current point, end of its chain (exclusive)
+    int64_t trips;
+    int64_t idx;              // pipeline: list index of the item's point
+    double T, mu, xi;
+    double x[5];
+    double prev_mass;         // branch chain: M_u of the previous point
+    int phase, it, attempt, flags, steps, passes;
+    int prev_phase;
+    int kind, k;              // pipeline: item kind and seed index
+    bool fresh;               // next pass is the first evaluation of the phase
+    bool sym;                 // seed has phi_u == phi_d: keep them equal
+    bool have_best;
+    bool have_prev;
+    bool plan_multi;          // this point runs MultiSeed as its plan
+    bool judged;              // stage 1: the point is in the learned seed's reference-judge mask
+};
+static_assert(sizeof(St) <= 24 * sizeof(double), "St must fit its LDS slot");
 
 // a group's storage in LDS: hot state (at), candidate records (rec) and the
 // solver state (st), addressed from the LDS array itself (an offset, not a
@@ -230,7 +251,7 @@ struct Scr {
     __device__ __forceinline__ double& at(int f) const { return scan_lds[off + f]; }
     __device__ __forceinline__ double& rec(int f) const { return scan_lds[off + SB_REC + f]; }
     __device__ __forceinline__ double* ptr(int f) const { return scan_lds + off + f; }
-    __device__ __forceinline__ St& st() const;
+    __device__ __forceinline__ St& st() const { return *reinterpret_cast<St*>(scan_lds + off + SB_ST); }
 };
 
 // p as a global-memory pointer.  Out-of-line functions read the launch
@@ -512,9 +533,11 @@ __device__ __forceinline__ void assemble_core(const DevParams& P, const Scr& s, 
 // Candidate record at x: thermodynamics (calculate_thermo + densities, in
 // closed form from the sums) and the physicality gate
 // (_default_is_physical_solution + _all_finite_thermo, ImplicitSolver.jl:50-64).
-__device__ __noinline__ void put_rec(Scr s, int blk, const double* x, double mu, double T, bool sym,
+// x is the group's current state st.x, read here from LDS.
+__device__ __noinline__ void put_rec(Scr s, int blk, double mu, double T, bool sym,
                         double phi_tol, double omega, double res, int it, bool fconv, bool xconv)
 {
+    const double* x = s.st().x;
     const DevParams& P = cold_args.P;
     double M[3];
     mass_vec(P, x, M);
@@ -886,29 +909,6 @@ enum Phase { PH_IDLE = 0, PH_NEWTON = 1, PH_TR = 2 };
 constexpr int ATT_SURR = -2;
 constexpr int ATT_PRIMARY = -1;  // 0..5: MultiSeed candidates
 
-// The group's solver state.  It lives in the group's LDS block (every lane
-// of the group reads and writes the same values), which keeps it out of the
-// registers the node loop needs.
-struct St {
-    int64_t item, item_end;   // current chunk of items (chains: of chains)
-    int64_t pt, pt_end;       // current point, end of its chain (exclusive)
-    int64_t trips;
-    int64_t idx;              // pipeline: list index of the item's point
-    double T, mu, xi;
-    double x[5];
-    double prev_mass;         // branch chain: M_u of the previous point
-    int phase, it, attempt, flags, steps, passes;
-    int prev_phase;
-    int kind, k;              // pipeline: item kind and seed index
-    bool fresh;               // next pass is the first evaluation of the phase
-    bool sym;                 // seed has phi_u == phi_d: keep them equal
-    bool have_best;
-    bool have_prev;
-    bool plan_multi;          // this point runs MultiSeed as its plan
-    bool judged;              // stage 1: the point is in the learned seed's reference-judge mask
-};
-static_assert(sizeof(St) <= 24 * sizeof(double), "St must fit its LDS slot");
-__device__ __forceinline__ St& Scr::st() const { return *reinterpret_cast<St*>(scan_lds + off + SB_ST); }
 static_assert(SB_ST + 24 <= SB_FJ && SB_REC + REC_AREA <= STATE_STRIDE && STATE_STRIDE % 2 == 1, "state layout");
 
 __host__ __device__ inline bool branch_policy(int p)
@@ -1406,7 +1406,7 @@ __device__ __noinline__ bool tr_step(Scr s, St&, double omega)
     if (st.fresh) {
         st.fresh = false;
         bool fconv = norm_inf5(F) < A.ftol;
-        put_rec(s, SB_TRC, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(F), 0, fconv, false);
+        put_rec(s, SB_TRC, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(F), 0, fconv, false);
 #pragma unroll
         for (int i = 0; i < 5; i++) r[i] = F[i];
 #pragma unroll
@@ -1471,7 +1471,7 @@ __device__ __noinline__ bool tr_step(Scr s, St&, double omega)
             bool xconv = !xnan && cheb < A.xtol;
             bool fconv = norm_inf5(r) < A.ftol;
             converged = xconv || fconv;
-            put_rec(s, SB_TRC, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(r), st.it, fconv, xconv);
+            put_rec(s, SB_TRC, st.mu, st.T, st.sym, A.phi_tol, omega, norm_inf5(r), st.it, fconv, xconv);
         } else {
 #pragma unroll
             for (int i = 0; i < 5; i++) st.x[i] = s.at(SB_XOLD + i);
@@ -1554,7 +1554,7 @@ __device__ __forceinline__ void control_step(const ScanArgs& A, const Scr& s, St
         }
         // Newton finished: candidate at x
         double res = norm_inf5(F);
-        put_rec(s, SB_PRIM, st.x, st.mu, st.T, st.sym, A.phi_tol, omega, res, st.it, fconv, xconv);
+        put_rec(s, SB_PRIM, st.mu, st.T, st.sym, A.phi_tol, omega, res, st.it, fconv, xconv);
         bool good = rec_good(s, SB_PRIM, A.resmax);
         if (A.tr_fb && !good && st.attempt != ATT_SURR) {
             st.phase = PH_TR;
