@@ -42,7 +42,11 @@
 // flags) sit in LDS behind its hot state: an attempt's end reads back the
 // record it has just written (the convergence gate, the candidate choice,
 // the copy to the item's output slot), and with the records in HBM each of
-// those reads was a round trip the whole wavefront waited for.
+// those reads was a round trip the whole wavefront waited for.  The storage
+// is handed to functions as an offset into the kernels' LDS array (Scr), so
+// that out-of-line functions address it as LDS, and they cast the launch
+// arguments' pointers to global ones (gmem) where they touch HBM on every
+// item: generic pointers would make both flat accesses.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
