@@ -65,6 +65,12 @@ __device__ __forceinline__ Tabs make_tabs(const double* t)
     return q;
 }
 
+// The pole-search kernels' quadrature tables.  Their B0 routine is out of
+// line; it rebuilds its table pointers from this array rather than taking
+// them from the caller, so that it reads the tables with LDS instructions
+// (a pointer passed in would be a generic address: flat loads).
+__shared__ double mass_tab[kTabLen];
+
 struct Phys {
     double Lambda, Nc;
 };
@@ -1205,8 +1211,9 @@ __device__ __forceinline__ int mass_pairs(int m, int* pairs)
 // B0 (symmetrised for u sbar, + correction) of the meson's pairs at k0
 //   MesonMass.jl:34-60, :222-231 (MesonMass passes num_s 0 for s sbar, equivalent to 2)
 template <int L = 1>
-__device__ __noinline__ void mass_b0(const Phys& P, const Tabs& T, const pnjl_mass_query& a, double k0, MassB0& B)
+__device__ __noinline__ void mass_b0(const Phys& P, const Tabs&, const pnjl_mass_query& a, double k0, MassB0& B)
 {
+    const Tabs T = make_tabs(mass_tab);
     int pairs[2];
     const int np = mass_pairs(a.meson, pairs);
 #pragma unroll 1
@@ -1362,7 +1369,7 @@ __global__ void __launch_bounds__(256, 3) mass_kernel(Phys P, const double* tab,
                                                    const int32_t* in_list, const unsigned* in_count,
                                                    int32_t* out_list, unsigned* out_count, int budget)
 {
-    __shared__ double lds[kTabLen];
+    double* const lds = mass_tab;
     load_tabs(lds, tab);
     const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ts % stride) return;
@@ -1494,7 +1501,7 @@ __global__ void __launch_bounds__(256, 2) mass_tail_kernel(Phys P, const double*
 {
     // blocks past the list leave before the table load (uniform per block)
     if ((int64_t)blockIdx.x * (blockDim.x / kMassLanes) >= (int64_t)*in_count) return;
-    __shared__ double lds[kTabLen];
+    double* const lds = mass_tab;
     load_tabs(lds, tab);
     const int64_t ts = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t slot = ts / kMassLanes;
