@@ -997,20 +997,20 @@ __device__ __forceinline__ void finish_point(const ScanArgs& A, const Scr& s, St
     if (branch && !ok) {
         for (int64_t i = st.pt + 1 + gl; i < st.pt_end; i += LPP) {
 #pragma unroll
-            for (int k = 0; k < 5; k++) A.x_out[i * 5 + k] = qnan();
-            A.omega[i] = A.pressure[i] = A.rho_norm[i] = A.entropy[i] = A.energy[i] = qnan();
+            for (int k = 0; k < 5; k++) gmem(A.x_out)[i * 5 + k] = qnan();
+            gmem(A.omega)[i] = gmem(A.pressure)[i] = gmem(A.rho_norm)[i] = gmem(A.entropy)[i] = gmem(A.energy)[i] = qnan();
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                A.masses[i * 3 + k] = qnan();
-                if (A.n_q) A.n_q[i * 3 + k] = qnan();
-                if (A.n_qbar) A.n_qbar[i * 3 + k] = qnan();
+                gmem(A.masses)[i * 3 + k] = qnan();
+                if (A.n_q) gmem(A.n_q)[i * 3 + k] = qnan();
+                if (A.n_qbar) gmem(A.n_qbar)[i * 3 + k] = qnan();
             }
-            A.resnorm[i] = qnan();
-            A.iterations[i] = 0;
-            A.converged[i] = 0;
-            if (A.flags) A.flags[i] = PNJL_PT_SKIPPED;
-            if (A.steps) A.steps[i] = 0;
-            if (A.passes) A.passes[i] = 0;
+            gmem(A.resnorm)[i] = qnan();
+            gmem(A.iterations)[i] = 0;
+            gmem(A.converged)[i] = 0;
+            if (A.flags) gmem(A.flags)[i] = PNJL_PT_SKIPPED;
+            if (A.steps) gmem(A.steps)[i] = 0;
+            if (A.passes) gmem(A.passes)[i] = 0;
         }
         st.pt = st.pt_end - 1;      // next_chain_point moves on to the next chain
     }
